@@ -1,0 +1,25 @@
+"""MI355X (gfx950) pairwise-alignment engine: banded affine Smith-Waterman and
+gap-affine WFA over 2-bit packed SequenceBatch buffers.
+
+Host-side mirror of the reference's alignment interface (Computational-
+Genomics-Mojo: genomics/core/sequence.mojo, genomics/cpu/align_cpu.mojo,
+genomics/gpu/device.mojo, genomics/gpu/align_gpu.mojo) over the C ABI of
+libcgm_align.so (include/cgm_align.h).  The package directory name is not a
+Python identifier; load it with importlib (see tests/conftest.py).
+"""
+from . import _lib
+from .align import (AlignConfig, AlignResult, SwResult, WFAConfig, default_config,
+                    default_wfa_config, launch_banded_sw, launch_wfa_affine,
+                    smith_waterman_banded, sw_banded_batch, wfa_affine, wfa_affine_batch,
+                    wfa_affine_cpu)
+from .device import DeviceArray, GenomicsDevice, Timer
+from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_to_2bit,
+                       get_base_from_word, get_view, pack_codes)
+
+__all__ = [
+    "AlignConfig", "AlignResult", "SwResult", "WFAConfig", "default_config",
+    "default_wfa_config", "launch_banded_sw", "launch_wfa_affine", "smith_waterman_banded",
+    "sw_banded_batch", "wfa_affine", "wfa_affine_batch", "wfa_affine_cpu", "DeviceArray",
+    "GenomicsDevice", "Timer", "BASES_PER_WORD", "Base", "SequenceBatch", "SequenceView",
+    "ascii_to_2bit", "get_base_from_word", "get_view", "pack_codes",
+]

@@ -1,0 +1,187 @@
+"""Alignment API: the reference's CPU signatures and GPU launchers, run on
+the gfx950 kernels of libcgm_align.so.
+
+Reference interfaces mirrored (Computational-Genomics-Mojo):
+  AlignConfig / AlignResult / default_config   genomics/cpu/align_cpu.mojo:18-53
+  WFAConfig / default_wfa_config               genomics/cpu/align_cpu.mojo:62-72
+  smith_waterman_banded(query, reference, cfg) genomics/cpu/align_cpu.mojo:229-309
+  wfa_affine_cpu(query, ref_seq, cfg)          genomics/cpu/align_cpu.mojo:75-226
+  launch_banded_sw(device, ...)                genomics/gpu/align_gpu.mojo:460-493
+  launch_wfa_affine(device, ...)               genomics/gpu/align_gpu.mojo:496-535
+Results are bit-identical to align_cpu.mojo (score, query_end, ref_end for
+SW; cost or -1 for WFA); query_start/ref_start are 0 and the CIGAR is empty,
+as in the reference.  Every function runs on the GPU and raises when the
+library or the device is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, GenomicsDevice
+from .sequence import SequenceBatch, SequenceView
+
+
+@dataclass
+class AlignConfig:
+    """align_cpu.mojo:18-25 -- penalties are negative values."""
+    match_score: int
+    mismatch: int
+    gap_open: int
+    gap_extend: int
+    band_width: int
+
+
+@dataclass
+class AlignResult:
+    """align_cpu.mojo:28-36"""
+    score: int
+    query_start: int
+    ref_start: int
+    query_end: int
+    ref_end: int
+    cigar: list = field(default_factory=list)
+
+
+@dataclass
+class WFAConfig:
+    """align_cpu.mojo:62-68 -- positive costs."""
+    x: int
+    o: int
+    e: int
+    max_error: int
+
+
+def default_config() -> AlignConfig:
+    """align_cpu.mojo:46-53"""
+    return AlignConfig(match_score=2, mismatch=-3, gap_open=-5, gap_extend=-2, band_width=32)
+
+
+def default_wfa_config() -> WFAConfig:
+    """align_cpu.mojo:71-72"""
+    return WFAConfig(x=4, o=6, e=2, max_error=200)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _nonempty(a):
+    return a if a.size else np.zeros(1, dtype=a.dtype)
+
+
+def sw_banded_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignConfig,
+                    band=None, layout=_lib.LAYOUT_AUTO):
+    """Aligns pair i = (query i, reference i) for every i; host arrays in,
+    host arrays (score, query_end, ref_end) out.  band: per-pair array or
+    None for cfg.band_width."""
+    q_packed = np.ascontiguousarray(q_packed, np.uint64)
+    r_packed = np.ascontiguousarray(r_packed, np.uint64)
+    ql, rl, qo, ro, bd = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(band)
+    n = int(ql.size)
+    score = np.zeros(n, np.int32)
+    qe = np.zeros(n, np.int32)
+    re_ = np.zeros(n, np.int32)
+    if n:
+        _lib.call("cgm_sw_banded_host", _lib.ptr(_nonempty(q_packed)), _lib.ptr(_nonempty(r_packed)),
+                  _lib.ptr(ql), _lib.ptr(rl), _lib.ptr(qo), _lib.ptr(ro), n, q_packed.size,
+                  r_packed.size, _lib.ptr(bd), cfg.band_width, cfg.match_score, cfg.mismatch,
+                  cfg.gap_open, cfg.gap_extend, _lib.ptr(score), _lib.ptr(qe), _lib.ptr(re_),
+                  layout)
+    return score, qe, re_
+
+
+def wfa_affine_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: WFAConfig,
+                     max_error=None, layout=_lib.LAYOUT_AUTO):
+    """Gap-affine WFA cost (or -1) of every pair; host arrays in and out."""
+    q_packed = np.ascontiguousarray(q_packed, np.uint64)
+    r_packed = np.ascontiguousarray(r_packed, np.uint64)
+    ql, rl, qo, ro, me = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(max_error)
+    n = int(ql.size)
+    cost = np.zeros(n, np.int32)
+    if n:
+        _lib.call("cgm_wfa_affine_host", _lib.ptr(_nonempty(q_packed)), _lib.ptr(_nonempty(r_packed)),
+                  _lib.ptr(ql), _lib.ptr(rl), _lib.ptr(qo), _lib.ptr(ro), n, q_packed.size,
+                  r_packed.size, _lib.ptr(me), cfg.max_error, cfg.x, cfg.o, cfg.e,
+                  _lib.ptr(cost), layout)
+    return cost
+
+
+def _view_arrays(v: SequenceView):
+    return np.ascontiguousarray(v.packed, np.uint64), np.zeros(1, np.int32), \
+        np.array([v.length], np.int32)
+
+
+def smith_waterman_banded(query: SequenceView, reference: SequenceView,
+                          cfg: AlignConfig) -> AlignResult:
+    """align_cpu.mojo:229-309, computed by the GPU kernel."""
+    qp, qo, ql = _view_arrays(query)
+    rp, ro, rl = _view_arrays(reference)
+    s, qe, re_ = sw_banded_batch(qp, qo, ql, rp, ro, rl, cfg)
+    return AlignResult(int(s[0]), 0, 0, int(qe[0]), int(re_[0]), [])
+
+
+def wfa_affine_cpu(query: SequenceView, ref_seq: SequenceView, cfg: WFAConfig) -> int:
+    """align_cpu.mojo:75-226 (same name and signature), computed by the GPU
+    kernel."""
+    qp, qo, ql = _view_arrays(query)
+    rp, ro, rl = _view_arrays(ref_seq)
+    return int(wfa_affine_batch(qp, qo, ql, rp, ro, rl, cfg)[0])
+
+
+wfa_affine = wfa_affine_cpu
+
+
+class SwResult:
+    """Device outputs of launch_banded_sw: scores (the reference's return
+    buffer) plus the end coordinates the CPU result carries."""
+
+    def __init__(self, n: int):
+        self.scores = DeviceArray(max(n, 1), np.int32)
+        self.query_end = DeviceArray(max(n, 1), np.int32)
+        self.ref_end = DeviceArray(max(n, 1), np.int32)
+
+
+def launch_banded_sw(device: GenomicsDevice, queries_buf: DeviceArray, refs_buf: DeviceArray,
+                     q_lengths_buf: DeviceArray, r_lengths_buf: DeviceArray,
+                     q_offsets_buf: DeviceArray, r_offsets_buf: DeviceArray, n_pairs: int,
+                     total_words: int, band: int = 64, cfg: AlignConfig | None = None,
+                     band_buf: DeviceArray | None = None, r_total_words: int | None = None,
+                     layout: int = _lib.LAYOUT_AUTO) -> SwResult:
+    """align_gpu.mojo:460-493.  Asynchronous on device.stream; synchronise
+    (device.synchronize()) before reading the result.  total_words bounds the
+    query buffer and, unless r_total_words is given, the reference buffer."""
+    cfg = cfg or default_config()
+    out = SwResult(n_pairs)
+    rw = total_words if r_total_words is None else r_total_words
+    _lib.call("cgm_sw_banded_launch", queries_buf.dptr, refs_buf.dptr, q_lengths_buf.dptr,
+              r_lengths_buf.dptr, q_offsets_buf.dptr, r_offsets_buf.dptr, n_pairs, total_words,
+              rw, band_buf.dptr if band_buf is not None else None, band, cfg.match_score,
+              cfg.mismatch, cfg.gap_open, cfg.gap_extend, out.scores.dptr, out.query_end.dptr,
+              out.ref_end.dptr, layout, device.stream)
+    return out
+
+
+def launch_wfa_affine(device: GenomicsDevice, queries_buf: DeviceArray, refs_buf: DeviceArray,
+                      q_lengths_buf: DeviceArray, r_lengths_buf: DeviceArray,
+                      q_offsets_buf: DeviceArray, r_offsets_buf: DeviceArray, n_pairs: int,
+                      total_words: int, max_error: int = 200, x: int = 4, o: int = 6, e: int = 2,
+                      max_error_buf: DeviceArray | None = None, r_total_words: int | None = None,
+                      layout: int = _lib.LAYOUT_AUTO) -> DeviceArray:
+    """align_gpu.mojo:496-535 -- returns the device buffer of costs (-1 when
+    the cost exceeds max_error).  Asynchronous on device.stream."""
+    scores = DeviceArray(max(n_pairs, 1), np.int32)
+    rw = total_words if r_total_words is None else r_total_words
+    _lib.call("cgm_wfa_affine_launch", queries_buf.dptr, refs_buf.dptr, q_lengths_buf.dptr,
+              r_lengths_buf.dptr, q_offsets_buf.dptr, r_offsets_buf.dptr, n_pairs, total_words,
+              rw, max_error_buf.dptr if max_error_buf is not None else None, max_error, x, o, e,
+              scores.dptr, layout, device.stream)
+    return scores
+
+
+def batch_arrays(queries: SequenceBatch, refs: SequenceBatch):
+    """(q_packed, q_off, q_len, r_packed, r_off, r_len) of two batches."""
+    return (queries.packed, queries.offsets, queries.lengths, refs.packed, refs.offsets,
+            refs.lengths)

@@ -1,0 +1,108 @@
+// cgm_internal.h -- shared declarations of libcgm_align.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/cgm_align.h"
+
+namespace cgm {
+
+// ---------------------------------------------------------------- errors
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *what);
+#define CGM_HIP(call)                                        \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) return ::cgm::hip_error(_e, #call); \
+    } while (0)
+
+// Limits enforced by every entry point (documented in cgm_align.h).
+constexpr int32_t kMaxScoreMagnitude = 1 << 20;  // |match|,|mismatch|,|open|,|extend|
+constexpr int32_t kMaxBand = 1 << 15;            // SW band upper limit
+constexpr int32_t kMaxWfaError = 1 << 20;        // WFA max_error upper limit
+constexpr int64_t kMaxScoreProduct = int64_t(1) << 30;  // match * min(qlen, rlen)
+
+// Pair batch in device memory.  pair i aligns query i with reference i.
+struct Batch {
+    const uint64_t *q_packed;
+    const uint64_t *r_packed;
+    const int32_t *q_len;
+    const int32_t *r_len;
+    const int32_t *q_off;
+    const int32_t *r_off;
+    int64_t n_pairs;
+    int64_t q_words;
+    int64_t r_words;
+};
+
+struct SwScoring {
+    int32_t match, mismatch, gap_open, gap_extend;
+};
+
+struct SwOut {
+    int32_t *score;
+    int32_t *q_end;
+    int32_t *r_end;
+};
+
+struct WfaPen {
+    int32_t x, o, e;
+};
+
+// ------------------------------------------------------------ device code
+#ifdef __HIPCC__
+// Base `pos` of a packed sequence (sequence.mojo:96-101, dna.mojo:97-100).
+__device__ __forceinline__ int base_at(const uint64_t *w, int pos) {
+    return (int)((w[pos >> 5] >> (62 - 2 * (pos & 31))) & 3ull);
+}
+
+// Loads and bounds-checks pair i.  Returns false when the pair's lengths or
+// offsets fall outside the packed buffers; nothing may then be read.
+__device__ __forceinline__ bool load_pair(const Batch &b, int64_t i, const uint64_t *&q,
+                                          const uint64_t *&r, int &qlen, int &rlen) {
+    int ql = b.q_len[i], rl = b.r_len[i], qo = b.q_off[i], ro = b.r_off[i];
+    if (ql < 0 || rl < 0 || qo < 0 || ro < 0) return false;
+    int64_t qw = ((int64_t)ql + 31) >> 5, rw = ((int64_t)rl + 31) >> 5;
+    if ((int64_t)qo + qw > b.q_words || (int64_t)ro + rw > b.r_words) return false;
+    q = b.q_packed + qo;
+    r = b.r_packed + ro;
+    qlen = ql;
+    rlen = rl;
+    return true;
+}
+#endif
+
+// ------------------------------------------------------------- launchers
+// All launchers validate scalar arguments, return a CGM_* status and run
+// asynchronously on `stream`.
+int launch_sw_plain(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                    SwOut out, hipStream_t stream);
+int launch_sw_lane(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                   SwOut out, hipStream_t stream);
+int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                     WfaPen p, int32_t *out, hipStream_t stream);
+
+// Device-side max of an int32 array (blocking; used by launchers that size
+// LDS or scratch from a per-pair device array).
+int device_max_i32(const int32_t *a, int64_t n, hipStream_t stream, int32_t *result);
+// Max over valid pairs of the effective band min(band, max(qlen, rlen))
+// (blocking).  A band wider than both sequences adds no cell and changes no
+// boundary value, so every SW layout aligns with the effective band.
+int device_max_band_eff(const Batch &b, const int32_t *band, int32_t band_scalar,
+                        hipStream_t stream, int32_t *result);
+
+#ifdef __HIPCC__
+__device__ __forceinline__ int effective_band(int band, int qlen, int rlen) {
+    const int m = qlen > rlen ? qlen : rlen;
+    return band < m ? band : m;
+}
+#endif
+
+int check_sw_scoring(const SwScoring &sc);
+int check_wfa_pen(const WfaPen &p);
+int check_batch(const Batch &b);
+
+}  // namespace cgm

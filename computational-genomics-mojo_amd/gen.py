@@ -1,0 +1,117 @@
+"""Seeded synthetic pair batches (cgm_gen_* in libcgm_align.so).
+
+Configs (sizes of the five benchmark workloads):
+  1  plain SW: 150 bp targets, 2% subs, 0.5% indels of 1-3 bp, band 32
+  2  batched SW: targets 100-300 bp, divergence 1-5% (sub:indel 3:1,
+     indel length geometric p=0.5), band 16-64 per pair
+  3  WFA: targets log-uniform 1-10 kb, divergence 0.5-5% (geometric p=0.3)
+  4  mixed: 50% SW 100-300 bp band 32, 40% SW 1-2 kb band 128,
+     10% WFA 5-20 kb at 1-10% divergence
+  5  serving pair mix: length log-normal (median 250, sigma 1) clipped to
+     100-20000 bp, SW band 64 below 2 kb, WFA above
+All configs put 0.1% N positions into the query (packed as A + NMask bit).
+WFA pairs carry max_error = planted edit cost + 4 * #N + 8.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+SW_SCORING = dict(match=2, mismatch=-4, gap_open=-4, gap_extend=-2)
+WFA_PENALTIES = dict(x=4, o=6, e=2)
+DEFAULT_PAIRS = {1: 1000, 2: 200_000, 3: 20_000, 4: 100_000, 5: 200_000}
+DEFAULT_SEED = 20260402
+
+
+@dataclass
+class Workload:
+    config: int
+    seed: int
+    q_packed: np.ndarray
+    q_nmask: np.ndarray
+    q_off: np.ndarray
+    q_len: np.ndarray
+    r_packed: np.ndarray
+    r_nmask: np.ndarray
+    r_off: np.ndarray
+    r_len: np.ndarray
+    kind: np.ndarray
+    band: np.ndarray
+    max_error: np.ndarray
+
+    @property
+    def n_pairs(self) -> int:
+        return int(self.q_len.size)
+
+    def subset(self, idx) -> "Workload":
+        """A new workload holding pairs idx (re-packed contiguously)."""
+        idx = np.asarray(idx, dtype=np.int64)
+
+        def take(packed, nmask, off, ln):
+            nw = (ln[idx].astype(np.int64) + 31) // 32
+            new_off = np.zeros(idx.size, np.int64)
+            if idx.size:
+                new_off[1:] = np.cumsum(nw)[:-1]
+            total = int(nw.sum())
+            p = np.zeros(total, np.uint64)
+            m = np.zeros(total, np.uint64)
+            for k, i in enumerate(idx):
+                a, c = int(off[i]), int(nw[k])
+                p[new_off[k]:new_off[k] + c] = packed[a:a + c]
+                m[new_off[k]:new_off[k] + c] = nmask[a:a + c]
+            return p, m, new_off.astype(np.int32), ln[idx].astype(np.int32)
+
+        qp, qn, qo, ql = take(self.q_packed, self.q_nmask, self.q_off, self.q_len)
+        rp, rn, ro, rl = take(self.r_packed, self.r_nmask, self.r_off, self.r_len)
+        return Workload(self.config, self.seed, qp, qn, qo, ql, rp, rn, ro, rl,
+                        self.kind[idx].copy(), self.band[idx].copy(), self.max_error[idx].copy())
+
+    def sw_cells(self) -> np.ndarray:
+        """Valid band cells per pair: sum over rows of the in-matrix band
+        width (the cells align_cpu.mojo:265-268 does not skip)."""
+        return valid_band_cells(self.q_len, self.r_len, self.band)
+
+
+def valid_band_cells(q_len, r_len, band) -> np.ndarray:
+    """sum_{qi<qlen} max(0, min(rlen-1, qi+band) - max(0, qi-band) + 1) per
+    pair, in closed form (vectorised)."""
+    n = np.asarray(q_len, np.int64)
+    r = np.asarray(r_len, np.int64)
+    b = np.broadcast_to(np.asarray(band, np.int64), n.shape).astype(np.int64)
+    c = r - 1
+    # sum_{qi<n} min(c, qi + b) + 1
+    k = np.clip(c - b + 1, 0, n)
+    s_min = k * (k - 1) // 2 + k * b + (n - k) * c + n
+    # sum_{qi<n} max(0, qi - b)
+    m = np.maximum(0, n - b - 1)
+    s_max = m * (m + 1) // 2
+    # rows with qi >= rlen + band contribute negative widths above: add back
+    neg = np.maximum(0, n - (r + b))
+    total = s_min - s_max + neg * (neg - 1) // 2
+    return np.where((n > 0) & (r > 0), total, 0)
+
+
+def generate(config: int, n_pairs: int | None = None, seed: int = DEFAULT_SEED) -> Workload:
+    n = DEFAULT_PAIRS[config] if n_pairs is None else int(n_pairs)
+    h = ctypes.c_void_p()
+    _lib.call("cgm_gen_create", config, n, seed, ctypes.byref(h))
+    try:
+        npairs, qw, rw = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("cgm_gen_sizes", h, ctypes.byref(npairs), ctypes.byref(qw), ctypes.byref(rw))
+        qp = np.zeros(qw.value, np.uint64)
+        qn = np.zeros(qw.value, np.uint64)
+        rp = np.zeros(rw.value, np.uint64)
+        rn = np.zeros(rw.value, np.uint64)
+        arrs = [np.zeros(n, np.int32) for _ in range(7)]
+        qo, ql, ro, rl, kind, band, me = arrs
+        P = _lib.ptr
+        _lib.call("cgm_gen_copy", h, P(qp) if qp.size else None, P(qn) if qn.size else None,
+                  P(qo), P(ql), P(rp) if rp.size else None, P(rn) if rn.size else None, P(ro),
+                  P(rl), P(kind), P(band), P(me))
+    finally:
+        _lib.load().cgm_gen_destroy(h)
+    return Workload(config, seed, qp, qn, qo, ql, rp, rn, ro, rl, kind, band, me)

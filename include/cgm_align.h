@@ -1,0 +1,168 @@
+/*
+ * cgm_align.h -- C ABI of the MI355X (gfx950) pairwise-alignment library
+ * (libcgm_align.so): banded affine Smith-Waterman and gap-affine WFA over
+ * 2-bit packed SequenceBatch structure-of-arrays buffers.
+ *
+ * Every entry point is plain C: pointers, sizes and int status codes, no
+ * C++ or torch types, so a Mojo `external_call`, Python ctypes or a C caller
+ * can bind it directly.  Citations name the reference (Computational-
+ * Genomics-Mojo) interface each entry point replaces.
+ *
+ * Batch layout (genomics/core/sequence.mojo:23-75 and the launcher arguments
+ * of genomics/gpu/align_gpu.mojo:460-470):
+ *   q_packed / r_packed  uint64 words, 32 bases per word, base i of a word at
+ *                        bits (63-2i)..(62-2i) (MSB first), A=0 C=1 G=2 T=3.
+ *                        N and other non-ACGT bytes are stored as A; their
+ *                        NMask is not consulted when scoring, exactly as the
+ *                        reference does (sequence.mojo:96-101).
+ *   q_off / r_off        int32 word offset of each sequence in its buffer.
+ *   q_len / r_len        int32 length in bases.
+ *   pair i aligns query i against reference i; q_words / r_words are the
+ *   lengths of the two packed buffers (the reference's `total_words`), used
+ *   for bounds checks.
+ *
+ * Outputs are caller-allocated device arrays of n_pairs int32.  A pair whose
+ * lengths/offsets fall outside its buffer (or whose band / max_error is
+ * negative or above the supported maximum) is not aligned: it gets
+ * score = CGM_BAD_PAIR and ends = -1, and nothing is read for it.
+ *
+ * Status: 0 = CGM_OK; otherwise an error code, with a message available from
+ * cgm_last_error() (thread-local).  Launch entry points are asynchronous on
+ * the given stream (NULL = the default stream) and thread-safe per stream.
+ */
+#ifndef CGM_ALIGN_H
+#define CGM_ALIGN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGM_OK 0
+#define CGM_ERR_INVALID 1   /* bad argument */
+#define CGM_ERR_HIP 2       /* HIP runtime error */
+#define CGM_ERR_NOMEM 3     /* allocation or budget refused */
+#define CGM_ERR_BUSY 4      /* queue full (serving engine back-pressure) */
+#define CGM_ERR_NODEV 5     /* no GPU visible */
+#define CGM_ERR_NOTFOUND 6  /* unknown request id */
+
+#define CGM_BAD_PAIR ((int32_t)0x80000000)
+
+/* Kernel layouts for the launch entry points. */
+#define CGM_LAYOUT_AUTO 0   /* pick by the autotuned length/band table */
+#define CGM_LAYOUT_PLAIN 1  /* one workgroup per pair (baseline) */
+#define CGM_LAYOUT_LANE 2   /* SW: one pair per lane (short pairs) */
+#define CGM_LAYOUT_WAVE 3   /* one 64-lane wavefront per pair */
+
+/* ------------------------------------------------------------ errors, info */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *cgm_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int cgm_version(void);
+
+/* ---------------------------------------------------------------- device
+ * Replace GenomicsDevice (genomics/gpu/device.mojo:21-138): device context,
+ * typed allocation, host<->device copies, synchronisation. */
+int cgm_device_count(int *count);
+int cgm_set_device(int device);
+/* name: at least name_len bytes; any output pointer may be NULL. */
+int cgm_device_info(int device, char *name, int name_len, int *compute_units,
+                    int *max_clock_khz, int64_t *hbm_bytes, int *lds_bytes_per_cu);
+int cgm_malloc(void **dptr, int64_t bytes);
+int cgm_free(void *dptr);
+/* Pinned host memory (for asynchronous copies). */
+int cgm_host_malloc(void **hptr, int64_t bytes);
+int cgm_host_free(void *hptr);
+int cgm_memcpy_htod(void *dst, const void *src, int64_t bytes, void *stream);
+int cgm_memcpy_dtoh(void *dst, const void *src, int64_t bytes, void *stream);
+int cgm_stream_create(void **stream);
+int cgm_stream_destroy(void *stream);
+int cgm_stream_synchronize(void *stream);
+int cgm_device_synchronize(void);
+int cgm_event_create(void **event);
+int cgm_event_destroy(void *event);
+int cgm_event_record(void *event, void *stream);
+int cgm_event_synchronize(void *event);
+int cgm_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* -------------------------------------------------------- banded SW batch
+ * Replaces launch_banded_sw[band] (genomics/gpu/align_gpu.mojo:460-493),
+ * with the semantics of smith_waterman_banded (genomics/cpu/align_cpu.mojo:
+ * 229-309) per pair: local alignment restricted to |ri - qi| <= band,
+ * out_score = best H, out_q_end/out_r_end = first cell reaching it in
+ * row-major order (query_start = ref_start = 0 and the CIGAR is empty, as in
+ * the reference).  band: per-pair device array, or NULL to use band_scalar
+ * for every pair.  Scoring: match > 0 expected, penalties <= 0 (AlignConfig,
+ * align_cpu.mojo:18-25); |value| <= 2^20.  The reference's band is a
+ * compile-time parameter; here it is a runtime value per pair. */
+int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed,
+                         const int32_t *q_len, const int32_t *r_len,
+                         const int32_t *q_off, const int32_t *r_off,
+                         int64_t n_pairs, int64_t q_words, int64_t r_words,
+                         const int32_t *band, int32_t band_scalar,
+                         int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                         int32_t *out_score, int32_t *out_q_end, int32_t *out_r_end,
+                         int32_t layout, void *stream);
+
+/* ------------------------------------------------------ gap-affine WFA batch
+ * Replaces launch_wfa_affine[max_error, x, o, e] (genomics/gpu/align_gpu.mojo:
+ * 496-535) with the semantics of wfa_affine_cpu (align_cpu.mojo:75-226):
+ * out_cost = minimum global cost (mismatch x, gap of length l = o + l*e,
+ * match 0) or -1 when it exceeds max_error.  max_error: per-pair device
+ * array or NULL for max_error_scalar.  Requires x >= 1, e >= 1, o >= 0 (the
+ * reference GPU kernel asserts e >= 1, align_gpu.mojo:288). */
+int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
+                          const int32_t *q_len, const int32_t *r_len,
+                          const int32_t *q_off, const int32_t *r_off,
+                          int64_t n_pairs, int64_t q_words, int64_t r_words,
+                          const int32_t *max_error, int32_t max_error_scalar,
+                          int32_t x, int32_t o, int32_t e,
+                          int32_t *out_cost, int32_t layout, void *stream);
+
+/* -------------------------------------------- host-pointer conveniences
+ * Same as the launch entry points but every pointer is a host pointer; the
+ * call copies in, runs on an internal stream and copies out before it
+ * returns.  These back the per-pair CPU signatures smith_waterman_banded /
+ * wfa_affine_cpu (align_cpu.mojo:229-233, :75-79) in the Mojo and Python
+ * mirrors. */
+int cgm_sw_banded_host(const uint64_t *q_packed, const uint64_t *r_packed,
+                       const int32_t *q_len, const int32_t *r_len,
+                       const int32_t *q_off, const int32_t *r_off,
+                       int64_t n_pairs, int64_t q_words, int64_t r_words,
+                       const int32_t *band, int32_t band_scalar,
+                       int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                       int32_t *out_score, int32_t *out_q_end, int32_t *out_r_end,
+                       int32_t layout);
+int cgm_wfa_affine_host(const uint64_t *q_packed, const uint64_t *r_packed,
+                        const int32_t *q_len, const int32_t *r_len,
+                        const int32_t *q_off, const int32_t *r_off,
+                        int64_t n_pairs, int64_t q_words, int64_t r_words,
+                        const int32_t *max_error, int32_t max_error_scalar,
+                        int32_t x, int32_t o, int32_t e,
+                        int32_t *out_cost, int32_t layout);
+
+/* ------------------------------------------------------- synthetic batches
+ * Seeded workload generator (replaces make_random_batch,
+ * benchmarks/harness.mojo:55-67, whose queries equal their references).
+ * Every pair is generated from splitmix64 keyed by (seed, config, pair
+ * index), so the batch does not depend on thread count or call order.
+ * config: 1 = plain SW (150 bp, band 32), 2 = batched SW (100-300 bp,
+ * band 16-64), 3 = WFA (1-10 kb), 4 = mixed, 5 = serving pair mix.
+ * kind[i]: 0 = SW pair, 1 = WFA pair.  For WFA pairs max_error[i] = planted
+ * edit cost + x * (#N in the query) + 8, an upper bound of the optimum. */
+int cgm_gen_create(int32_t config, int64_t n_pairs, uint64_t seed, void **handle);
+int cgm_gen_sizes(void *handle, int64_t *n_pairs, int64_t *q_words, int64_t *r_words);
+/* Copies the batch into host arrays (any pointer may be NULL to skip it):
+ * packed words/NMask words (q_words / r_words entries), offsets and lengths
+ * (n_pairs), kind, band and max_error (n_pairs). */
+int cgm_gen_copy(void *handle, uint64_t *q_packed, uint64_t *q_nmask, int32_t *q_off,
+                 int32_t *q_len, uint64_t *r_packed, uint64_t *r_nmask, int32_t *r_off,
+                 int32_t *r_len, int32_t *kind, int32_t *band, int32_t *max_error);
+int cgm_gen_destroy(void *handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CGM_ALIGN_H */

@@ -1,0 +1,144 @@
+"""ctypes loader for the CPU oracle (oracle/cgm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, always as the checker or as the timed CPU
+baseline -- never as a code path of the shipped library.
+
+Functions follow the reference Mojo sources:
+  sw_banded    genomics/cpu/align_cpu.mojo:229-309 (smith_waterman_banded)
+  wfa_affine   genomics/cpu/align_cpu.mojo:75-226  (wfa_affine_cpu)
+  pack_ascii   genomics/core/sequence.mojo:38-66   (SequenceBatch.add_sequence)
+plus the independent cross-checks sw_antidiag and gotoh_global.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libcgm_oracle.so")
+_lib = None
+
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_vp = ctypes.c_void_p
+
+
+def build(force: bool = False) -> str:
+    """Compile oracle/cgm_oracle.c with its Makefile (gcc)."""
+    if force and os.path.exists(_LIB_PATH):
+        os.remove(_LIB_PATH)
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.or_pack_ascii.restype = _i64
+        L.or_pack_ascii.argtypes = [ctypes.c_char_p, _i64, _vp, _vp]
+        L.or_get_base_public.restype = ctypes.c_int
+        L.or_get_base_public.argtypes = [_vp, _i64]
+        for name in ("or_sw_banded", "or_sw_antidiag"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp]
+        for name in ("or_wfa_affine", "or_gotoh_global"):
+            f = getattr(L, name)
+            f.restype = _i64
+            f.argtypes = [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64]
+        L.or_batch.restype = ctypes.c_int
+        L.or_batch.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
+                               _vp, _i32, _i32, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _i32,
+                               _vp, _vp, _vp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def pack_ascii(seq: str | bytes):
+    """Returns (words uint64[ceil(len/32)], nmask uint64[same])."""
+    b = seq.encode() if isinstance(seq, str) else bytes(seq)
+    nw = (len(b) + 31) // 32
+    words = np.zeros(max(nw, 1), dtype=np.uint64)
+    nmask = np.zeros(max(nw, 1), dtype=np.uint64)
+    lib().or_pack_ascii(b, len(b), _ptr(words), _ptr(nmask))
+    return words[:nw].copy(), nmask[:nw].copy()
+
+
+def _words(seq):
+    if isinstance(seq, (str, bytes)):
+        w, _ = pack_ascii(seq)
+        return w, len(seq)
+    w, n = seq
+    return np.ascontiguousarray(w, dtype=np.uint64), int(n)
+
+
+def _nonempty(w):
+    return w if w.size else np.zeros(1, dtype=np.uint64)
+
+
+def sw_banded(q, r, match=2, mismatch=-3, gap_open=-5, gap_extend=-2, band=32):
+    """(score, query_end, ref_end) of align_cpu.mojo:229-309.  q/r are ASCII
+    strings or (packed words, length) tuples.  Defaults = default_config()."""
+    qw, ql = _words(q)
+    rw, rl = _words(r)
+    out = np.zeros(3, dtype=np.int64)
+    lib().or_sw_banded(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl,
+                       match, mismatch, gap_open, gap_extend, band, _ptr(out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def sw_antidiag(q, r, match=2, mismatch=-3, gap_open=-5, gap_extend=-2, band=32):
+    qw, ql = _words(q)
+    rw, rl = _words(r)
+    out = np.zeros(3, dtype=np.int64)
+    lib().or_sw_antidiag(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl,
+                         match, mismatch, gap_open, gap_extend, band, _ptr(out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def wfa_affine(q, r, x=4, o=6, e=2, max_error=200):
+    """Cost or -1 of align_cpu.mojo:75-226.  Defaults = default_wfa_config()."""
+    qw, ql = _words(q)
+    rw, rl = _words(r)
+    return int(lib().or_wfa_affine(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl,
+                                   x, o, e, max_error))
+
+
+def gotoh_global(q, r, x=4, o=6, e=2, max_error=-1):
+    qw, ql = _words(q)
+    rw, rl = _words(r)
+    return int(lib().or_gotoh_global(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl,
+                                     x, o, e, max_error))
+
+
+def batch(mode, q_packed, r_packed, q_off, r_off, q_len, r_len, *, kind=None,
+          band=None, band_scalar=-1, match=2, mismatch=-4, gap_open=-4, gap_extend=-2,
+          max_error=None, max_error_scalar=-1, x=4, o=6, e=2, threads=1):
+    """Whole-batch driver (mode 0 = SW, 1 = WFA, kind[] overrides per pair).
+    Returns (score, q_end, r_end) int32 arrays."""
+    n = int(len(q_len))
+    score = np.zeros(n, dtype=np.int32)
+    qe = np.zeros(n, dtype=np.int32)
+    re_ = np.zeros(n, dtype=np.int32)
+    args = [np.ascontiguousarray(a) for a in (q_packed, r_packed)]
+    i32 = [None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+           for a in (q_off, r_off, q_len, r_len, kind, band, max_error)]
+    rc = lib().or_batch(mode, _ptr(_nonempty(args[0])), _ptr(_nonempty(args[1])),
+                        _ptr(i32[0]), _ptr(i32[1]), _ptr(i32[2]), _ptr(i32[3]), n,
+                        _ptr(i32[4]), _ptr(i32[5]), band_scalar, match, mismatch, gap_open,
+                        gap_extend, _ptr(i32[6]), max_error_scalar, x, o, e,
+                        _ptr(score), _ptr(qe), _ptr(re_), int(threads))
+    if rc != 0:
+        raise RuntimeError("oracle batch failed")
+    return score, qe, re_

@@ -1,0 +1,125 @@
+"""GPU parity of the banded SW kernels against the CPU oracle (bit-exact on
+score, query_end and ref_end), through the C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gpu_helpers import (CONFIG_SW, DEFAULT_SW, Uploaded, assert_same, sw_gpu, sw_oracle,
+                         workload_from_strings)
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = {"plain": 1, "lane": 2, "auto": 0}
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "alignment_vectors.json")
+
+
+@pytest.fixture(scope="module")
+def gen_mod(cgm):
+    from cgm_amd import gen
+    return gen
+
+
+def test_sw_reference_known_answers(cgm, dev):
+    cfg = cgm.default_config()
+    b = cgm.SequenceBatch()
+    for s in ("ACGTACGT", "ACGAACGT", "AAAA", "TTTT"):
+        b.add_sequence(s)
+    cfg.band_width = 8
+    r = cgm.smith_waterman_banded(cgm.get_view(b, 0), cgm.get_view(b, 0), cfg)
+    assert (r.score, r.query_start, r.ref_start, r.query_end, r.ref_end, r.cigar) == \
+        (16, 0, 0, 7, 7, [])
+    cfg.band_width = 4
+    r = cgm.smith_waterman_banded(cgm.get_view(b, 0), cgm.get_view(b, 1), cfg)
+    assert r.score > 0 and r.score >= 8
+    r = cgm.smith_waterman_banded(cgm.get_view(b, 2), cgm.get_view(b, 3), cfg)
+    assert r.score == 0
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_sw_golden_vectors(cgm, dev, layout):
+    vec = json.load(open(GOLDEN))["sw_regress"]
+    for sc in (DEFAULT_SW, CONFIG_SW):
+        vs = [v for v in vec if v["scoring"] == sc]
+        w = workload_from_strings([(v["q"], v["r"]) for v in vs], band=[v["band"] for v in vs])
+        got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout], scoring=sc)
+        exp = (np.array([v["score"] for v in vs]), np.array([v["q_end"] for v in vs]),
+               np.array([v["r_end"] for v in vs]))
+        assert_same(got, exp, f"golden {layout} {sc}")
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_sw_config1_parity(cgm, dev, orc, gen_mod, layout):
+    w = gen_mod.generate(1, 1000, seed=101)
+    print(f"[case] config1 1000 pairs layout={layout}", flush=True)
+    got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
+    assert_same(got, sw_oracle(orc, w), f"config1 {layout}")
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_sw_config2_sample_parity(cgm, dev, orc, gen_mod, layout):
+    w = gen_mod.generate(2, 4000, seed=202)
+    print(f"[case] config2 4000 pairs layout={layout}", flush=True)
+    got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
+    assert_same(got, sw_oracle(orc, w), f"config2 {layout}")
+
+
+def test_sw_config2_full_size(cgm, dev, orc, gen_mod):
+    """200,000 pairs: every layout agrees on the whole batch, and a random
+    sample of 2,000 pairs equals the oracle."""
+    w = gen_mod.generate(2, 200_000)
+    up = Uploaded(dev, w)
+    print("[case] config2 full 200000 pairs", flush=True)
+    ref = sw_gpu(cgm, dev, up, LAYOUTS["plain"])
+    for name in ("lane", "auto"):
+        assert_same(sw_gpu(cgm, dev, up, LAYOUTS[name]), ref, f"config2 full {name} vs plain")
+    idx = np.random.default_rng(0).choice(w.n_pairs, 2000, replace=False)
+    sub = w.subset(idx)
+    assert_same(tuple(a[idx] for a in ref), sw_oracle(orc, sub), "config2 full sample")
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_sw_edge_cases(cgm, dev, orc, layout):
+    pairs, bands = [], []
+    s = "ACGTTGCAACGGTCAGTCCGATAGCTAGGCTA" * 5
+    for q, r, b in [("", "ACGT", 4), ("ACGT", "", 4), ("", "", 0), ("A", "A", 0), ("A", "C", 3),
+                    (s, s, 0), (s, s, 1), (s[:31], s[:33], 2), (s[:64], s[:65], 64),
+                    (s[:65], s[:63], 100), (s[:10], s, 5), (s, s[:10], 5), (s, s[40:], 3),
+                    (s[40:], s, 50), ("NNNNACGT", "AAAAACGT", 4), ("ACGTN", "ACGTA", 2),
+                    (s, s[::-1], 32), (s[:100], s[50:150], 64), (s, s, 1000)]:
+        pairs.append((q, r))
+        bands.append(b)
+    w = workload_from_strings(pairs, band=bands)
+    got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
+    assert_same(got, sw_oracle(orc, w), f"edge {layout}")
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_sw_long_identical_pair_exceeds_int16(cgm, dev, layout):
+    """A 20 kb identical pair scores 40,000 (above int16) at the last cell."""
+    rng = np.random.default_rng(1)
+    s = "".join(rng.choice(list("ACGT"), 20000))
+    w = workload_from_strings([(s, s), (s[:17000], s[:17000])], band=[16, 3])
+    sc, qe, re_ = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
+    assert sc.tolist() == [40000, 34000]
+    assert qe.tolist() == [19999, 16999] and re_.tolist() == [19999, 16999]
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_sw_bad_pairs_are_flagged_not_read(cgm, dev, layout):
+    w = workload_from_strings([("ACGT" * 10, "ACGT" * 10)] * 4, band=[4, -1, 4, 4])
+    w.q_off[2] = 10_000       # outside the query buffer
+    w.r_len[3] = 1_000_000    # runs past the reference buffer
+    sc, qe, re_ = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
+    bad = -(2 ** 31)
+    assert sc.tolist() == [80, bad, bad, bad]
+    assert qe.tolist()[1:] == [-1, -1, -1]
+
+
+def test_sw_is_deterministic(cgm, dev, gen_mod):
+    w = gen_mod.generate(2, 20000, seed=9)
+    up = Uploaded(dev, w)
+    a = sw_gpu(cgm, dev, up, LAYOUTS["auto"])
+    b = sw_gpu(cgm, dev, up, LAYOUTS["auto"])
+    assert_same(a, b, "run twice")

@@ -1,0 +1,84 @@
+"""GPU parity of the gap-affine WFA kernels against the CPU oracle
+(bit-exact cost or -1), through the C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gpu_helpers import Uploaded, assert_same, wfa_gpu, wfa_oracle, workload_from_strings
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = {"plain": 1, "auto": 0}
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "alignment_vectors.json")
+
+
+@pytest.fixture(scope="module")
+def gen_mod(cgm):
+    from cgm_amd import gen
+    return gen
+
+
+def test_wfa_reference_known_answers(cgm, dev):
+    cfg = cgm.default_wfa_config()
+    cases = [("ACGTACGT", "ACGTACGT", 0), ("ACGT", "ACAT", 4), ("ACGT", "ACGGT", 8),
+             ("ACGT", "ACGGGT", 10), ("ACGGT", "ACGT", 8), ("AAAA", "TTTT", 16)]
+    for q, r, c in cases:
+        b = cgm.SequenceBatch()
+        b.add_sequence(q)
+        b.add_sequence(r)
+        assert cgm.wfa_affine_cpu(cgm.get_view(b, 0), cgm.get_view(b, 1), cfg) == c
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_wfa_golden_vectors(cgm, dev, layout):
+    vec = json.load(open(GOLDEN))["wfa_regress"]
+    pens = sorted({(v["x"], v["o"], v["e"]) for v in vec})
+    for x, o, e in pens:
+        vs = [v for v in vec if (v["x"], v["o"], v["e"]) == (x, o, e)]
+        w = workload_from_strings([(v["q"], v["r"]) for v in vs],
+                                  max_error=[v["max_error"] for v in vs])
+        got = wfa_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout], pen=dict(x=x, o=o, e=e))
+        assert_same(got, np.array([v["cost"] for v in vs]), f"golden {layout} {(x, o, e)}")
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_wfa_config3_sample_parity(cgm, dev, orc, gen_mod, layout):
+    w = gen_mod.generate(3, 20000)
+    idx = np.random.default_rng(3).choice(w.n_pairs, 160, replace=False)
+    sub = w.subset(idx)
+    print(f"[case] config3 sample 160 pairs layout={layout}", flush=True)
+    got = wfa_gpu(cgm, dev, Uploaded(dev, sub), LAYOUTS[layout])
+    exp = wfa_oracle(orc, sub)
+    assert (exp >= 0).all()
+    assert_same(got, exp, f"config3 {layout}")
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_wfa_edge_cases(cgm, dev, orc, layout):
+    s = "ACGTTGCAACGGTCAGTCCGATAGCTAGGCTA" * 4
+    cases = [("", "", 0), ("", "ACGT", 14), ("ACGT", "", 14), ("ACGT", "", 13), (s, s, 0),
+             (s, s[:-3], 12), (s, s[:-3], 11), (s[:40], s, 3), (s, s[::-1], 400),
+             (s, s[::-1], 20), ("NNNN" + s, "AAAA" + s, 10), (s[:33], s[1:34], 40),
+             (s[:31] + "G" + s[32:], s, 4), ("A" * 70, "A" * 64 + "C" * 6, 100)]
+    w = workload_from_strings([(q, r) for q, r, _ in cases], max_error=[m for _, _, m in cases])
+    got = wfa_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
+    exp = wfa_oracle(orc, w)
+    assert (exp == -1).any() and (exp > 0).any()
+    assert_same(got, exp, f"edge {layout}")
+
+
+def test_wfa_scalar_max_error_and_bad_pairs(cgm, dev, orc):
+    s = "ACGTTGCAACGGTCAGTCCGATAGCTAGGCTA" * 3
+    w = workload_from_strings([(s, s[1:]), (s, s[:50]), (s, s)])
+    w.r_off[2] = 99_999
+    got = wfa_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS["auto"], me_scalar=30)
+    exp = [orc.wfa_affine(s, s[1:], max_error=30), orc.wfa_affine(s, s[:50], max_error=30)]
+    assert got.tolist() == exp + [-(2 ** 31)]
+
+
+def test_wfa_is_deterministic(cgm, dev, gen_mod):
+    w = gen_mod.generate(3, 400, seed=4)
+    up = Uploaded(dev, w)
+    assert_same(wfa_gpu(cgm, dev, up, 0), wfa_gpu(cgm, dev, up, 0), "run twice")
