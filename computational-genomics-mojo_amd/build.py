@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libcgm_align.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["cgm_runtime.hip", "cgm_sw.hip", "cgm_wfa.hip"]
+HIP_SOURCES = ["cgm_runtime.hip", "cgm_sw.hip", "cgm_sw_lane.hip", "cgm_wfa.hip", "cgm_sort.hip"]
 CXX_SOURCES = ["cgm_gen.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
@@ -41,17 +41,19 @@ def _run(cmd):
 
 def build(verbose: bool = False) -> str:
     """Compile all sources and link libcgm_align.so; returns its path."""
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
     with tempfile.TemporaryDirectory(prefix="cgm_build_") as tmp:
+        jobs = []
         for src in HIP_SOURCES:
             obj = os.path.join(tmp, src + ".o")
-            _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", obj])
-            objs.append(obj)
+            jobs.append((obj, [HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", obj]))
         for src in CXX_SOURCES:
             obj = os.path.join(tmp, src + ".o")
-            _run(["g++", *COMMON, "-march=x86-64-v2", "-pthread", "-c", os.path.join(CSRC, src),
-                  "-o", obj])
-            objs.append(obj)
+            jobs.append((obj, ["g++", *COMMON, "-march=x86-64-v2", "-pthread", "-c",
+                               os.path.join(CSRC, src), "-o", obj]))
+        with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+            list(ex.map(lambda j: _run(j[1]), jobs))
+        objs = [j[0] for j in jobs]
         tmp_lib = LIB + ".tmp"
         _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp_lib])
         os.replace(tmp_lib, LIB)

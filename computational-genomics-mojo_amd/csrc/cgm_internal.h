@@ -80,10 +80,17 @@ __device__ __forceinline__ bool load_pair(const Batch &b, int64_t i, const uint6
 // asynchronously on `stream`.
 int launch_sw_plain(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                     SwOut out, hipStream_t stream);
-int launch_sw_lane(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
-                   SwOut out, hipStream_t stream);
 int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
                      WfaPen p, int32_t *out, hipStream_t stream);
+// order: optional device permutation of pair indices (work-sorted); NULL = 0..n-1.
+int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                    WfaPen p, int32_t *out, const int32_t *order, hipStream_t stream);
+
+// WFA wave layout: LDS budget per 64-lane workgroup and workgroups per CU.
+constexpr int kWfaLdsBytes = 38 * 1024;
+constexpr int kWfaBlocksPerCu = 4;
+// Largest diagonal |k| reachable at score s: (s - o) / e from o + e on.
+inline int reach_host(int s, const WfaPen &p) { return s >= p.o + p.e ? (s - p.o) / p.e : 0; }
 
 // Device-side max of an int32 array (blocking; used by launchers that size
 // LDS or scratch from a per-pair device array).
@@ -100,6 +107,37 @@ __device__ __forceinline__ int effective_band(int band, int qlen, int rlen) {
     return band < m ? band : m;
 }
 #endif
+
+// Work ordering (cgm_sort.hip): order[] = pair indices sorted by a
+// descending key.
+int sort_desc_by_key(uint32_t *keys, int32_t *idx, int32_t *order_out, int64_t n,
+                     hipStream_t stream);
+
+// SW pair classes of the automatic layout.  0..3: register lane kernels for
+// effective bands [0,16], [17,32], [33,48], [49,64] (non-positive penalties,
+// h < 2^23); 4: LDS lane kernel (band <= kLdsLaneMaxBand); 5: one workgroup
+// per pair (band <= kPlainMaxBand); 6: not alignable (reported bad).
+constexpr int kSwClasses = 7;
+constexpr int kSwClassLds = 4, kSwClassPlain = 5, kSwClassBad = 6;
+constexpr int kLdsLaneMaxBand = 150;
+constexpr int kPlainMaxBand = 1600;
+// Classifies, writes bad-pair outputs, sorts pair indices by (class, band,
+// length) and fills ranges[2*c], ranges[2*c+1] = [start, end) of class c
+// in order[] -- all on the device.
+int sw_classify(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                SwOut out, int32_t *order, int32_t *ranges, hipStream_t stream);
+int launch_sw_reg_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                          SwOut out, const int32_t *order, const int32_t *ranges, int grid,
+                          hipStream_t stream);
+int launch_sw_lds_lane(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                       SwOut out, const int32_t *order, const int32_t *range, int grid,
+                       hipStream_t stream);
+int launch_sw_plain_range(const Batch &b, const int32_t *band, int32_t band_scalar,
+                          SwScoring sc, SwOut out, const int32_t *order, const int32_t *range,
+                          int grid, hipStream_t stream);
+int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                   SwOut out, hipStream_t stream);
+int device_cu_count(int *cus);
 
 int check_sw_scoring(const SwScoring &sc);
 int check_wfa_pen(const WfaPen &p);

@@ -304,7 +304,7 @@ int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed, con
             return launch_sw_plain(b, band, band_scalar, sc, out, (hipStream_t)stream);
         case CGM_LAYOUT_AUTO:
         case CGM_LAYOUT_LANE:
-            return launch_sw_lane(b, band, band_scalar, sc, out, (hipStream_t)stream);
+            return launch_sw_auto(b, band, band_scalar, sc, out, (hipStream_t)stream);
         default:
             return set_error(CGM_ERR_INVALID, "unknown SW layout");
     }
@@ -321,10 +321,13 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed, co
     if (!max_error && max_error_scalar < 0)
         return set_error(CGM_ERR_INVALID, "max_error_scalar < 0");
     switch (layout) {
-        case CGM_LAYOUT_AUTO:
         case CGM_LAYOUT_PLAIN:
             return launch_wfa_plain(b, max_error, max_error_scalar, p, out_cost,
                                     (hipStream_t)stream);
+        case CGM_LAYOUT_AUTO:
+        case CGM_LAYOUT_WAVE:
+            return launch_wfa_wave(b, max_error, max_error_scalar, p, out_cost, nullptr,
+                                   (hipStream_t)stream);
         default:
             return set_error(CGM_ERR_INVALID, "unknown WFA layout");
     }

@@ -56,12 +56,9 @@ __device__ __forceinline__ bool sw_pair_ok(int qlen, int rlen, int band, int ban
 // ---------------------------------------------------------------------------
 constexpr int kPlainThreads = 128;
 
-__global__ __launch_bounds__(kPlainThreads) void sw_plain_kernel(Batch b, const int32_t *band_arr,
-                                                                 int32_t band_scalar, int32_t band_cap,
-                                                                 SwScoring sc, SwOut out) {
-    extern __shared__ int32_t lds[];
-    __shared__ int red_h[kPlainThreads], red_q[kPlainThreads], red_r[kPlainThreads];
-    const int64_t pair = blockIdx.x;
+__device__ void sw_plain_pair(const Batch &b, int64_t pair, const int32_t *band_arr,
+                              int32_t band_scalar, int32_t band_cap, const SwScoring &sc,
+                              const SwOut &out, int32_t *lds, int *red_h, int *red_q, int *red_r) {
     const int tid = threadIdx.x;
     const uint64_t *q, *r;
     int qlen, rlen;
@@ -130,6 +127,24 @@ __global__ __launch_bounds__(kPlainThreads) void sw_plain_kernel(Batch b, const 
         out.q_end[pair] = pos ? red_q[0] : 0;
         out.r_end[pair] = pos ? red_r[0] : 0;
     }
+    __syncthreads();
+}
+
+// Pairs order[range[0] + i] (or pair i when order is NULL, i < n_pairs),
+// one workgroup per pair, grid-stride.
+__global__ __launch_bounds__(kPlainThreads) void sw_plain_kernel(Batch b, const int32_t *band_arr,
+                                                                 int32_t band_scalar, int32_t band_cap,
+                                                                 SwScoring sc, SwOut out,
+                                                                 const int32_t *order,
+                                                                 const int32_t *range) {
+    extern __shared__ int32_t lds[];
+    __shared__ int red_h[kPlainThreads], red_q[kPlainThreads], red_r[kPlainThreads];
+    const int64_t start = range ? range[0] : 0;
+    const int64_t end = range ? range[1] : b.n_pairs;
+    for (int64_t item = start + blockIdx.x; item < end; item += gridDim.x) {
+        const int64_t pair = order ? order[item] : item;
+        sw_plain_pair(b, pair, band_arr, band_scalar, band_cap, sc, out, lds, red_h, red_q, red_r);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -145,13 +160,10 @@ __global__ __launch_bounds__(kPlainThreads) void sw_plain_kernel(Batch b, const 
 // ---------------------------------------------------------------------------
 constexpr int kLaneThreads = 64;
 
-__global__ __launch_bounds__(kLaneThreads) void sw_lane_kernel(Batch b, const int32_t *band_arr,
-                                                               int32_t band_scalar, int32_t band_cap,
-                                                               SwScoring sc, SwOut out) {
-    extern __shared__ int32_t lds[];
+__device__ void sw_lds_lane_pair(const Batch &b, int64_t pair, const int32_t *band_arr,
+                                 int32_t band_scalar, int32_t band_cap, const SwScoring &sc,
+                                 const SwOut &out, int32_t *lds) {
     const int tid = threadIdx.x;
-    const int64_t pair = (int64_t)blockIdx.x * kLaneThreads + tid;
-    if (pair >= b.n_pairs) return;
     const uint64_t *q, *r;
     int qlen, rlen;
     const bool ok = load_pair(b, pair, q, r, qlen, rlen);
@@ -216,6 +228,21 @@ __global__ __launch_bounds__(kLaneThreads) void sw_lane_kernel(Batch b, const in
     out.r_end[pair] = br;
 }
 
+__global__ __launch_bounds__(kLaneThreads) void sw_lane_kernel(Batch b, const int32_t *band_arr,
+                                                               int32_t band_scalar, int32_t band_cap,
+                                                               SwScoring sc, SwOut out,
+                                                               const int32_t *order,
+                                                               const int32_t *range) {
+    extern __shared__ int32_t lds[];
+    const int64_t start = range[0], end = range[1];
+    for (int64_t base = start + (int64_t)blockIdx.x * kLaneThreads; base < end;
+         base += (int64_t)gridDim.x * kLaneThreads) {
+        const int64_t item = base + threadIdx.x;
+        if (item < end)
+            sw_lds_lane_pair(b, order[item], band_arr, band_scalar, band_cap, sc, out, lds);
+    }
+}
+
 }  // namespace
 
 int launch_sw_plain(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
@@ -226,42 +253,80 @@ int launch_sw_plain(const Batch &b, const int32_t *band, int32_t band_scalar, Sw
     int32_t max_band = 0;
     if (int rc = device_max_band_eff(b, band, band_scalar, stream, &max_band)) return rc;
     if (max_band < 0) max_band = 0;
-    // Pairs whose band exceeds what LDS can hold are reported as bad pairs.
-    const int32_t lds_cap_band = (int32_t)((64 * 1024 / (5 * sizeof(int32_t)) - 1) / 2);
-    int32_t cap = max_band < lds_cap_band ? max_band : lds_cap_band;
-    if (cap > kMaxBand) cap = kMaxBand;
+    int32_t cap = max_band < kPlainMaxBand ? max_band : kPlainMaxBand;
     const size_t lds_bytes = (size_t)5 * (2 * cap + 1) * sizeof(int32_t);
     hipLaunchKernelGGL(sw_plain_kernel, dim3((unsigned)b.n_pairs), dim3(kPlainThreads), lds_bytes,
-                       stream, b, band, band_scalar, cap, sc, out);
+                       stream, b, band, band_scalar, cap, sc, out, nullptr, nullptr);
     CGM_HIP(hipGetLastError());
     return CGM_OK;
 }
 
-}  // namespace cgm
+int launch_sw_plain_range(const Batch &b, const int32_t *band, int32_t band_scalar,
+                          SwScoring sc, SwOut out, const int32_t *order, const int32_t *range,
+                          int grid, hipStream_t stream) {
+    const size_t lds_bytes = (size_t)5 * (2 * kPlainMaxBand + 1) * sizeof(int32_t);
+    hipLaunchKernelGGL(sw_plain_kernel, dim3(grid), dim3(kPlainThreads), lds_bytes, stream, b, band,
+                       band_scalar, kPlainMaxBand, sc, out, order, range);
+    CGM_HIP(hipGetLastError());
+    return CGM_OK;
+}
 
-namespace cgm {
+int launch_sw_lds_lane(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                       SwOut out, const int32_t *order, const int32_t *range, int grid,
+                       hipStream_t stream) {
+    const size_t lds_bytes = (size_t)(2 * kLdsLaneMaxBand + 2) * kLaneThreads * sizeof(int32_t);
+    static bool attr_set = false;
+    if (!attr_set) {
+        CGM_HIP(hipFuncSetAttribute((const void *)sw_lane_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(sw_lane_kernel, dim3(grid), dim3(kLaneThreads), lds_bytes, stream, b, band,
+                       band_scalar, kLdsLaneMaxBand, sc, out, order, range);
+    CGM_HIP(hipGetLastError());
+    return CGM_OK;
+}
 
-int launch_sw_lane(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+int device_cu_count(int *cus) {
+    static int cached[64] = {0};
+    int dev = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    if (dev < 64 && cached[dev]) {
+        *cus = cached[dev];
+        return CGM_OK;
+    }
+    int n = 0;
+    CGM_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    if (dev < 64) cached[dev] = n;
+    *cus = n;
+    return CGM_OK;
+}
+
+// Automatic layout: classify + sort on the device, then one persistent
+// launch per class (register buckets, LDS lanes, one workgroup per pair).
+int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                    SwOut out, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
     if (int rc = check_sw_scoring(sc)) return rc;
     if (b.n_pairs == 0) return CGM_OK;
-    int32_t max_band = 0;
-    if (int rc = device_max_band_eff(b, band, band_scalar, stream, &max_band)) return rc;
-    if (max_band < 0) max_band = 0;
-    // (2*band + 2) slots x 64 lanes x 4 bytes must fit the 160 KiB LDS of a CU.
-    const int32_t lds_cap_band = (int32_t)((160 * 1024 / (kLaneThreads * sizeof(int32_t)) - 2) / 2);
-    int32_t cap = max_band < lds_cap_band ? max_band : lds_cap_band;
-    const size_t lds_bytes = (size_t)(2 * cap + 2) * kLaneThreads * sizeof(int32_t);
-    if (lds_bytes > 64 * 1024) {
-        CGM_HIP(hipFuncSetAttribute((const void *)sw_lane_kernel,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-    }
-    const int64_t blocks = (b.n_pairs + kLaneThreads - 1) / kLaneThreads;
-    hipLaunchKernelGGL(sw_lane_kernel, dim3((unsigned)blocks), dim3(kLaneThreads), lds_bytes, stream,
-                       b, band, band_scalar, cap, sc, out);
-    CGM_HIP(hipGetLastError());
-    return CGM_OK;
+    int cus = 0;
+    if (int rc = device_cu_count(&cus)) return rc;
+    int32_t *order = nullptr, *ranges = nullptr;
+    CGM_HIP(hipMallocAsync((void **)&order, (size_t)b.n_pairs * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&ranges, 2 * kSwClasses * 4, stream));
+    int rc = sw_classify(b, band, band_scalar, sc, out, order, ranges, stream);
+    const int64_t waves = (b.n_pairs + 63) / 64;
+    const int reg_grid = (int)(waves < (int64_t)cus * 16 ? waves : (int64_t)cus * 16);
+    if (!rc) rc = launch_sw_reg_buckets(b, band, band_scalar, sc, out, order, ranges, reg_grid, stream);
+    if (!rc)
+        rc = launch_sw_lds_lane(b, band, band_scalar, sc, out, order, ranges + 2 * kSwClassLds,
+                                reg_grid < cus * 2 ? reg_grid : cus * 2, stream);
+    if (!rc)
+        rc = launch_sw_plain_range(b, band, band_scalar, sc, out, order,
+                                   ranges + 2 * kSwClassPlain, cus * 2, stream);
+    CGM_HIP(hipFreeAsync(order, stream));
+    CGM_HIP(hipFreeAsync(ranges, stream));
+    return rc;
 }
 
 }  // namespace cgm

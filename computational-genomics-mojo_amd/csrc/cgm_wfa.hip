@@ -143,6 +143,164 @@ __global__ __launch_bounds__(kPlainThreads) void wfa_plain_kernel(Batch b, const
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave layout: one 64-lane wavefront per pair, persistent workgroups pulling
+// pair indices from an atomic queue (optionally through a work-sorted
+// order[]).  Exact reductions of the reference loop:
+//  * only scores that are multiples of g = gcd(x, o, e) can hold a wavefront
+//    (every recurrence step costs x, o+e or e), so the others are skipped;
+//  * diagonal k needs a gap opening and |k| extensions, so at score s only
+//    |k| <= R(s) = (s - o) / e (0 below o + e) can be reached; the range is
+//    [max(-rlen, -me, -R(s)), min(qlen, me, R(s))] and widening it to the
+//    target diagonal, as the reference does, only adds unreachable cells;
+//  * ranges only grow with s, so score slots are initialised once and never
+//    cleared: a slot's entries outside its range were never written;
+//  * NEXT and EXTEND fuse per diagonal: EXTEND of diagonal k touches only
+//    M[s,k] and NEXT never reads the current score.
+// Extension compares 32 bases per step: two funnel-shifted 64-bit windows
+// XORed, leading zeros / 2 = matching bases, capped at the sequence ends.
+// Wavefront slots live in LDS when 4 * slots * W bytes fit the block's LDS
+// budget, otherwise in a per-block HBM slab.
+// ---------------------------------------------------------------------------
+constexpr int kWaveThreads = 64;
+constexpr int kNegRow = -(1 << 30);
+
+__device__ __forceinline__ uint64_t window64(const uint64_t *s, int pos, int nwords) {
+    const int w = pos >> 5, sh = 2 * (pos & 31);
+    const uint64_t lo = s[w];
+    if (!sh) return lo;
+    const uint64_t hi = (w + 1 < nwords) ? s[w + 1] : 0ull;
+    return (lo << sh) | (hi >> (64 - sh));
+}
+
+// Furthest row reached from (row, row - k) following matching bases.
+__device__ __forceinline__ int extend_diag(const uint64_t *q, int qlen, int qw, const uint64_t *r,
+                                           int rlen, int rw, int row, int k) {
+    int col = row - k;
+    if (col < 0) return row;
+    while (row < qlen && col < rlen) {
+        const uint64_t x = window64(q, row, qw) ^ window64(r, col, rw);
+        const int n = x ? (__clzll(x) >> 1) : 32;
+        const int lim = min(qlen - row, rlen - col);
+        if (n >= lim) return row + lim;
+        row += n;
+        col += n;
+        if (n < 32) break;
+    }
+    return row;
+}
+
+struct WfaGeom {
+    int g, nM, nI;  // score step, M slots, I/D slots
+};
+
+__device__ __forceinline__ int reach(int s, const WfaPen &p) {
+    return s >= p.o + p.e ? (s - p.o) / p.e : 0;
+}
+
+template <typename Ptr>
+__device__ int wfa_pair_wave(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
+                             const WfaPen &p, const WfaGeom &gm, Ptr wf, int W, int K) {
+    const int lane = threadIdx.x;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int slots = gm.nM + 2 * gm.nI;
+    for (int i = lane; i < slots * W; i += kWaveThreads) wf[i] = kNegRow;
+    __syncthreads();
+    Ptr M = wf;
+    Ptr I = wf + gm.nM * W;
+    Ptr D = I + gm.nI * W;
+    const int target = qlen - rlen;
+    if (lane == 0) M[K] = extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+    __syncthreads();
+    if (target == 0 && M[K] >= qlen) return 0;
+    for (int s = gm.g; s <= me; s += gm.g) {
+        const int si = s / gm.g;
+        const int ms = (si % gm.nM) * W, is = (si % gm.nI) * W;
+        const int Rs = reach(s, p);
+        const int lo = max(max(-rlen, -me), -Rs), hi = min(min(qlen, me), Rs);
+        const int mx = s >= p.x ? (((s - p.x) / gm.g) % gm.nM) * W : -1;
+        const int mo = s >= p.o + p.e ? (((s - p.o - p.e) / gm.g) % gm.nM) * W : -1;
+        const int ie = s >= p.e ? (((s - p.e) / gm.g) % gm.nI) * W : -1;
+        bool found = false;
+        for (int k = lo + lane; k <= hi; k += kWaveThreads) {
+            const int di = k + K;
+            int iv = kNegRow, dv = kNegRow, mv = kNegRow;
+            if (mo >= 0) {
+                if (di > 0) iv = M[mo + di - 1] + 1;
+                if (di < W - 1) dv = M[mo + di + 1];
+            }
+            if (ie >= 0) {
+                if (di > 0) iv = max(iv, I[ie + di - 1] + 1);
+                if (di < W - 1) dv = max(dv, D[ie + di + 1]);
+            }
+            if (mx >= 0) mv = M[mx + di] + 1;
+            iv = iv < 0 ? kNegRow : iv;
+            dv = dv < 0 ? kNegRow : dv;
+            mv = max(max(mv, iv), dv);
+            I[is + di] = iv;
+            D[is + di] = dv;
+            if (mv >= 0) {
+                mv = extend_diag(q, qlen, qw, r, rlen, rw, mv, k);
+            } else {
+                mv = kNegRow;
+            }
+            M[ms + di] = mv;
+            found |= (k == target && mv >= qlen);
+        }
+        if (__any(found)) return s;
+        __syncthreads();
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(kWaveThreads) void wfa_wave_kernel(
+    Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap, WfaPen p, WfaGeom gm,
+    int32_t *out, const int32_t *order, int32_t *queue, int32_t lds_words, int32_t *slab,
+    int64_t slab_words) {
+    extern __shared__ int32_t lds[];
+    __shared__ int next_item;
+    int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
+    for (;;) {
+        if (threadIdx.x == 0) next_item = atomicAdd(queue, 1);
+        __syncthreads();
+        const int64_t item = next_item;
+        __syncthreads();
+        if (item >= b.n_pairs) break;
+        const int64_t pair = order ? order[item] : item;
+        const uint64_t *q, *r;
+        int qlen, rlen;
+        const bool ok = load_pair(b, pair, q, r, qlen, rlen);
+        const int me = me_arr ? me_arr[pair] : me_scalar;
+        int res;
+        if (!ok || me < 0 || me > me_cap) {
+            res = CGM_BAD_PAIR;
+        } else if (qlen == 0 && rlen == 0) {
+            res = 0;
+        } else {
+            const int target = qlen - rlen;
+            const int Rme = reach(me, p);
+            const int K = min(min(me, max(qlen, rlen)), Rme);
+            const int W = 2 * K + 1;
+            const int need = (gm.nM + 2 * gm.nI) * W;
+            if (target != 0 && (target > K || -target > K)) {
+                res = -1;  // the target diagonal is never reached within max_error
+            } else if (need <= lds_words) {
+                res = wfa_pair_wave(q, qlen, r, rlen, me, p, gm, lds, W, K);
+            } else if (need <= slab_words) {
+                res = wfa_pair_wave(q, qlen, r, rlen, me, p, gm, my_slab, W, K);
+            } else {
+                res = CGM_BAD_PAIR;
+            }
+        }
+        if (threadIdx.x == 0) out[pair] = res;
+        __syncthreads();
+    }
+}
+
+__global__ void zero_i32_kernel(int32_t *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 }  // namespace
 
 int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
@@ -168,6 +326,64 @@ int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error
                        max_error, max_error_scalar, me_max, p, out, scratch, slab);
     hipError_t le = hipGetLastError();
     CGM_HIP(hipFreeAsync(scratch, stream));
+    CGM_HIP(le);
+    return CGM_OK;
+}
+
+}  // namespace cgm
+
+namespace cgm {
+
+int wfa_geometry(const WfaPen &p, int *g, int *nM, int *nI) {
+    auto gcd = [](int a, int b) {
+        while (b) {
+            int t = a % b;
+            a = b;
+            b = t;
+        }
+        return a;
+    };
+    *g = gcd(gcd(p.x, p.o), p.e);
+    *nM = (p.x > p.o + p.e ? p.x : p.o + p.e) / *g + 1;
+    *nI = p.e / *g + 1;
+    return CGM_OK;
+}
+
+int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
+                    int32_t *out, const int32_t *order, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    if (int rc = check_wfa_pen(p)) return rc;
+    if (b.n_pairs == 0) return CGM_OK;
+    int32_t me_max = max_error_scalar;
+    if (max_error) {
+        if (int rc = device_max_i32(max_error, b.n_pairs, stream, &me_max)) return rc;
+    }
+    if (me_max < 0) me_max = 0;
+    if (me_max > kMaxWfaError) me_max = kMaxWfaError;
+    WfaGeom gm;
+    wfa_geometry(p, &gm.g, &gm.nM, &gm.nI);
+    const int slots = gm.nM + 2 * gm.nI;
+    const int Kmax = me_max < reach_host(me_max, p) ? me_max : reach_host(me_max, p);
+    const int64_t slab_words = (int64_t)slots * (2 * (int64_t)Kmax + 1);
+    const int32_t lds_words = kWfaLdsBytes / 4;
+    int dev = 0, cus = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    CGM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int64_t grid = (int64_t)cus * kWfaBlocksPerCu;
+    if (grid > b.n_pairs) grid = b.n_pairs;
+    // HBM slabs only for pairs whose wavefronts exceed the LDS budget.
+    const bool need_slab = slab_words > lds_words;
+    int32_t *slab = nullptr, *queue = nullptr;
+    CGM_HIP(hipMallocAsync((void **)&queue, 64, stream));
+    if (need_slab)
+        CGM_HIP(hipMallocAsync((void **)&slab, (size_t)(grid * slab_words * 4), stream));
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, queue, 16);
+    hipLaunchKernelGGL(wfa_wave_kernel, dim3((unsigned)grid), dim3(kWaveThreads),
+                       (size_t)kWfaLdsBytes, stream, b, max_error, max_error_scalar, me_max, p, gm,
+                       out, order, queue, lds_words, slab, need_slab ? slab_words : 0);
+    hipError_t le = hipGetLastError();
+    if (slab) CGM_HIP(hipFreeAsync(slab, stream));
+    CGM_HIP(hipFreeAsync(queue, stream));
     CGM_HIP(le);
     return CGM_OK;
 }

@@ -10,7 +10,7 @@ from gpu_helpers import Uploaded, assert_same, wfa_gpu, wfa_oracle, workload_fro
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = {"plain": 1, "auto": 0}
+LAYOUTS = {"plain": 1, "wave": 3, "auto": 0}
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "alignment_vectors.json")
 
 
