@@ -76,6 +76,34 @@ class SwRunner:
                 d.download(self.out.ref_end, self.n))
 
 
+def wfa_extra(m, dev, gen, n_pairs=20_000):
+    """Gap-affine WFA on 20,000 pairs of 1-10 kb (x=4, o=6, e=2): pairs/s of
+    the wave layout and of the one-workgroup-per-pair layout."""
+    w = gen.generate(3, n_pairs)
+    up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
+                                                  "q_off", "r_off", "max_error")}
+    out = m.DeviceArray(w.n_pairs, np.int32)
+
+    def launch(layout):
+        m._lib.call("cgm_wfa_affine_launch", up["q_packed"].dptr, up["r_packed"].dptr,
+                    up["q_len"].dptr, up["r_len"].dptr, up["q_off"].dptr, up["r_off"].dptr,
+                    w.n_pairs, up["q_packed"].count, up["r_packed"].count, up["max_error"].dptr,
+                    0, 4, 6, 2, out.dptr, layout, dev.stream)
+
+    res = {}
+    costs = {}
+    for name, layout, steps in (("wave", 3, 3), ("plain", 1, 1)):
+        wall, kms = time_steps(m, dev, lambda: launch(layout), steps, 1)
+        costs[name] = dev.download(out, w.n_pairs)
+        res[name] = {"pairs_per_s": w.n_pairs * steps / wall, "ms_per_step": wall / steps * 1e3}
+        log(f"[bench] wfa {name}: {res[name]}")
+    res["identical"] = bool(np.array_equal(costs["wave"], costs["plain"]))
+    res["speedup_wave_vs_plain"] = res["wave"]["pairs_per_s"] / res["plain"]["pairs_per_s"]
+    res["mean_cost"] = float(costs["wave"].mean())
+    res["pairs"] = w.n_pairs
+    return res
+
+
 def time_steps(m, dev, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -135,6 +163,7 @@ def main():
 
     extras = {}
     if not args.no_extras:
+        extras["wfa_config3"] = wfa_extra(m, dev, gen)
         wall_p, kms_p = time_steps(m, dev, lambda: run.launch(1), max(2, args.steps // 5), 1)
         plain_res = run.results()
         same = all(np.array_equal(a, b) for a, b in zip(plain_res, gpu_res))

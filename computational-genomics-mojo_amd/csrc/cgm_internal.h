@@ -139,6 +139,13 @@ int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
                    SwOut out, hipStream_t stream);
 int device_cu_count(int *cus);
 
+// Fork/join of auxiliary streams (default priority, non-blocking) so that
+// independent launches of one call overlap on the GPU; both sides are
+// event-ordered, so the pattern also records correctly under graph capture.
+constexpr int kMaxAuxStreams = 4;
+int fork_streams(hipStream_t main, int n, hipStream_t *aux);
+int join_streams(hipStream_t main, int n, hipStream_t *aux);
+
 int check_sw_scoring(const SwScoring &sc);
 int check_wfa_pen(const WfaPen &p);
 int check_batch(const Batch &b);

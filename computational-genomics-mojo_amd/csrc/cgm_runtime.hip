@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -131,6 +132,60 @@ int device_max_i32(const int32_t *a, int64_t n, hipStream_t stream, int32_t *res
     CGM_HIP(e2);
     CGM_HIP(e3);
     *result = h;
+    return CGM_OK;
+}
+
+namespace {
+struct AuxStreams {
+    hipStream_t s[kMaxAuxStreams] = {};
+    hipEvent_t fork[kMaxAuxStreams] = {}, join[kMaxAuxStreams] = {};
+    bool ready = false;
+};
+constexpr int kMaxDevices = 64;
+AuxStreams g_aux[kMaxDevices];
+std::mutex g_aux_mu;
+thread_local hipEvent_t t_fork_event[kMaxDevices] = {};
+}  // namespace
+
+static int aux_for_device(AuxStreams **out) {
+    int dev = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(CGM_ERR_INVALID, "device index too large");
+    std::lock_guard<std::mutex> lk(g_aux_mu);
+    AuxStreams &a = g_aux[dev];
+    if (!a.ready) {
+        for (int i = 0; i < kMaxAuxStreams; ++i) {
+            CGM_HIP(hipStreamCreateWithFlags(&a.s[i], hipStreamNonBlocking));
+            CGM_HIP(hipEventCreateWithFlags(&a.join[i], hipEventDisableTiming));
+        }
+        a.ready = true;
+    }
+    *out = &a;
+    return CGM_OK;
+}
+
+int fork_streams(hipStream_t main, int n, hipStream_t *aux) {
+    AuxStreams *a = nullptr;
+    if (int rc = aux_for_device(&a)) return rc;
+    int dev = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    hipEvent_t &ev = t_fork_event[dev];
+    if (!ev) CGM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    CGM_HIP(hipEventRecord(ev, main));
+    for (int i = 0; i < n && i < kMaxAuxStreams; ++i) {
+        CGM_HIP(hipStreamWaitEvent(a->s[i], ev, 0));
+        aux[i] = a->s[i];
+    }
+    return CGM_OK;
+}
+
+int join_streams(hipStream_t main, int n, hipStream_t *aux) {
+    AuxStreams *a = nullptr;
+    if (int rc = aux_for_device(&a)) return rc;
+    for (int i = 0; i < n && i < kMaxAuxStreams; ++i) {
+        CGM_HIP(hipEventRecord(a->join[i], aux[i]));
+        CGM_HIP(hipStreamWaitEvent(main, a->join[i], 0));
+    }
     return CGM_OK;
 }
 

@@ -17,13 +17,13 @@ namespace {
 __global__ void sw_class_kernel(Batch b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                                 SwOut out, uint32_t *keys, int32_t *idx, int32_t *counts) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= b.n_pairs) return;
+    const bool in = i < b.n_pairs;
     const uint64_t *q, *r;
     int ql = 0, rl = 0;
-    int cls = kSwClassBad, be = 0;
+    int cls = in ? kSwClassBad : -1, be = 0;
     const bool reg_scoring = sc.mismatch <= 0 && sc.gap_extend <= 0 &&
                              sc.gap_open + sc.gap_extend <= 0;
-    if (load_pair(b, i, q, r, ql, rl)) {
+    if (in && load_pair(b, i, q, r, ql, rl)) {
         const int bd = band ? band[i] : band_scalar;
         if (bd >= 0) {
             be = effective_band(bd, ql, rl);
@@ -39,12 +39,18 @@ __global__ void sw_class_kernel(Batch b, const int32_t *band, int32_t band_scala
             }
         }
     }
+    // one atomic per class per wavefront
+    const int lane = threadIdx.x & 63;
+    for (int c = 0; c < kSwClasses; ++c) {
+        const uint64_t m = __ballot(cls == c);
+        if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&counts[c], __popcll(m));
+    }
+    if (!in) return;
     if (cls == kSwClassBad) {
         out.score[i] = CGM_BAD_PAIR;
         out.q_end[i] = -1;
         out.r_end[i] = -1;
     }
-    atomicAdd(&counts[cls], 1);
     const uint32_t len = (uint32_t)min(max(ql, rl), (1 << 17) - 1);
     keys[i] = ((uint32_t)(7 - cls) << 28) | ((uint32_t)min(be, 2047) << 17) | len;
     idx[i] = (int32_t)i;

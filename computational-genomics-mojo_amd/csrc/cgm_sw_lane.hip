@@ -183,16 +183,20 @@ __global__ __launch_bounds__(64) void sw_reg_kernel(Batch b, const int32_t *band
 int launch_sw_reg_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                           SwOut out, const int32_t *order, const int32_t *ranges, int grid,
                           hipStream_t stream) {
+    // The buckets are independent: run them side by side on forked streams
+    // so that together they fill the GPU.
+    hipStream_t aux[4];
+    if (int rc = fork_streams(stream, 4, aux)) return rc;
 #define CGM_REG(K, LO, HI)                                                                   \
-    hipLaunchKernelGGL((sw_reg_kernel<LO, HI>), dim3(grid), dim3(64), 0, stream, b, band,     \
-                       band_scalar, sc, out, order, ranges + 2 * (K));
-    CGM_REG(0, 0, 16)
-    CGM_REG(1, 17, 32)
-    CGM_REG(2, 33, 48)
+    hipLaunchKernelGGL((sw_reg_kernel<LO, HI>), dim3(grid), dim3(64), 0, aux[3 - (K)], b,     \
+                       band, band_scalar, sc, out, order, ranges + 2 * (K));
     CGM_REG(3, 49, 64)
+    CGM_REG(2, 33, 48)
+    CGM_REG(1, 17, 32)
+    CGM_REG(0, 0, 16)
 #undef CGM_REG
     CGM_HIP(hipGetLastError());
-    return CGM_OK;
+    return join_streams(stream, 4, aux);
 }
 
 
