@@ -104,6 +104,39 @@ def wfa_extra(m, dev, gen, n_pairs=20_000):
     return res
 
 
+def mixed_extra(m, dev, gen, n_pairs=100_000):
+    """Mixed batch (50% SW 100-300 bp band 32, 40% SW 1-2 kb band 128, 10%
+    WFA 5-20 kb): pairs/s of the persistent task-queue kernel replayed from
+    a hipGraph, of the same enqueued eagerly, and of the one-workgroup-per-
+    pair kernels."""
+    w = gen.generate(4, n_pairs)
+    n = w.n_pairs
+    bufs = tuple(dev.upload(a) for a in (w.q_packed, w.r_packed, w.q_len, w.r_len, w.q_off,
+                                          w.r_off))
+    kind, band, me = (dev.upload(a) for a in (w.kind, w.band, w.max_error))
+    cfg = m.AlignConfig(SW["match"], SW["mismatch"], SW["gap_open"], SW["gap_extend"], 0)
+    out = m.SwResult(n)
+    plan = m.MixedPlan(n, int(w.max_error.max()))
+    res = {"pairs": n}
+    plan.capture(dev, bufs, n, kind, band, me, cfg, out)
+    wall, kms = time_steps(m, dev, lambda: plan.replay(dev), 3, 1)
+    graph_res = [dev.download(a, n) for a in (out.scores, out.query_end, out.ref_end)]
+    res["persistent_graph"] = {"pairs_per_s": n * 3 / wall, "ms_per_step": wall / 3 * 1e3}
+    wall, kms = time_steps(m, dev, lambda: plan.align(dev, bufs, n, kind, band, me, cfg, out), 2, 0)
+    res["persistent_eager"] = {"pairs_per_s": n * 2 / wall, "ms_per_step": wall / 2 * 1e3}
+    log(f"[bench] mixed persistent: {res['persistent_graph']} eager {res['persistent_eager']}")
+    wall, kms = time_steps(m, dev, lambda: m.align_mixed_plain(dev, bufs, n, kind, band, me, cfg,
+                                                               out), 1, 0)
+    plain_res = [dev.download(a, n) for a in (out.scores, out.query_end, out.ref_end)]
+    res["plain_wg_per_pair"] = {"pairs_per_s": n / wall, "ms_per_step": wall * 1e3}
+    res["identical"] = all(bool(np.array_equal(a, b)) for a, b in zip(graph_res, plain_res))
+    res["speedup_vs_plain"] = res["persistent_graph"]["pairs_per_s"] / \
+        res["plain_wg_per_pair"]["pairs_per_s"]
+    log(f"[bench] mixed plain: {res['plain_wg_per_pair']} speedup {res['speedup_vs_plain']:.2f}")
+    plan.close()
+    return res
+
+
 def time_steps(m, dev, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -164,6 +197,7 @@ def main():
     extras = {}
     if not args.no_extras:
         extras["wfa_config3"] = wfa_extra(m, dev, gen)
+        extras["mixed_config4"] = mixed_extra(m, dev, gen)
         wall_p, kms_p = time_steps(m, dev, lambda: run.launch(1), max(2, args.steps // 5), 1)
         plain_res = run.results()
         same = all(np.array_equal(a, b) for a, b in zip(plain_res, gpu_res))

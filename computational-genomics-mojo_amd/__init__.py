@@ -8,7 +8,9 @@ libcgm_align.so (include/cgm_align.h).  The package directory name is not a
 Python identifier; load it with importlib (see tests/conftest.py).
 """
 from . import _lib
-from .align import (AlignConfig, AlignResult, SwResult, WFAConfig, default_config,
+from .align import (AlignConfig, AlignResult, MixedPlan, SwResult, WFAConfig, align_mixed_batch,
+                    align_mixed_plain,
+                    default_config,
                     default_wfa_config, launch_banded_sw, launch_wfa_affine,
                     smith_waterman_banded, sw_banded_batch, wfa_affine, wfa_affine_batch,
                     wfa_affine_cpu)
@@ -17,7 +19,7 @@ from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_
                        get_base_from_word, get_view, pack_codes)
 
 __all__ = [
-    "AlignConfig", "AlignResult", "SwResult", "WFAConfig", "default_config",
+    "AlignConfig", "AlignResult", "MixedPlan", "align_mixed_batch", "align_mixed_plain", "SwResult", "WFAConfig", "default_config",
     "default_wfa_config", "launch_banded_sw", "launch_wfa_affine", "smith_waterman_banded",
     "sw_banded_batch", "wfa_affine", "wfa_affine_batch", "wfa_affine_cpu", "DeviceArray",
     "GenomicsDevice", "Timer", "BASES_PER_WORD", "Base", "SequenceBatch", "SequenceView",

@@ -60,6 +60,14 @@ SIGNATURES = {
                              _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp],
     "cgm_wfa_affine_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                               _i32, _i32, _i32, _vp, _i32, _vp],
+    "cgm_plan_create": [_i64, _i32, _i32, _i32, _i32, _i32, _i32, _pvp],
+    "cgm_plan_destroy": [_vp],
+    "cgm_align_mixed": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32,
+                        _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32,
+                        _vp],
+    "cgm_plan_capture": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32,
+                         _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
+    "cgm_plan_replay": [_vp, _vp],
     "cgm_sw_banded_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                            _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32],
     "cgm_wfa_affine_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,

@@ -185,3 +185,85 @@ def batch_arrays(queries: SequenceBatch, refs: SequenceBatch):
     """(q_packed, q_off, q_len, r_packed, r_off, r_len) of two batches."""
     return (queries.packed, queries.offsets, queries.lengths, refs.packed, refs.offsets,
             refs.lengths)
+
+
+def _mixed_args(bufs, n, kind, band, max_error, cfg: AlignConfig, out: SwResult, pen):
+    q, r, ql, rl, qo, ro = bufs
+    x, o, e = pen
+    ptr = lambda a: a.dptr if a is not None else None  # noqa: E731
+    return (q.dptr, r.dptr, ql.dptr, rl.dptr, qo.dptr, ro.dptr, n, q.count, r.count,
+            ptr(kind), 0, ptr(band), cfg.band_width, ptr(max_error), 0, cfg.match_score,
+            cfg.mismatch, cfg.gap_open, cfg.gap_extend, x, o, e, out.scores.dptr,
+            out.query_end.dptr, out.ref_end.dptr)
+
+
+class MixedPlan:
+    """Preallocated workspace of the persistent mixed SW + WFA layout
+    (cgm_plan_create): sort keys, task queue and HBM wavefront slabs for WFA
+    pairs up to max_error_cap.  align() enqueues one call without any
+    allocation or host synchronisation; capture()/replay() record it into a
+    hipGraph and launch it again.  One device, one call at a time."""
+
+    def __init__(self, max_pairs: int, max_error_cap: int, x: int = 4, o: int = 6, e: int = 2,
+                 waves_per_cu: int = 0, lds_bytes: int = 0):
+        import ctypes
+        h = ctypes.c_void_p()
+        _lib.call("cgm_plan_create", int(max_pairs), int(max_error_cap), x, o, e, waves_per_cu,
+                  lds_bytes, ctypes.byref(h))
+        self.handle = h.value
+        self.pen = (x, o, e)
+
+    def align(self, device: GenomicsDevice, bufs, n: int, kind, band, max_error, cfg: AlignConfig,
+              out: SwResult, layout: int = _lib.LAYOUT_AUTO) -> None:
+        _lib.call("cgm_align_mixed", self.handle,
+                  *_mixed_args(bufs, n, kind, band, max_error, cfg, out, self.pen), layout,
+                  device.stream)
+
+    def capture(self, device: GenomicsDevice, bufs, n: int, kind, band, max_error,
+                cfg: AlignConfig, out: SwResult) -> None:
+        _lib.call("cgm_plan_capture", self.handle,
+                  *_mixed_args(bufs, n, kind, band, max_error, cfg, out, self.pen), device.stream)
+
+    def replay(self, device: GenomicsDevice) -> None:
+        _lib.call("cgm_plan_replay", self.handle, device.stream)
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().cgm_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def align_mixed_plain(device: GenomicsDevice, bufs, n: int, kind, band, max_error,
+                      cfg: AlignConfig, out: SwResult, pen=(4, 6, 2)) -> None:
+    """The one-workgroup-per-pair baseline for a mixed batch (no plan)."""
+    _lib.call("cgm_align_mixed", None, *_mixed_args(bufs, n, kind, band, max_error, cfg, out, pen),
+              _lib.LAYOUT_PLAIN, device.stream)
+
+
+def align_mixed_batch(device: GenomicsDevice, w, cfg: AlignConfig, x: int = 4, o: int = 6,
+                      e: int = 2, layout: int = _lib.LAYOUT_AUTO, plan: "MixedPlan | None" = None):
+    """Host convenience: uploads a workload (packed buffers, kind, band,
+    max_error arrays), aligns every pair (SW or WFA by kind) and returns host
+    (score, query_end, ref_end); WFA pairs carry their cost and ends -1."""
+    nz = lambda a: a if a.size else np.zeros(1, a.dtype)  # noqa: E731
+    n = int(w.q_len.size)
+    bufs = tuple(device.upload(nz(a)) for a in (w.q_packed, w.r_packed, w.q_len, w.r_len,
+                                                 w.q_off, w.r_off))
+    kind, band, me = (device.upload(nz(a)) for a in (w.kind, w.band, w.max_error))
+    out = SwResult(n)
+    if layout == _lib.LAYOUT_PLAIN:
+        align_mixed_plain(device, bufs, n, kind, band, me, cfg, out, (x, o, e))
+    else:
+        own = plan is None
+        if own:
+            plan = MixedPlan(max(n, 1), int(max(int(w.max_error.max(initial=0)), 1)), x, o, e)
+        plan.align(device, bufs, n, kind, band, me, cfg, out, layout)
+    device.synchronize()
+    return (device.download(out.scores, n), device.download(out.query_end, n),
+            device.download(out.ref_end, n))

@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libcgm_align.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["cgm_runtime.hip", "cgm_sw.hip", "cgm_sw_lane.hip", "cgm_wfa.hip", "cgm_sort.hip"]
+HIP_SOURCES = ["cgm_runtime.hip", "cgm_sw.hip", "cgm_sw_lane.hip", "cgm_wfa.hip", "cgm_sort.hip",
+               "cgm_mixed.hip"]
 CXX_SOURCES = ["cgm_gen.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]

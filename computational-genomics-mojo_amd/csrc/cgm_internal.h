@@ -86,6 +86,32 @@ int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error
 int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
                     WfaPen p, int32_t *out, const int32_t *order, hipStream_t stream);
 
+int wfa_geometry(const WfaPen &p, int *g, int *nM, int *nI);
+// One-workgroup-per-pair baselines restricted to one kind of a mixed batch
+// (kind[i] == 0: SW, 1: WFA); kind == NULL means every pair.
+int launch_sw_plain_kind(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                         SwOut out, const int32_t *kind, hipStream_t stream);
+int launch_wfa_plain_kind(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                          WfaPen p, int32_t *out, const int32_t *kind, int32_t *q_end,
+                          int32_t *r_end, hipStream_t stream);
+
+// Mixed SW + WFA batches (cgm_mixed.hip).
+struct MixPlan;
+int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per_cu,
+                    int lds_bytes, MixPlan **out);
+void mix_plan_destroy(MixPlan *P);
+int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
+                SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream);
+int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
+                SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream);
+int mix_replay(MixPlan *P, hipStream_t stream);
+
+// Defaults of the persistent mixed kernel (overridden by cgm_plan_create).
+constexpr int kMixDefaultWavesPerCu = 8;
+constexpr int kMixDefaultLdsBytes = 20 * 1024;
+
 // WFA wave layout: LDS budget per 64-lane workgroup and workgroups per CU.
 constexpr int kWfaLdsBytes = 38 * 1024;
 constexpr int kWfaBlocksPerCu = 4;

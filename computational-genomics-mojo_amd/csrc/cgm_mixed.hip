@@ -1,0 +1,392 @@
+// cgm_mixed.hip -- mixed SW + WFA batches: one persistent kernel pulling a
+// work-sorted task queue, with every buffer preallocated in a plan so the
+// whole enqueue (classify, sort, task build, persistent launch, fallbacks)
+// performs no allocation and no host synchronisation and can be captured
+// into a hipGraph and replayed.
+//
+// Pipeline (all device work, one stream):
+//  1. classify each pair (kind, effective band, lengths, max_error) into
+//     0 SW register lanes (band <= 64), 1 SW packed register lanes (band 128),
+//     2 WFA wavefront, 3 SW LDS lanes, 4 SW one-workgroup-per-pair, 5 bad;
+//     bad pairs get their outputs written here;
+//  2. radix-sort pair indices by (class, band, length) or (class, WFA work);
+//  3. build tasks: 64 consecutive SW pairs of one register class per task,
+//     one WFA pair per task, each with an estimated cost in wave-cycles;
+//  4. radix-sort tasks by cost, largest first (longest-processing-time
+//     order, so the tail of the launch is made of small tasks);
+//  5. the persistent kernel: CUs x waves_per_cu one-wavefront workgroups
+//     take task indices from an atomic counter until the queue is empty;
+//  6. classes 3 and 4 (unusual scorings or bands) run the LDS-lane and
+//     one-workgroup-per-pair kernels over their ranges.
+
+#include <hipcub/hipcub.hpp>
+
+#include "cgm_sw_dev.h"
+#include "cgm_wfa_dev.h"
+
+namespace cgm {
+
+namespace {
+
+constexpr int kMixClasses = 6;
+constexpr int kMixSwReg = 0, kMixSwPacked = 1, kMixWfa = 2, kMixSwLds = 3, kMixSwPlain = 4,
+              kMixBad = 5;
+constexpr int kPackedBand = 128;
+
+enum TaskType : int32_t { kTaskSwReg = 0, kTaskSwPacked = 1, kTaskWfa = 2 };
+
+struct Task {
+    int32_t type, start, count, pad;
+};
+
+__device__ __forceinline__ uint32_t wfa_work_estimate(int me, int qlen, int rlen,
+                                                      const WfaPen &p, int g) {
+    // sum over scores of ceil(W(s) / 64) wave steps, ~25 units each, plus
+    // the extension of the longer sequence
+    const int64_t R = me >= p.o + p.e ? (me - p.o) / p.e : 0;
+    const int64_t steps = (int64_t)(me / g) * (R / 32 + 1);
+    const int64_t est = 25 * steps + (max(qlen, rlen) >> 3) + 1;
+    return (uint32_t)min(est, (int64_t)((1 << 28) - 1));
+}
+
+__global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scalar,
+                                 const int32_t *band, int32_t band_scalar, const int32_t *me_arr,
+                                 int32_t me_scalar, int32_t me_cap, SwScoring sc, WfaPen p, int g,
+                                 SwOut out, uint32_t *keys, int32_t *idx, int32_t *counts) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool in = i < b.n_pairs;
+    const uint64_t *q, *r;
+    int ql = 0, rl = 0;
+    int cls = in ? kMixBad : -1;
+    uint32_t payload = 0;
+    if (in && load_pair(b, i, q, r, ql, rl)) {
+        const int k = kind ? kind[i] : kind_scalar;
+        if (k == 0) {
+            const int bd = band ? band[i] : band_scalar;
+            if (bd >= 0) {
+                const int be = effective_band(bd, ql, rl);
+                const int64_t hmax = (int64_t)max(sc.match, 0) * min(ql, rl);
+                const int oe = sc.gap_open + sc.gap_extend;
+                const bool reg_scoring = sc.mismatch <= 0 && sc.gap_extend <= 0 && oe <= 0;
+                if (hmax > kMaxScoreProduct) {
+                    cls = kMixBad;
+                } else if (reg_scoring && be <= 64 && hmax < (1 << 23)) {
+                    cls = kMixSwReg;
+                } else if (reg_scoring && be == kPackedBand && hmax + oe <= 32767 && oe >= -32768) {
+                    cls = kMixSwPacked;
+                } else if (be <= kLdsLaneMaxBand) {
+                    cls = kMixSwLds;
+                } else if (be <= kPlainMaxBand) {
+                    cls = kMixSwPlain;
+                }
+                payload = ((uint32_t)min(be, 2047) << 17) | (uint32_t)min(max(ql, rl), (1 << 17) - 1);
+            }
+        } else if (k == 1) {
+            const int me = me_arr ? me_arr[i] : me_scalar;
+            if (me >= 0 && me <= me_cap) {
+                cls = kMixWfa;
+                payload = wfa_work_estimate(me, ql, rl, p, g);
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    for (int c = 0; c < kMixClasses; ++c) {
+        const uint64_t m = __ballot(cls == c);
+        if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&counts[c], __popcll(m));
+    }
+    if (!in) return;
+    if (cls == kMixBad) {
+        out.score[i] = CGM_BAD_PAIR;
+        if (out.q_end) out.q_end[i] = -1;
+        if (out.r_end) out.r_end[i] = -1;
+    }
+    keys[i] = ((uint32_t)(7 - cls) << 28) | (payload & 0x0fffffffu);
+    idx[i] = (int32_t)i;
+}
+
+__global__ void mix_ranges_kernel(const int32_t *counts, int32_t *ranges, int32_t *n_tasks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int32_t s = 0;
+    for (int c = 0; c < kMixClasses; ++c) {
+        ranges[2 * c] = s;
+        s += counts[c];
+        ranges[2 * c + 1] = s;
+    }
+    *n_tasks = (counts[kMixSwReg] + 63) / 64 + (counts[kMixSwPacked] + 63) / 64 + counts[kMixWfa];
+}
+
+__global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
+                                 const uint32_t *sorted_keys, int64_t cap, Task *tasks,
+                                 uint32_t *tkeys, int32_t *tidx) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= cap) return;
+    const int32_t c0 = ranges[2 * kMixSwReg + 1] - ranges[2 * kMixSwReg];
+    const int32_t c1 = ranges[2 * kMixSwPacked + 1] - ranges[2 * kMixSwPacked];
+    const int64_t n0 = (c0 + 63) / 64, n1 = (c1 + 63) / 64;
+    Task task = {0, 0, 0, 0};
+    uint32_t key = 0;
+    if (t < *n_tasks) {
+        uint32_t payload;
+        if (t < n0 + n1) {
+            const bool packed = t >= n0;
+            const int64_t g = packed ? t - n0 : t;
+            const int32_t base = packed ? ranges[2 * kMixSwPacked] : ranges[2 * kMixSwReg];
+            const int32_t cnt = packed ? c1 : c0;
+            task.type = packed ? kTaskSwPacked : kTaskSwReg;
+            task.start = base + (int32_t)(64 * g);
+            task.count = (int32_t)min((int64_t)64, cnt - 64 * g);
+            payload = sorted_keys[task.start] & 0x0fffffffu;
+            const uint32_t band = payload >> 17, len = payload & 0x1ffffu;
+            // ~12 VALU ops x 4 cycles per wave-step of 64 cells, in units of 16 cycles
+            key = (uint32_t)min((uint64_t)len * (2 * band + 1) * 3 + 1, (uint64_t)0xffffffffu);
+        } else {
+            task.type = kTaskWfa;
+            task.start = ranges[2 * kMixWfa] + (int32_t)(t - n0 - n1);
+            task.count = 1;
+            key = (sorted_keys[task.start] & 0x0fffffffu) + 1;
+        }
+    }
+    tasks[t] = task;
+    tkeys[t] = key;
+    tidx[t] = (int32_t)t;
+}
+
+__global__ void mix_zero_kernel(int32_t *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// The persistent kernel: one wavefront per workgroup, tasks in order.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void mix_persistent_kernel(
+    Batch b, const int32_t *band_arr, int32_t band_scalar, const int32_t *me_arr,
+    int32_t me_scalar, SwScoring sc, WfaPen p, wfadev::WfaGeom gm, SwOut out,
+    const int32_t *order, const Task *tasks, const int32_t *task_order, const int32_t *n_tasks,
+    int32_t *queue, int32_t lds_words, int32_t *slab, int64_t slab_words) {
+    extern __shared__ int32_t lds[];
+    __shared__ int next_task;
+    int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
+    const int total = *n_tasks;
+    for (;;) {
+        if (threadIdx.x == 0) next_task = atomicAdd(queue, 1);
+        __syncthreads();
+        const int t = next_task;
+        __syncthreads();
+        if (t >= total) break;
+        const Task task = tasks[task_order[t]];
+        if (task.type == kTaskSwReg) {
+            swdev::sw_reg_group<0, 64, false>(b, band_arr, band_scalar, sc, out, order, task.start,
+                                              task.count);
+        } else if (task.type == kTaskSwPacked) {
+            swdev::sw_reg_group<kPackedBand, kPackedBand, true>(b, band_arr, band_scalar, sc, out,
+                                                                order, task.start, task.count);
+        } else {
+            const int64_t pair = order[task.start];
+            const uint64_t *q, *r;
+            int qlen, rlen;
+            load_pair(b, pair, q, r, qlen, rlen);  // validated by the classifier
+            const int me = me_arr ? me_arr[pair] : me_scalar;
+            int res;
+            if (qlen == 0 && rlen == 0) {
+                res = 0;
+            } else {
+                const int target = qlen - rlen;
+                const int K = min(min(me, max(qlen, rlen)), wfadev::reach(me, p));
+                const int W = 2 * K + 1;
+                const int need = (gm.nM + 2 * gm.nI) * W;
+                if (target != 0 && (target > K || -target > K)) {
+                    res = -1;
+                } else if (need <= lds_words) {
+                    res = wfadev::wfa_pair_wave(q, qlen, r, rlen, me, p, gm, lds, W, K);
+                } else {
+                    res = wfadev::wfa_pair_wave(q, qlen, r, rlen, me, p, gm, my_slab, W, K);
+                }
+            }
+            if (threadIdx.x == 0) {
+                out.score[pair] = res;
+                if (out.q_end) out.q_end[pair] = -1;
+                if (out.r_end) out.r_end[pair] = -1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+struct MixPlan {
+    int device = 0;
+    int64_t cap = 0;        // max pairs per call
+    int64_t task_cap = 0;   // max tasks
+    int32_t me_cap = 0;
+    WfaPen pen{};
+    wfadev::WfaGeom gm{};
+    int grid = 0;
+    int lds_bytes = 0;
+    // workspace
+    uint32_t *keys = nullptr, *keys_out = nullptr, *tkeys = nullptr, *tkeys_out = nullptr;
+    int32_t *idx = nullptr, *order = nullptr, *tidx = nullptr, *task_order = nullptr;
+    int32_t *small = nullptr;  // counts[8] | ranges[16] | n_tasks | queue
+    Task *tasks = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int32_t *slab = nullptr;
+    int64_t slab_words = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~MixPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        void *ptrs[] = {keys, keys_out, tkeys, tkeys_out, idx, order, tidx, task_order, small,
+                        tasks, sort_tmp, slab};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per_cu,
+                    int lds_bytes, MixPlan **out) {
+    if (max_pairs <= 0 || max_pairs > (int64_t(1) << 31) - 1)
+        return set_error(CGM_ERR_INVALID, "plan: max_pairs out of range");
+    if (me_cap < 0 || me_cap > kMaxWfaError) return set_error(CGM_ERR_INVALID, "plan: bad max_error cap");
+    if (int rc = check_wfa_pen(pen)) return rc;
+    if (waves_per_cu < 1 || waves_per_cu > 32) return set_error(CGM_ERR_INVALID, "plan: waves_per_cu");
+    if (lds_bytes < 0 || lds_bytes > 160 * 1024) return set_error(CGM_ERR_INVALID, "plan: lds bytes");
+    MixPlan *P = new MixPlan();
+    auto fail = [&](int rc) {
+        delete P;
+        return rc;
+    };
+    P->cap = max_pairs;
+    P->task_cap = max_pairs + (max_pairs + 63) / 64 + 1;
+    P->me_cap = me_cap;
+    P->pen = pen;
+    wfa_geometry(pen, &P->gm.g, &P->gm.nM, &P->gm.nI);
+    if (hipGetDevice(&P->device) != hipSuccess) return fail(set_error(CGM_ERR_HIP, "hipGetDevice"));
+    int cus = 0;
+    if (int rc = device_cu_count(&cus)) return fail(rc);
+    P->grid = cus * waves_per_cu;
+    P->lds_bytes = lds_bytes & ~255;
+    const int64_t n = max_pairs, nt = P->task_cap;
+#define CGM_ALLOC(ptr, bytes)                                                      \
+    do {                                                                           \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (size_t)(bytes));               \
+        if (e_ != hipSuccess) return fail(hip_error(e_, "plan workspace"));        \
+    } while (0)
+    CGM_ALLOC(P->keys, n * 4);
+    CGM_ALLOC(P->keys_out, n * 4);
+    CGM_ALLOC(P->idx, n * 4);
+    CGM_ALLOC(P->order, n * 4);
+    CGM_ALLOC(P->tkeys, nt * 4);
+    CGM_ALLOC(P->tkeys_out, nt * 4);
+    CGM_ALLOC(P->tidx, nt * 4);
+    CGM_ALLOC(P->task_order, nt * 4);
+    CGM_ALLOC(P->tasks, nt * sizeof(Task));
+    CGM_ALLOC(P->small, 64 * 4);
+    size_t b1 = 0, b2 = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b1, P->keys, P->keys_out, P->idx,
+                                                     P->order, (int)n, 0, 32) != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b2, P->tkeys, P->tkeys_out, P->tidx,
+                                                     P->task_order, (int)nt, 0, 32) != hipSuccess)
+        return fail(set_error(CGM_ERR_HIP, "plan: sort size query"));
+    P->sort_tmp_bytes = b1 > b2 ? b1 : b2;
+    CGM_ALLOC(P->sort_tmp, P->sort_tmp_bytes + 256);
+    // HBM wavefront slab per workgroup for WFA pairs too large for LDS.
+    const int K = me_cap < reach_host(me_cap, pen) ? me_cap : reach_host(me_cap, pen);
+    P->slab_words = (int64_t)(P->gm.nM + 2 * P->gm.nI) * (2 * (int64_t)K + 1);
+    if (P->slab_words * 4 > P->lds_bytes) {
+        CGM_ALLOC(P->slab, (int64_t)P->grid * P->slab_words * 4);
+    } else {
+        P->slab_words = 0;
+    }
+#undef CGM_ALLOC
+    if (P->lds_bytes > 64 * 1024) {
+        if (hipFuncSetAttribute((const void *)mix_persistent_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                P->lds_bytes) != hipSuccess)
+            return fail(set_error(CGM_ERR_HIP, "plan: LDS attribute"));
+    }
+    *out = P;
+    return CGM_OK;
+}
+
+void mix_plan_destroy(MixPlan *P) { delete P; }
+
+// Enqueues the whole mixed pipeline on `stream` (no allocation, no sync).
+int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
+                SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    if (int rc = check_sw_scoring(sc)) return rc;
+    if (b.n_pairs > P->cap) return set_error(CGM_ERR_INVALID, "batch larger than the plan");
+    if (pen.x != P->pen.x || pen.o != P->pen.o || pen.e != P->pen.e)
+        return set_error(CGM_ERR_INVALID, "WFA penalties differ from the plan's");
+    if (b.n_pairs == 0) return CGM_OK;
+    int dev = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    if (dev != P->device) return set_error(CGM_ERR_INVALID, "plan belongs to another device");
+    int32_t *counts = P->small, *ranges = P->small + 8, *n_tasks = P->small + 24,
+            *queue = P->small + 25;
+    const int64_t n = b.n_pairs, nt = P->task_cap;
+    hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(64), 0, stream, P->small, 64);
+    hipLaunchKernelGGL(mix_class_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, b,
+                       kind, kind_scalar, band, band_scalar, me, me_scalar, P->me_cap, sc, pen,
+                       P->gm.g, out, P->keys, P->idx, counts);
+    hipLaunchKernelGGL(mix_ranges_kernel, dim3(1), dim3(64), 0, stream, counts, ranges, n_tasks);
+    size_t tb = P->sort_tmp_bytes;
+    CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->keys, P->keys_out,
+                                                         P->idx, P->order, (int)n, 0, 32, stream));
+    const int64_t tasks_needed = n + (n + 63) / 64 + 1;
+    const int64_t tn = tasks_needed < nt ? tasks_needed : nt;
+    hipLaunchKernelGGL(mix_tasks_kernel, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, stream,
+                       ranges, n_tasks, P->keys_out, tn, P->tasks, P->tkeys, P->tidx);
+    tb = P->sort_tmp_bytes;
+    CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->tkeys, P->tkeys_out,
+                                                         P->tidx, P->task_order, (int)tn, 0, 32,
+                                                         stream));
+    hipLaunchKernelGGL(mix_persistent_kernel, dim3(P->grid), dim3(64), (size_t)P->lds_bytes, stream,
+                       b, band, band_scalar, me, me_scalar, sc, pen, P->gm, out, P->order, P->tasks,
+                       P->task_order, n_tasks, queue, P->lds_bytes / 4, P->slab, P->slab_words);
+    CGM_HIP(hipGetLastError());
+    // uncommon SW classes: LDS lanes, then one workgroup per pair
+    int cus = 0;
+    if (int rc = device_cu_count(&cus)) return rc;
+    if (int rc = launch_sw_lds_lane(b, band, band_scalar, sc, out, P->order, ranges + 2 * kMixSwLds,
+                                    cus * 2, stream))
+        return rc;
+    return launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
+                                 ranges + 2 * kMixSwPlain, cus * 2, stream);
+}
+
+int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
+                SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream) {
+    if (!stream) return set_error(CGM_ERR_INVALID, "graph capture needs a non-default stream");
+    if (P->exec) {
+        (void)hipGraphExecDestroy(P->exec);
+        P->exec = nullptr;
+    }
+    if (P->graph) {
+        (void)hipGraphDestroy(P->graph);
+        P->graph = nullptr;
+    }
+    CGM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = mix_enqueue(P, b, kind, kind_scalar, band, band_scalar, me, me_scalar, sc, pen, out,
+                         stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    CGM_HIP(e);
+    P->graph = g;
+    CGM_HIP(hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0));
+    return CGM_OK;
+}
+
+int mix_replay(MixPlan *P, hipStream_t stream) {
+    if (!P->exec) return set_error(CGM_ERR_INVALID, "no captured graph in this plan");
+    CGM_HIP(hipGraphLaunch(P->exec, stream));
+    return CGM_OK;
+}
+
+}  // namespace cgm

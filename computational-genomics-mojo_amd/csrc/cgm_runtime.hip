@@ -388,6 +388,80 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed, co
     }
 }
 
+int cgm_plan_create(int64_t max_pairs, int32_t max_error_cap, int32_t x, int32_t o, int32_t e,
+                    int32_t waves_per_cu, int32_t lds_bytes, void **plan) {
+    if (!plan) return set_error(CGM_ERR_INVALID, "null plan pointer");
+    MixPlan *P = nullptr;
+    int rc = mix_plan_create(max_pairs, max_error_cap, WfaPen{x, o, e},
+                             waves_per_cu > 0 ? waves_per_cu : kMixDefaultWavesPerCu,
+                             lds_bytes > 0 ? lds_bytes : kMixDefaultLdsBytes, &P);
+    if (rc) return rc;
+    *plan = P;
+    return CGM_OK;
+}
+
+int cgm_plan_destroy(void *plan) {
+    mix_plan_destroy((MixPlan *)plan);
+    return CGM_OK;
+}
+
+int cgm_align_mixed(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                    const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                    const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                    const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                    int32_t band_scalar, const int32_t *max_error, int32_t max_error_scalar,
+                    int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                    int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                    int32_t *out_r_end, int32_t layout, void *stream) {
+    Batch b = {q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words, r_words};
+    SwScoring sc = {match, mismatch, gap_open, gap_extend};
+    WfaPen p = {x, o, e};
+    SwOut out = {out_score, out_q_end, out_r_end};
+    if (n_pairs > 0 && (!out_score || !out_q_end || !out_r_end))
+        return set_error(CGM_ERR_INVALID, "null output array");
+    if (layout == CGM_LAYOUT_PLAIN) {
+        if (int rc = check_batch(b)) return rc;
+        if (int rc = check_sw_scoring(sc)) return rc;
+        if (int rc = check_wfa_pen(p)) return rc;
+        if (!kind) {
+            if (kind_scalar == 0)
+                return launch_sw_plain(b, band, band_scalar, sc, out, (hipStream_t)stream);
+            return launch_wfa_plain_kind(b, max_error, max_error_scalar, p, out_score, nullptr,
+                                         out_q_end, out_r_end, (hipStream_t)stream);
+        }
+        if (int rc = launch_sw_plain_kind(b, band, band_scalar, sc, out, kind, (hipStream_t)stream))
+            return rc;
+        return launch_wfa_plain_kind(b, max_error, max_error_scalar, p, out_score, kind, out_q_end,
+                                     out_r_end, (hipStream_t)stream);
+    }
+    if (!plan) return set_error(CGM_ERR_INVALID, "the automatic mixed layout needs a plan");
+    return mix_enqueue((MixPlan *)plan, b, kind, kind_scalar, band, band_scalar, max_error,
+                       max_error_scalar, sc, p, out, (hipStream_t)stream);
+}
+
+int cgm_plan_capture(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                     const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                     const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                     const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                     int32_t band_scalar, const int32_t *max_error, int32_t max_error_scalar,
+                     int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                     int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                     int32_t *out_r_end, void *stream) {
+    if (!plan) return set_error(CGM_ERR_INVALID, "null plan");
+    Batch b = {q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words, r_words};
+    SwScoring sc = {match, mismatch, gap_open, gap_extend};
+    SwOut out = {out_score, out_q_end, out_r_end};
+    if (n_pairs > 0 && (!out_score || !out_q_end || !out_r_end))
+        return set_error(CGM_ERR_INVALID, "null output array");
+    return mix_capture((MixPlan *)plan, b, kind, kind_scalar, band, band_scalar, max_error,
+                       max_error_scalar, sc, WfaPen{x, o, e}, out, (hipStream_t)stream);
+}
+
+int cgm_plan_replay(void *plan, void *stream) {
+    if (!plan) return set_error(CGM_ERR_INVALID, "null plan");
+    return mix_replay((MixPlan *)plan, (hipStream_t)stream);
+}
+
 int cgm_sw_banded_host(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_len,
                        const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
                        int64_t n_pairs, int64_t q_words, int64_t r_words, const int32_t *band,

@@ -136,13 +136,15 @@ __global__ __launch_bounds__(kPlainThreads) void sw_plain_kernel(Batch b, const 
                                                                  int32_t band_scalar, int32_t band_cap,
                                                                  SwScoring sc, SwOut out,
                                                                  const int32_t *order,
-                                                                 const int32_t *range) {
+                                                                 const int32_t *range,
+                                                                 const int32_t *kind) {
     extern __shared__ int32_t lds[];
     __shared__ int red_h[kPlainThreads], red_q[kPlainThreads], red_r[kPlainThreads];
     const int64_t start = range ? range[0] : 0;
     const int64_t end = range ? range[1] : b.n_pairs;
     for (int64_t item = start + blockIdx.x; item < end; item += gridDim.x) {
         const int64_t pair = order ? order[item] : item;
+        if (kind && kind[pair] != 0) continue;
         sw_plain_pair(b, pair, band_arr, band_scalar, band_cap, sc, out, lds, red_h, red_q, red_r);
     }
 }
@@ -247,6 +249,11 @@ __global__ __launch_bounds__(kLaneThreads) void sw_lane_kernel(Batch b, const in
 
 int launch_sw_plain(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                     SwOut out, hipStream_t stream) {
+    return launch_sw_plain_kind(b, band, band_scalar, sc, out, nullptr, stream);
+}
+
+int launch_sw_plain_kind(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                         SwOut out, const int32_t *kind, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
     if (int rc = check_sw_scoring(sc)) return rc;
     if (b.n_pairs == 0) return CGM_OK;
@@ -256,7 +263,7 @@ int launch_sw_plain(const Batch &b, const int32_t *band, int32_t band_scalar, Sw
     int32_t cap = max_band < kPlainMaxBand ? max_band : kPlainMaxBand;
     const size_t lds_bytes = (size_t)5 * (2 * cap + 1) * sizeof(int32_t);
     hipLaunchKernelGGL(sw_plain_kernel, dim3((unsigned)b.n_pairs), dim3(kPlainThreads), lds_bytes,
-                       stream, b, band, band_scalar, cap, sc, out, nullptr, nullptr);
+                       stream, b, band, band_scalar, cap, sc, out, nullptr, nullptr, kind);
     CGM_HIP(hipGetLastError());
     return CGM_OK;
 }
@@ -266,7 +273,7 @@ int launch_sw_plain_range(const Batch &b, const int32_t *band, int32_t band_scal
                           int grid, hipStream_t stream) {
     const size_t lds_bytes = (size_t)5 * (2 * kPlainMaxBand + 1) * sizeof(int32_t);
     hipLaunchKernelGGL(sw_plain_kernel, dim3(grid), dim3(kPlainThreads), lds_bytes, stream, b, band,
-                       band_scalar, kPlainMaxBand, sc, out, order, range);
+                       band_scalar, kPlainMaxBand, sc, out, order, range, nullptr);
     CGM_HIP(hipGetLastError());
     return CGM_OK;
 }

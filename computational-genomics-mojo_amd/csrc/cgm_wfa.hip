@@ -11,11 +11,13 @@
 // the target diagonal qlen - rlen and clamped to +-max_error, and the result
 // is the first s with M[s, qlen-rlen] >= qlen, or -1 past max_error.
 
-#include "cgm_internal.h"
+#include "cgm_wfa_dev.h"
 
 namespace cgm {
 
 namespace {
+
+using namespace wfadev;
 
 constexpr int kWfaNeg = -1000000;  // align_cpu.mojo:107
 
@@ -33,13 +35,18 @@ constexpr int kPlainThreads = 128;
 __global__ __launch_bounds__(kPlainThreads) void wfa_plain_kernel(Batch b, const int32_t *me_arr,
                                                                   int32_t me_scalar, int32_t me_cap,
                                                                   WfaPen p, int32_t *out,
-                                                                  int32_t *scratch, int64_t slab) {
+                                                                  int32_t *scratch, int64_t slab,
+                                                                  const int32_t *kind,
+                                                                  int32_t *q_end, int32_t *r_end) {
     __shared__ int found;
     int32_t *M = scratch + (int64_t)blockIdx.x * slab;
     const int tid = threadIdx.x;
     const int m_hist = max(p.x, p.o + p.e) + 1;
     const int ie_hist = p.e + 1;
     for (int64_t pair = blockIdx.x; pair < b.n_pairs; pair += gridDim.x) {
+        if (kind && kind[pair] != 1) continue;
+        if (tid == 0 && q_end) q_end[pair] = -1;
+        if (tid == 0 && r_end) r_end[pair] = -1;
         const uint64_t *q, *r;
         int qlen, rlen;
         const bool ok = load_pair(b, pair, q, r, qlen, rlen);
@@ -143,116 +150,6 @@ __global__ __launch_bounds__(kPlainThreads) void wfa_plain_kernel(Batch b, const
     }
 }
 
-// ---------------------------------------------------------------------------
-// Wave layout: one 64-lane wavefront per pair, persistent workgroups pulling
-// pair indices from an atomic queue (optionally through a work-sorted
-// order[]).  Exact reductions of the reference loop:
-//  * only scores that are multiples of g = gcd(x, o, e) can hold a wavefront
-//    (every recurrence step costs x, o+e or e), so the others are skipped;
-//  * diagonal k needs a gap opening and |k| extensions, so at score s only
-//    |k| <= R(s) = (s - o) / e (0 below o + e) can be reached; the range is
-//    [max(-rlen, -me, -R(s)), min(qlen, me, R(s))] and widening it to the
-//    target diagonal, as the reference does, only adds unreachable cells;
-//  * ranges only grow with s, so score slots are initialised once and never
-//    cleared: a slot's entries outside its range were never written;
-//  * NEXT and EXTEND fuse per diagonal: EXTEND of diagonal k touches only
-//    M[s,k] and NEXT never reads the current score.
-// Extension compares 32 bases per step: two funnel-shifted 64-bit windows
-// XORed, leading zeros / 2 = matching bases, capped at the sequence ends.
-// Wavefront slots live in LDS when 4 * slots * W bytes fit the block's LDS
-// budget, otherwise in a per-block HBM slab.
-// ---------------------------------------------------------------------------
-constexpr int kWaveThreads = 64;
-constexpr int kNegRow = -(1 << 30);
-
-__device__ __forceinline__ uint64_t window64(const uint64_t *s, int pos, int nwords) {
-    const int w = pos >> 5, sh = 2 * (pos & 31);
-    const uint64_t lo = s[w];
-    if (!sh) return lo;
-    const uint64_t hi = (w + 1 < nwords) ? s[w + 1] : 0ull;
-    return (lo << sh) | (hi >> (64 - sh));
-}
-
-// Furthest row reached from (row, row - k) following matching bases.
-__device__ __forceinline__ int extend_diag(const uint64_t *q, int qlen, int qw, const uint64_t *r,
-                                           int rlen, int rw, int row, int k) {
-    int col = row - k;
-    if (col < 0) return row;
-    while (row < qlen && col < rlen) {
-        const uint64_t x = window64(q, row, qw) ^ window64(r, col, rw);
-        const int n = x ? (__clzll(x) >> 1) : 32;
-        const int lim = min(qlen - row, rlen - col);
-        if (n >= lim) return row + lim;
-        row += n;
-        col += n;
-        if (n < 32) break;
-    }
-    return row;
-}
-
-struct WfaGeom {
-    int g, nM, nI;  // score step, M slots, I/D slots
-};
-
-__device__ __forceinline__ int reach(int s, const WfaPen &p) {
-    return s >= p.o + p.e ? (s - p.o) / p.e : 0;
-}
-
-template <typename Ptr>
-__device__ int wfa_pair_wave(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
-                             const WfaPen &p, const WfaGeom &gm, Ptr wf, int W, int K) {
-    const int lane = threadIdx.x;
-    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
-    const int slots = gm.nM + 2 * gm.nI;
-    for (int i = lane; i < slots * W; i += kWaveThreads) wf[i] = kNegRow;
-    __syncthreads();
-    Ptr M = wf;
-    Ptr I = wf + gm.nM * W;
-    Ptr D = I + gm.nI * W;
-    const int target = qlen - rlen;
-    if (lane == 0) M[K] = extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
-    __syncthreads();
-    if (target == 0 && M[K] >= qlen) return 0;
-    for (int s = gm.g; s <= me; s += gm.g) {
-        const int si = s / gm.g;
-        const int ms = (si % gm.nM) * W, is = (si % gm.nI) * W;
-        const int Rs = reach(s, p);
-        const int lo = max(max(-rlen, -me), -Rs), hi = min(min(qlen, me), Rs);
-        const int mx = s >= p.x ? (((s - p.x) / gm.g) % gm.nM) * W : -1;
-        const int mo = s >= p.o + p.e ? (((s - p.o - p.e) / gm.g) % gm.nM) * W : -1;
-        const int ie = s >= p.e ? (((s - p.e) / gm.g) % gm.nI) * W : -1;
-        bool found = false;
-        for (int k = lo + lane; k <= hi; k += kWaveThreads) {
-            const int di = k + K;
-            int iv = kNegRow, dv = kNegRow, mv = kNegRow;
-            if (mo >= 0) {
-                if (di > 0) iv = M[mo + di - 1] + 1;
-                if (di < W - 1) dv = M[mo + di + 1];
-            }
-            if (ie >= 0) {
-                if (di > 0) iv = max(iv, I[ie + di - 1] + 1);
-                if (di < W - 1) dv = max(dv, D[ie + di + 1]);
-            }
-            if (mx >= 0) mv = M[mx + di] + 1;
-            iv = iv < 0 ? kNegRow : iv;
-            dv = dv < 0 ? kNegRow : dv;
-            mv = max(max(mv, iv), dv);
-            I[is + di] = iv;
-            D[is + di] = dv;
-            if (mv >= 0) {
-                mv = extend_diag(q, qlen, qw, r, rlen, rw, mv, k);
-            } else {
-                mv = kNegRow;
-            }
-            M[ms + di] = mv;
-            found |= (k == target && mv >= qlen);
-        }
-        if (__any(found)) return s;
-        __syncthreads();
-    }
-    return -1;
-}
-
 __global__ __launch_bounds__(kWaveThreads) void wfa_wave_kernel(
     Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap, WfaPen p, WfaGeom gm,
     int32_t *out, const int32_t *order, int32_t *queue, int32_t lds_words, int32_t *slab,
@@ -305,6 +202,13 @@ __global__ void zero_i32_kernel(int32_t *p, int n) {
 
 int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
                      int32_t *out, hipStream_t stream) {
+    return launch_wfa_plain_kind(b, max_error, max_error_scalar, p, out, nullptr, nullptr, nullptr,
+                                 stream);
+}
+
+int launch_wfa_plain_kind(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                          WfaPen p, int32_t *out, const int32_t *kind, int32_t *q_end,
+                          int32_t *r_end, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
     if (int rc = check_wfa_pen(p)) return rc;
     if (b.n_pairs == 0) return CGM_OK;
@@ -323,7 +227,8 @@ int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error
     int32_t *scratch = nullptr;
     CGM_HIP(hipMallocAsync((void **)&scratch, (size_t)(grid * slab * sizeof(int32_t)), stream));
     hipLaunchKernelGGL(wfa_plain_kernel, dim3((unsigned)grid), dim3(kPlainThreads), 0, stream, b,
-                       max_error, max_error_scalar, me_max, p, out, scratch, slab);
+                       max_error, max_error_scalar, me_max, p, out, scratch, slab, kind, q_end,
+                       r_end);
     hipError_t le = hipGetLastError();
     CGM_HIP(hipFreeAsync(scratch, stream));
     CGM_HIP(le);

@@ -76,6 +76,24 @@ class Workload:
         return valid_band_cells(self.q_len, self.r_len, self.band)
 
 
+def concat(parts) -> "Workload":
+    """One workload holding the pairs of several, in order."""
+    def cat(name):
+        return np.concatenate([getattr(p, name) for p in parts])
+
+    def offs(name_off, name_len, packed):
+        out, base = [], 0
+        for p in parts:
+            out.append(getattr(p, name_off).astype(np.int64) + base)
+            base += getattr(p, packed).size
+        return np.concatenate(out).astype(np.int32)
+
+    return Workload(0, 0, cat("q_packed"), cat("q_nmask"), offs("q_off", "q_len", "q_packed"),
+                    cat("q_len"), cat("r_packed"), cat("r_nmask"),
+                    offs("r_off", "r_len", "r_packed"), cat("r_len"), cat("kind"), cat("band"),
+                    cat("max_error"))
+
+
 def valid_band_cells(q_len, r_len, band) -> np.ndarray:
     """sum_{qi<qlen} max(0, min(rlen-1, qi+band) - max(0, qi-band) + 1) per
     pair, in closed form (vectorised)."""

@@ -121,6 +121,44 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                           int32_t x, int32_t o, int32_t e,
                           int32_t *out_cost, int32_t layout, void *stream);
 
+/* ------------------------------------------------ mixed SW + WFA batches
+ * The batched persistent form of the two launchers above for a batch that
+ * mixes SW pairs (kind[i] == 0) and WFA pairs (kind[i] == 1): one call
+ * aligns every pair, writing the SW score/ends or the WFA cost (ends -1).
+ * No reference counterpart takes a mixed batch; per pair the results are
+ * those of launch_banded_sw / launch_wfa_affine (align_gpu.mojo:460-535),
+ * i.e. of smith_waterman_banded / wfa_affine_cpu.
+ *
+ * A plan owns every buffer the automatic layout needs (sort keys, task
+ * queue, HBM wavefront slabs for WFA pairs up to max_error_cap), so
+ * cgm_align_mixed performs no allocation and no host synchronisation and
+ * can be captured: cgm_plan_capture records one call into a hipGraph kept
+ * in the plan, cgm_plan_replay launches it again (same pointers; refill
+ * their contents between replays).  waves_per_cu / lds_bytes: 0 = tuned
+ * defaults.  A plan serves one device and one call at a time. */
+int cgm_plan_create(int64_t max_pairs, int32_t max_error_cap, int32_t x, int32_t o, int32_t e,
+                    int32_t waves_per_cu, int32_t lds_bytes, void **plan);
+int cgm_plan_destroy(void *plan);
+/* layout CGM_LAYOUT_AUTO: the persistent task-queue kernel (plan required);
+ * CGM_LAYOUT_PLAIN: one workgroup per pair for both kinds (plan ignored). */
+int cgm_align_mixed(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                    const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                    const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                    const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                    int32_t band_scalar, const int32_t *max_error, int32_t max_error_scalar,
+                    int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                    int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                    int32_t *out_r_end, int32_t layout, void *stream);
+int cgm_plan_capture(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                     const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                     const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                     const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                     int32_t band_scalar, const int32_t *max_error, int32_t max_error_scalar,
+                     int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                     int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                     int32_t *out_r_end, void *stream);
+int cgm_plan_replay(void *plan, void *stream);
+
 /* -------------------------------------------- host-pointer conveniences
  * Same as the launch entry points but every pointer is a host pointer; the
  * call copies in, runs on an internal stream and copies out before it

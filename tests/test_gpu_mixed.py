@@ -1,0 +1,105 @@
+"""GPU parity of the mixed SW + WFA batch path (one persistent kernel over a
+work-sorted task queue, plan workspace, hipGraph capture/replay) against the
+CPU oracle and against the one-workgroup-per-pair kernels."""
+import numpy as np
+import pytest
+
+from gpu_helpers import CONFIG_SW, Uploaded, assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gen_mod(cgm):
+    from cgm_amd import gen
+    return gen
+
+
+def mixed_sample(gen, seed=5):
+    """SW pairs of 100-300 bp (band 16-64), SW pairs of 1-2 kb (band 128) and
+    WFA pairs of 1-10 kb with small costs, interleaved."""
+    sw = gen.generate(2, 3000, seed=seed)
+    long_ = gen.generate(4, 3000, seed=seed)
+    long_ = long_.subset(np.nonzero((long_.kind == 0) & (long_.band == 128))[0][:300])
+    wfa = gen.generate(3, 600, seed=seed)
+    wfa = wfa.subset(np.argsort(wfa.max_error)[:250])
+    w = gen.concat([sw, long_, wfa])
+    perm = np.random.default_rng(seed).permutation(w.n_pairs)
+    return w.subset(perm)
+
+
+def oracle_mixed(orc, w):
+    return orc.batch(0, w.q_packed, w.r_packed, w.q_off, w.r_off, w.q_len, w.r_len,
+                     kind=w.kind, band=w.band, max_error=w.max_error, threads=8, **CONFIG_SW)
+
+
+def sw_cfg(cgm):
+    return cgm.AlignConfig(CONFIG_SW["match"], CONFIG_SW["mismatch"], CONFIG_SW["gap_open"],
+                           CONFIG_SW["gap_extend"], 0)
+
+
+@pytest.mark.parametrize("layout", ["plain", "auto"])
+def test_mixed_sample_parity(cgm, dev, orc, gen_mod, layout):
+    w = mixed_sample(gen_mod)
+    print(f"[case] mixed sample {w.n_pairs} pairs layout={layout}", flush=True)
+    got = cgm.align_mixed_batch(dev, w, sw_cfg(cgm), layout={"plain": 1, "auto": 0}[layout])
+    assert_same(got, oracle_mixed(orc, w), f"mixed {layout}")
+
+
+def test_mixed_config4_full_size(cgm, dev, orc, gen_mod):
+    """Config 4 (100,000 pairs incl. 5-20 kb WFA pairs): the persistent
+    layout equals the per-pair kernels on every pair; a sample of SW pairs
+    and of the cheaper WFA pairs equals the oracle."""
+    w = gen_mod.generate(4, 100_000)
+    print("[case] config4 full 100000 pairs", flush=True)
+    auto = cgm.align_mixed_batch(dev, w, sw_cfg(cgm), layout=0)
+    plain = cgm.align_mixed_batch(dev, w, sw_cfg(cgm), layout=1)
+    assert_same(auto, plain, "config4 persistent vs plain")
+    rng = np.random.default_rng(1)
+    sw_idx = rng.choice(np.nonzero(w.kind == 0)[0], 1500, replace=False)
+    wfa_idx = np.nonzero(w.kind == 1)[0]
+    wfa_idx = wfa_idx[np.argsort(w.max_error[wfa_idx])[:40]]
+    idx = np.concatenate([sw_idx, wfa_idx])
+    assert_same(tuple(a[idx] for a in auto), oracle_mixed(orc, w.subset(idx)), "config4 sample")
+
+
+def test_mixed_graph_replay_matches_eager(cgm, dev, gen_mod):
+    """Capture one mixed call into a hipGraph and replay it several times,
+    with eager calls and synchronisations in between; every replay must
+    reproduce the eager result (the queue counter and class counts are
+    cleared by kernels inside the graph, not by memset nodes)."""
+    w = mixed_sample(gen_mod, seed=9)
+    n = w.n_pairs
+    nz = lambda a: a if a.size else np.zeros(1, a.dtype)  # noqa: E731
+    bufs = tuple(dev.upload(nz(a)) for a in (w.q_packed, w.r_packed, w.q_len, w.r_len, w.q_off,
+                                             w.r_off))
+    kind, band, me = (dev.upload(a) for a in (w.kind, w.band, w.max_error))
+    plan = cgm.MixedPlan(n, int(w.max_error.max()))
+    eager_out = cgm.SwResult(n)
+    plan.align(dev, bufs, n, kind, band, me, sw_cfg(cgm), eager_out)
+    dev.synchronize()
+    eager = tuple(dev.download(a, n) for a in (eager_out.scores, eager_out.query_end,
+                                               eager_out.ref_end))
+    graph_out = cgm.SwResult(n)
+    plan.capture(dev, bufs, n, kind, band, me, sw_cfg(cgm), graph_out)
+    for rep in range(4):
+        fill = np.full(n, 12345, np.int32)
+        for a in (graph_out.scores, graph_out.query_end, graph_out.ref_end):
+            cgm._lib.call("cgm_memcpy_htod", a.dptr, fill.ctypes.data, fill.nbytes, dev.stream)
+        dev.synchronize()
+        plan.replay(dev)
+        dev.synchronize()
+        got = tuple(dev.download(a, n) for a in (graph_out.scores, graph_out.query_end,
+                                                 graph_out.ref_end))
+        assert_same(got, eager, f"graph replay {rep}")
+        # eager work between replays on the same plan
+        plan.align(dev, bufs, n, kind, band, me, sw_cfg(cgm), eager_out)
+        dev.synchronize()
+    plan.close()
+
+
+def test_mixed_is_deterministic(cgm, dev, gen_mod):
+    w = mixed_sample(gen_mod, seed=3)
+    a = cgm.align_mixed_batch(dev, w, sw_cfg(cgm))
+    b = cgm.align_mixed_batch(dev, w, sw_cfg(cgm))
+    assert_same(a, b, "run twice")
