@@ -110,11 +110,11 @@ int mix_replay(MixPlan *P, hipStream_t stream);
 
 // Defaults of the persistent mixed kernel (overridden by cgm_plan_create).
 constexpr int kMixDefaultWavesPerCu = 8;
-constexpr int kMixDefaultLdsBytes = 20 * 1024;
+constexpr int kMixDefaultLdsBytes = 64 * 1024;
 
-// WFA wave layout: LDS budget per 64-lane workgroup and workgroups per CU.
-constexpr int kWfaLdsBytes = 38 * 1024;
-constexpr int kWfaBlocksPerCu = 4;
+// WFA layout: LDS budget per 256-thread workgroup and workgroups per CU.
+constexpr int kWfaLdsBytes = 52 * 1024;
+constexpr int kWfaBlocksPerCu = 3;
 // Largest diagonal |k| reachable at score s: (s - o) / e from o + e on.
 inline int reach_host(int s, const WfaPen &p) { return s >= p.o + p.e ? (s - p.o) / p.e : 0; }
 

@@ -10,11 +10,12 @@
 //     2 WFA wavefront, 3 SW LDS lanes, 4 SW one-workgroup-per-pair, 5 bad;
 //     bad pairs get their outputs written here;
 //  2. radix-sort pair indices by (class, band, length) or (class, WFA work);
-//  3. build tasks: 64 consecutive SW pairs of one register class per task,
-//     one WFA pair per task, each with an estimated cost in wave-cycles;
+//  3. build tasks: 256 consecutive SW pairs of one register class per task
+//     (one 64-lane group per wavefront of the workgroup), one WFA pair per
+//     task (the whole workgroup), each with an estimated cost;
 //  4. radix-sort tasks by cost, largest first (longest-processing-time
 //     order, so the tail of the launch is made of small tasks);
-//  5. the persistent kernel: CUs x waves_per_cu one-wavefront workgroups
+//  5. the persistent kernel: CUs x waves_per_cu/4 four-wavefront workgroups
 //     take task indices from an atomic counter until the queue is empty;
 //  6. classes 3 and 4 (unusual scorings or bands) run the LDS-lane and
 //     one-workgroup-per-pair kernels over their ranges.
@@ -32,6 +33,10 @@ constexpr int kMixClasses = 6;
 constexpr int kMixSwReg = 0, kMixSwPacked = 1, kMixWfa = 2, kMixSwLds = 3, kMixSwPlain = 4,
               kMixBad = 5;
 constexpr int kPackedBand = 128;
+// One workgroup = four wavefronts: an SW task holds up to 4 x 64 pairs (one
+// lane-group per wavefront), a WFA task is one pair for the whole workgroup.
+constexpr int kMixThreads = 256;
+constexpr int kMixGroup = kMixThreads;
 
 enum TaskType : int32_t { kTaskSwReg = 0, kTaskSwPacked = 1, kTaskWfa = 2 };
 
@@ -112,7 +117,8 @@ __global__ void mix_ranges_kernel(const int32_t *counts, int32_t *ranges, int32_
         s += counts[c];
         ranges[2 * c + 1] = s;
     }
-    *n_tasks = (counts[kMixSwReg] + 63) / 64 + (counts[kMixSwPacked] + 63) / 64 + counts[kMixWfa];
+    *n_tasks = (counts[kMixSwReg] + kMixGroup - 1) / kMixGroup +
+               (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup + counts[kMixWfa];
 }
 
 __global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
@@ -122,7 +128,7 @@ __global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
     if (t >= cap) return;
     const int32_t c0 = ranges[2 * kMixSwReg + 1] - ranges[2 * kMixSwReg];
     const int32_t c1 = ranges[2 * kMixSwPacked + 1] - ranges[2 * kMixSwPacked];
-    const int64_t n0 = (c0 + 63) / 64, n1 = (c1 + 63) / 64;
+    const int64_t n0 = (c0 + kMixGroup - 1) / kMixGroup, n1 = (c1 + kMixGroup - 1) / kMixGroup;
     Task task = {0, 0, 0, 0};
     uint32_t key = 0;
     if (t < *n_tasks) {
@@ -133,11 +139,11 @@ __global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
             const int32_t base = packed ? ranges[2 * kMixSwPacked] : ranges[2 * kMixSwReg];
             const int32_t cnt = packed ? c1 : c0;
             task.type = packed ? kTaskSwPacked : kTaskSwReg;
-            task.start = base + (int32_t)(64 * g);
-            task.count = (int32_t)min((int64_t)64, cnt - 64 * g);
+            task.start = base + (int32_t)(kMixGroup * g);
+            task.count = (int32_t)min((int64_t)kMixGroup, cnt - kMixGroup * g);
             payload = sorted_keys[task.start] & 0x0fffffffu;
             const uint32_t band = payload >> 17, len = payload & 0x1ffffu;
-            // ~12 VALU ops x 4 cycles per wave-step of 64 cells, in units of 16 cycles
+            // ~10 VALU ops x 4 cycles per wave-step of 64 cells, units of 16 cycles
             key = (uint32_t)min((uint64_t)len * (2 * band + 1) * 3 + 1, (uint64_t)0xffffffffu);
         } else {
             task.type = kTaskWfa;
@@ -155,16 +161,18 @@ __global__ void mix_zero_kernel(int32_t *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-// The persistent kernel: one wavefront per workgroup, tasks in order.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void mix_persistent_kernel(
-    Batch b, const int32_t *band_arr, int32_t band_scalar, const int32_t *me_arr,
-    int32_t me_scalar, SwScoring sc, WfaPen p, wfadev::WfaGeom gm, SwOut out,
-    const int32_t *order, const Task *tasks, const int32_t *task_order, const int32_t *n_tasks,
-    int32_t *queue, int32_t lds_words, int32_t *slab, int64_t slab_words) {
-    extern __shared__ int32_t lds[];
+// The persistent kernel: four-wavefront workgroups take tasks in order.
+__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, const int32_t *me_arr,
+                      int32_t me_scalar, SwScoring sc, WfaPen p, wfadev::WfaGeom gm, SwOut out,
+                      const int32_t *order, const Task *tasks, const int32_t *task_order,
+                      const int32_t *n_tasks, int32_t *queue, int32_t lds_bytes, int32_t *slab,
+                      int64_t slab_words) {
+    extern __shared__ uint8_t lds_raw[];
     __shared__ int next_task;
     int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
     const int total = *n_tasks;
+    const int wave = threadIdx.x >> 6;
     for (;;) {
         if (threadIdx.x == 0) next_task = atomicAdd(queue, 1);
         __syncthreads();
@@ -172,34 +180,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         __syncthreads();
         if (t >= total) break;
         const Task task = tasks[task_order[t]];
-        if (task.type == kTaskSwReg) {
-            swdev::sw_reg_group<0, 64, false>(b, band_arr, band_scalar, sc, out, order, task.start,
-                                              task.count);
-        } else if (task.type == kTaskSwPacked) {
-            swdev::sw_reg_group<kPackedBand, kPackedBand, true>(b, band_arr, band_scalar, sc, out,
-                                                                order, task.start, task.count);
+        if (task.type == kTaskSwReg || task.type == kTaskSwPacked) {
+            const int64_t start = task.start + 64 * wave;
+            const int count = min(64, task.count - 64 * wave);
+            if (count > 0) {
+                if (task.type == kTaskSwReg)
+                    swdev::sw_reg_group<0, 64, false>(b, band_arr, band_scalar, sc, out, order,
+                                                      start, count);
+                else
+                    swdev::sw_reg_group<kPackedBand, kPackedBand, true>(
+                        b, band_arr, band_scalar, sc, out, order, start, count);
+            }
         } else {
             const int64_t pair = order[task.start];
             const uint64_t *q, *r;
             int qlen, rlen;
             load_pair(b, pair, q, r, qlen, rlen);  // validated by the classifier
             const int me = me_arr ? me_arr[pair] : me_scalar;
-            int res;
-            if (qlen == 0 && rlen == 0) {
-                res = 0;
-            } else {
-                const int target = qlen - rlen;
-                const int K = min(min(me, max(qlen, rlen)), wfadev::reach(me, p));
-                const int W = 2 * K + 1;
-                const int need = (gm.nM + 2 * gm.nI) * W;
-                if (target != 0 && (target > K || -target > K)) {
-                    res = -1;
-                } else if (need <= lds_words) {
-                    res = wfadev::wfa_pair_wave(q, qlen, r, rlen, me, p, gm, lds, W, K);
-                } else {
-                    res = wfadev::wfa_pair_wave(q, qlen, r, rlen, me, p, gm, my_slab, W, K);
-                }
-            }
+            const int res = wfadev::wfa_pair_block(q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes,
+                                                   my_slab, slab_words);
             if (threadIdx.x == 0) {
                 out.score[pair] = res;
                 if (out.q_end) out.q_end[pair] = -1;
@@ -256,14 +255,14 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
         return rc;
     };
     P->cap = max_pairs;
-    P->task_cap = max_pairs + (max_pairs + 63) / 64 + 1;
+    P->task_cap = max_pairs + (max_pairs + kMixGroup - 1) / kMixGroup + 1;
     P->me_cap = me_cap;
     P->pen = pen;
     wfa_geometry(pen, &P->gm.g, &P->gm.nM, &P->gm.nI);
     if (hipGetDevice(&P->device) != hipSuccess) return fail(set_error(CGM_ERR_HIP, "hipGetDevice"));
     int cus = 0;
     if (int rc = device_cu_count(&cus)) return fail(rc);
-    P->grid = cus * waves_per_cu;
+    P->grid = cus * ((waves_per_cu + 3) / 4);
     P->lds_bytes = lds_bytes & ~255;
     const int64_t n = max_pairs, nt = P->task_cap;
 #define CGM_ALLOC(ptr, bytes)                                                      \
@@ -292,18 +291,11 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
     // HBM wavefront slab per workgroup for WFA pairs too large for LDS.
     const int K = me_cap < reach_host(me_cap, pen) ? me_cap : reach_host(me_cap, pen);
     P->slab_words = (int64_t)(P->gm.nM + 2 * P->gm.nI) * (2 * (int64_t)K + 1);
-    if (P->slab_words * 4 > P->lds_bytes) {
-        CGM_ALLOC(P->slab, (int64_t)P->grid * P->slab_words * 4);
-    } else {
-        P->slab_words = 0;
-    }
+    CGM_ALLOC(P->slab, (int64_t)P->grid * P->slab_words * 4);
 #undef CGM_ALLOC
-    if (P->lds_bytes > 64 * 1024) {
-        if (hipFuncSetAttribute((const void *)mix_persistent_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                P->lds_bytes) != hipSuccess)
-            return fail(set_error(CGM_ERR_HIP, "plan: LDS attribute"));
-    }
+    if (hipFuncSetAttribute((const void *)mix_persistent_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, P->lds_bytes) != hipSuccess)
+        return fail(set_error(CGM_ERR_HIP, "plan: LDS attribute"));
     *out = P;
     return CGM_OK;
 }
@@ -334,7 +326,7 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     size_t tb = P->sort_tmp_bytes;
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->keys, P->keys_out,
                                                          P->idx, P->order, (int)n, 0, 32, stream));
-    const int64_t tasks_needed = n + (n + 63) / 64 + 1;
+    const int64_t tasks_needed = n + (n + kMixGroup - 1) / kMixGroup + 1;
     const int64_t tn = tasks_needed < nt ? tasks_needed : nt;
     hipLaunchKernelGGL(mix_tasks_kernel, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, stream,
                        ranges, n_tasks, P->keys_out, tn, P->tasks, P->tkeys, P->tidx);
@@ -342,9 +334,10 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->tkeys, P->tkeys_out,
                                                          P->tidx, P->task_order, (int)tn, 0, 32,
                                                          stream));
-    hipLaunchKernelGGL(mix_persistent_kernel, dim3(P->grid), dim3(64), (size_t)P->lds_bytes, stream,
-                       b, band, band_scalar, me, me_scalar, sc, pen, P->gm, out, P->order, P->tasks,
-                       P->task_order, n_tasks, queue, P->lds_bytes / 4, P->slab, P->slab_words);
+    hipLaunchKernelGGL(mix_persistent_kernel, dim3(P->grid), dim3(kMixThreads),
+                       (size_t)P->lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
+                       P->gm, out, P->order, P->tasks, P->task_order, n_tasks, queue,
+                       P->lds_bytes, P->slab, P->slab_words);
     CGM_HIP(hipGetLastError());
     // uncommon SW classes: LDS lanes, then one workgroup per pair
     int cus = 0;

@@ -150,11 +150,13 @@ __global__ __launch_bounds__(kPlainThreads) void wfa_plain_kernel(Batch b, const
     }
 }
 
-__global__ __launch_bounds__(kWaveThreads) void wfa_wave_kernel(
+constexpr int kTeamThreads = 256;
+
+__global__ __launch_bounds__(kTeamThreads) void wfa_wave_kernel(
     Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap, WfaPen p, WfaGeom gm,
-    int32_t *out, const int32_t *order, int32_t *queue, int32_t lds_words, int32_t *slab,
+    int32_t *out, const int32_t *order, int32_t *queue, int32_t lds_bytes, int32_t *slab,
     int64_t slab_words) {
-    extern __shared__ int32_t lds[];
+    extern __shared__ uint8_t lds_raw[];
     __shared__ int next_item;
     int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
     for (;;) {
@@ -168,30 +170,26 @@ __global__ __launch_bounds__(kWaveThreads) void wfa_wave_kernel(
         int qlen, rlen;
         const bool ok = load_pair(b, pair, q, r, qlen, rlen);
         const int me = me_arr ? me_arr[pair] : me_scalar;
-        int res;
-        if (!ok || me < 0 || me > me_cap) {
-            res = CGM_BAD_PAIR;
-        } else if (qlen == 0 && rlen == 0) {
-            res = 0;
-        } else {
-            const int target = qlen - rlen;
-            const int Rme = reach(me, p);
-            const int K = min(min(me, max(qlen, rlen)), Rme);
-            const int W = 2 * K + 1;
-            const int need = (gm.nM + 2 * gm.nI) * W;
-            if (target != 0 && (target > K || -target > K)) {
-                res = -1;  // the target diagonal is never reached within max_error
-            } else if (need <= lds_words) {
-                res = wfa_pair_wave(q, qlen, r, rlen, me, p, gm, lds, W, K);
-            } else if (need <= slab_words) {
-                res = wfa_pair_wave(q, qlen, r, rlen, me, p, gm, my_slab, W, K);
-            } else {
-                res = CGM_BAD_PAIR;
-            }
-        }
+        const int res = (!ok || me < 0 || me > me_cap)
+                            ? CGM_BAD_PAIR
+                            : wfa_pair_block(q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes,
+                                             my_slab, slab_words);
         if (threadIdx.x == 0) out[pair] = res;
         __syncthreads();
     }
+}
+
+// WFA work key for the longest-first order: sum over scores of the
+// diagonal count, from max_error (an upper bound of the cost).
+__global__ void wfa_key_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p, int g,
+                               uint32_t *keys, int32_t *idx) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= b.n_pairs) return;
+    const int me = me_arr ? me_arr[i] : me_scalar;
+    const int64_t R = me >= p.o + p.e ? (me - p.o) / p.e : 0;
+    const int64_t steps = (me > 0 ? me / g : 0) * (2 * R + 1);
+    keys[i] = (uint32_t)min(steps, (int64_t)0xffffffff);
+    idx[i] = (int32_t)i;
 }
 
 __global__ void zero_i32_kernel(int32_t *p, int n) {
@@ -270,25 +268,49 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
     const int slots = gm.nM + 2 * gm.nI;
     const int Kmax = me_max < reach_host(me_max, p) ? me_max : reach_host(me_max, p);
     const int64_t slab_words = (int64_t)slots * (2 * (int64_t)Kmax + 1);
-    const int32_t lds_words = kWfaLdsBytes / 4;
-    int dev = 0, cus = 0;
-    CGM_HIP(hipGetDevice(&dev));
-    CGM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int cus = 0;
+    if (int rc = device_cu_count(&cus)) return rc;
     int64_t grid = (int64_t)cus * kWfaBlocksPerCu;
     if (grid > b.n_pairs) grid = b.n_pairs;
-    // HBM slabs only for pairs whose wavefronts exceed the LDS budget.
-    const bool need_slab = slab_words > lds_words;
-    int32_t *slab = nullptr, *queue = nullptr;
+    // keep the HBM slabs within 4 GiB
+    while (grid > 1 && grid * slab_words * 4 > (int64_t(4) << 30)) grid /= 2;
+    static bool attr_set = false;
+    if (!attr_set) {
+        CGM_HIP(hipFuncSetAttribute((const void *)wfa_wave_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kWfaLdsBytes));
+        attr_set = true;
+    }
+    int32_t *slab = nullptr, *queue = nullptr, *own_order = nullptr;
+    uint32_t *keys = nullptr;
+    int32_t *idx = nullptr;
     CGM_HIP(hipMallocAsync((void **)&queue, 64, stream));
-    if (need_slab)
-        CGM_HIP(hipMallocAsync((void **)&slab, (size_t)(grid * slab_words * 4), stream));
+    CGM_HIP(hipMallocAsync((void **)&slab, (size_t)(grid * slab_words * 4), stream));
+    int rc = CGM_OK;
+    if (!order) {
+        // longest first: a late long pair would otherwise leave the GPU idle
+        CGM_HIP(hipMallocAsync((void **)&own_order, (size_t)b.n_pairs * 4, stream));
+        CGM_HIP(hipMallocAsync((void **)&keys, (size_t)b.n_pairs * 4, stream));
+        CGM_HIP(hipMallocAsync((void **)&idx, (size_t)b.n_pairs * 4, stream));
+        hipLaunchKernelGGL(wfa_key_kernel, dim3((unsigned)((b.n_pairs + 255) / 256)), dim3(256), 0,
+                           stream, b, max_error, max_error_scalar, p, gm.g, keys, idx);
+        rc = sort_desc_by_key(keys, idx, own_order, b.n_pairs, stream);
+        order = own_order;
+    }
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, queue, 16);
-    hipLaunchKernelGGL(wfa_wave_kernel, dim3((unsigned)grid), dim3(kWaveThreads),
-                       (size_t)kWfaLdsBytes, stream, b, max_error, max_error_scalar, me_max, p, gm,
-                       out, order, queue, lds_words, slab, need_slab ? slab_words : 0);
+    if (rc == CGM_OK) {
+        hipLaunchKernelGGL(wfa_wave_kernel, dim3((unsigned)grid), dim3(kTeamThreads),
+                           (size_t)kWfaLdsBytes, stream, b, max_error, max_error_scalar, me_max, p,
+                           gm, out, order, queue, kWfaLdsBytes, slab, slab_words);
+    }
     hipError_t le = hipGetLastError();
-    if (slab) CGM_HIP(hipFreeAsync(slab, stream));
+    CGM_HIP(hipFreeAsync(slab, stream));
     CGM_HIP(hipFreeAsync(queue, stream));
+    if (own_order) {
+        CGM_HIP(hipFreeAsync(own_order, stream));
+        CGM_HIP(hipFreeAsync(keys, stream));
+        CGM_HIP(hipFreeAsync(idx, stream));
+    }
+    if (rc) return rc;
     CGM_HIP(le);
     return CGM_OK;
 }

@@ -23,8 +23,9 @@ namespace wfadev {
 //    M[s,k] and NEXT never reads the current score.
 // Extension compares 32 bases per step: two funnel-shifted 64-bit windows
 // XORed, leading zeros / 2 = matching bases, capped at the sequence ends.
-// Wavefront slots live in LDS when 4 * slots * W bytes fit the block's LDS
-// budget, otherwise in a per-block HBM slab.
+// A whole workgroup works on one pair; sequences and int16 wavefronts sit
+// in LDS when they fit the workgroup's budget, otherwise the sequences are
+// read from global memory and int32 wavefronts live in an HBM slab.
 // ---------------------------------------------------------------------------
 constexpr int kWaveThreads = 64;
 constexpr int kNegRow = -(1 << 30);
@@ -62,21 +63,46 @@ __device__ __forceinline__ int reach(int s, const WfaPen &p) {
     return s >= p.o + p.e ? (s - p.o) / p.e : 0;
 }
 
-template <typename Ptr>
-__device__ int wfa_pair_wave(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
-                             const WfaPen &p, const WfaGeom &gm, Ptr wf, int W, int K) {
-    const int lane = threadIdx.x;
+// Storage of wavefront rows: int16 in LDS when every row fits (see
+// rows_fit_int16), int32 in an HBM slab otherwise.  Invalid entries hold the
+// type's most negative value; any stored value < 0 is invalid.
+template <typename T>
+struct RowNeg;
+template <>
+struct RowNeg<int32_t> {
+    static constexpr int32_t v = kNegRow;
+};
+template <>
+struct RowNeg<int16_t> {
+    static constexpr int16_t v = -32768;
+};
+
+// Rows grow past qlen only by +1 steps (a mismatch costs x, an insertion e),
+// so every row stays below qlen + me / min(x, e) + 2.
+__device__ __forceinline__ bool rows_fit_int16(int qlen, int me, const WfaPen &p) {
+    return (int64_t)qlen + me / min(p.x, p.e) + 2 <= 32767;
+}
+
+// One pair aligned by the whole workgroup (blockDim.x threads, any number
+// of wavefronts): threads stride over the diagonals of each score, and one
+// __syncthreads_or per score is both the barrier and the termination test.
+// q / r may point to LDS copies of the sequences or to global memory.
+template <typename T>
+__device__ int wfa_pair_team(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
+                             const WfaPen &p, const WfaGeom &gm, T *wf, int W, int K) {
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
     const int slots = gm.nM + 2 * gm.nI;
-    for (int i = lane; i < slots * W; i += kWaveThreads) wf[i] = kNegRow;
+    const T neg = RowNeg<T>::v;
+    for (int i = tid; i < slots * W; i += nt) wf[i] = neg;
     __syncthreads();
-    Ptr M = wf;
-    Ptr I = wf + gm.nM * W;
-    Ptr D = I + gm.nI * W;
+    T *M = wf;
+    T *I = wf + gm.nM * W;
+    T *D = I + gm.nI * W;
     const int target = qlen - rlen;
-    if (lane == 0) M[K] = extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+    if (tid == 0) M[K] = (T)extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
     __syncthreads();
-    if (target == 0 && M[K] >= qlen) return 0;
+    if (target == 0 && (int)M[K] >= qlen) return 0;
     for (int s = gm.g; s <= me; s += gm.g) {
         const int si = s / gm.g;
         const int ms = (si % gm.nM) * W, is = (si % gm.nI) * W;
@@ -85,36 +111,62 @@ __device__ int wfa_pair_wave(const uint64_t *q, int qlen, const uint64_t *r, int
         const int mx = s >= p.x ? (((s - p.x) / gm.g) % gm.nM) * W : -1;
         const int mo = s >= p.o + p.e ? (((s - p.o - p.e) / gm.g) % gm.nM) * W : -1;
         const int ie = s >= p.e ? (((s - p.e) / gm.g) % gm.nI) * W : -1;
-        bool found = false;
-        for (int k = lo + lane; k <= hi; k += kWaveThreads) {
+        int found = 0;
+        for (int k = lo + tid; k <= hi; k += nt) {
             const int di = k + K;
             int iv = kNegRow, dv = kNegRow, mv = kNegRow;
             if (mo >= 0) {
-                if (di > 0) iv = M[mo + di - 1] + 1;
-                if (di < W - 1) dv = M[mo + di + 1];
+                if (di > 0) iv = (int)M[mo + di - 1] + 1;
+                if (di < W - 1) dv = (int)M[mo + di + 1];
             }
             if (ie >= 0) {
-                if (di > 0) iv = max(iv, I[ie + di - 1] + 1);
-                if (di < W - 1) dv = max(dv, D[ie + di + 1]);
+                if (di > 0) iv = max(iv, (int)I[ie + di - 1] + 1);
+                if (di < W - 1) dv = max(dv, (int)D[ie + di + 1]);
             }
-            if (mx >= 0) mv = M[mx + di] + 1;
-            iv = iv < 0 ? kNegRow : iv;
-            dv = dv < 0 ? kNegRow : dv;
+            if (mx >= 0) mv = (int)M[mx + di] + 1;
             mv = max(max(mv, iv), dv);
-            I[is + di] = iv;
-            D[is + di] = dv;
+            I[is + di] = iv < 0 ? neg : (T)iv;
+            D[is + di] = dv < 0 ? neg : (T)dv;
             if (mv >= 0) {
                 mv = extend_diag(q, qlen, qw, r, rlen, rw, mv, k);
+                M[ms + di] = (T)mv;
+                found |= (k == target && mv >= qlen);
             } else {
-                mv = kNegRow;
+                M[ms + di] = neg;
             }
-            M[ms + di] = mv;
-            found |= (k == target && mv >= qlen);
         }
-        if (__any(found)) return s;
-        __syncthreads();
+        if (__syncthreads_or(found)) return s;
     }
     return -1;
+}
+
+// Everything one workgroup needs to align a WFA pair: sequences staged to
+// LDS and int16 wavefronts in LDS when both fit `lds_bytes`, otherwise the
+// sequences are read from global memory and the wavefronts live in the
+// workgroup's HBM slab (int32).  Returns the cost, -1, or CGM_BAD_PAIR.
+__device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, const uint64_t *rg,
+                                              int rlen, int me, const WfaPen &p,
+                                              const WfaGeom &gm, uint8_t *lds, int lds_bytes,
+                                              int32_t *slab, int64_t slab_words) {
+    if (qlen == 0 && rlen == 0) return 0;
+    const int target = qlen - rlen;
+    const int K = min(min(me, max(qlen, rlen)), reach(me, p));
+    const int W = 2 * K + 1;
+    if (target != 0 && (target > K || -target > K)) return -1;  // target never reached
+    const int slots = gm.nM + 2 * gm.nI;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int64_t seq_bytes = 8 * (int64_t)(qw + rw);
+    const int64_t wf16_bytes = 2 * (int64_t)slots * W;
+    if (rows_fit_int16(qlen, me, p) && seq_bytes + wf16_bytes <= lds_bytes) {
+        uint64_t *seq = (uint64_t *)lds;
+        for (int i = threadIdx.x; i < qw + rw; i += blockDim.x) seq[i] = i < qw ? qg[i] : rg[i - qw];
+        // the team function's first barrier orders these copies
+        return wfa_pair_team(seq, qlen, seq + qw, rlen, me, p, gm,
+                             (int16_t *)(lds + seq_bytes), W, K);
+    }
+    if ((int64_t)slots * W <= slab_words)
+        return wfa_pair_team(qg, qlen, rg, rlen, me, p, gm, slab, W, K);
+    return CGM_BAD_PAIR;
 }
 
 }  // namespace wfadev
