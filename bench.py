@@ -137,6 +137,31 @@ def mixed_extra(m, dev, gen, n_pairs=100_000):
     return res
 
 
+def serving_extra(m, duration=3.0):
+    """Serving loop on config-5 requests (1-64 pairs, log-normal lengths,
+    SW band 64 below 2 kb, WFA above): peak sustainable pairs/s in a closed
+    loop, then Poisson arrivals at 70% of that peak (p50/p99), then the
+    50,000 requests/s arrival rate of the workload definition."""
+    eng = m.ServingEngine(hbm_budget_bytes=16 << 30, n_slots=4, max_batch_pairs=16384,
+                          max_batch_words=1 << 19, max_error_cap=8192)
+    g = m.GeneratorHandle(5, 400_000, 11)
+    res = {"hbm_budget_bytes": 16 << 30, "device_bytes": eng.stats()["device_bytes"]}
+    eng.run_load(g.handle, 0.0, 0.5, max_outstanding=2048)  # warm-up
+    peak = eng.run_load(g.handle, 0.0, duration, max_outstanding=2048)
+    res["peak_closed_loop"] = peak
+    rate70 = 0.7 * peak["requests_per_s"]
+    res["at_70pct_peak"] = eng.run_load(g.handle, rate70, duration)
+    res["at_70pct_peak"]["offered_rps"] = rate70
+    res["at_50k_rps"] = eng.run_load(g.handle, 50_000.0, duration)
+    for k in ("peak_closed_loop", "at_70pct_peak", "at_50k_rps"):
+        r = res[k]
+        log(f"[bench] serving {k}: {r['requests_per_s']:.0f} req/s {r['pairs_per_s']:.0f} pairs/s "
+            f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms refused {r['refused']:.0f}")
+    g.close()
+    eng.close()
+    return res
+
+
 def time_steps(m, dev, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -198,6 +223,7 @@ def main():
     if not args.no_extras:
         extras["wfa_config3"] = wfa_extra(m, dev, gen)
         extras["mixed_config4"] = mixed_extra(m, dev, gen)
+        extras["serving_config5"] = serving_extra(m)
         wall_p, kms_p = time_steps(m, dev, lambda: run.launch(1), max(2, args.steps // 5), 1)
         plain_res = run.results()
         same = all(np.array_equal(a, b) for a, b in zip(plain_res, gpu_res))

@@ -15,6 +15,7 @@ from .align import (AlignConfig, AlignResult, MixedPlan, SwResult, WFAConfig, al
                     smith_waterman_banded, sw_banded_batch, wfa_affine, wfa_affine_batch,
                     wfa_affine_cpu)
 from .device import DeviceArray, GenomicsDevice, Timer
+from .serve import GeneratorHandle, ServingEngine
 from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_to_2bit,
                        get_base_from_word, get_view, pack_codes)
 
@@ -22,6 +23,6 @@ __all__ = [
     "AlignConfig", "AlignResult", "MixedPlan", "align_mixed_batch", "align_mixed_plain", "SwResult", "WFAConfig", "default_config",
     "default_wfa_config", "launch_banded_sw", "launch_wfa_affine", "smith_waterman_banded",
     "sw_banded_batch", "wfa_affine", "wfa_affine_batch", "wfa_affine_cpu", "DeviceArray",
-    "GenomicsDevice", "Timer", "BASES_PER_WORD", "Base", "SequenceBatch", "SequenceView",
+    "GenomicsDevice", "Timer", "GeneratorHandle", "ServingEngine", "BASES_PER_WORD", "Base", "SequenceBatch", "SequenceView",
     "ascii_to_2bit", "get_base_from_word", "get_view", "pack_codes",
 ]

@@ -68,6 +68,16 @@ SIGNATURES = {
     "cgm_plan_capture": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32,
                          _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
     "cgm_plan_replay": [_vp, _vp],
+    "cgm_engine_create": [_i64, _i32, _i64, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
+                          _pvp],
+    "cgm_engine_destroy": [_vp],
+    "cgm_engine_submit": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp,
+                          ctypes.c_double, _pi64],
+    "cgm_engine_poll": [_vp, _i64, _pi32],
+    "cgm_engine_wait": [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)],
+    "cgm_engine_stats": [_vp, _pi64],
+    "cgm_engine_now": [],
+    "cgm_serve_run": [_vp, _vp, ctypes.c_double, ctypes.c_double, _i32, _i32, _u64, _vp],
     "cgm_sw_banded_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                            _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32],
     "cgm_wfa_affine_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
@@ -77,7 +87,7 @@ SIGNATURES = {
     "cgm_gen_copy": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "cgm_gen_destroy": [_vp],
 }
-_RESTYPE = {"cgm_last_error": ctypes.c_char_p}
+_RESTYPE = {"cgm_last_error": ctypes.c_char_p, "cgm_engine_now": ctypes.c_double}
 
 _lib = None
 

@@ -100,6 +100,7 @@ struct MixPlan;
 int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per_cu,
                     int lds_bytes, MixPlan **out);
 void mix_plan_destroy(MixPlan *P);
+int64_t mix_plan_bytes(const MixPlan *P);
 int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                 SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream);

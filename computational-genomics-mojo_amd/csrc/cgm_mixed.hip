@@ -231,6 +231,7 @@ struct MixPlan {
     int64_t slab_words = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int64_t bytes = 0;  // device memory held by the plan
     ~MixPlan() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -265,10 +266,11 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
     P->grid = cus * ((waves_per_cu + 3) / 4);
     P->lds_bytes = lds_bytes & ~255;
     const int64_t n = max_pairs, nt = P->task_cap;
-#define CGM_ALLOC(ptr, bytes)                                                      \
+#define CGM_ALLOC(ptr, nbytes)                                                     \
     do {                                                                           \
-        hipError_t e_ = hipMalloc((void **)&(ptr), (size_t)(bytes));               \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (size_t)(nbytes));              \
         if (e_ != hipSuccess) return fail(hip_error(e_, "plan workspace"));        \
+        P->bytes += (int64_t)(nbytes);                                             \
     } while (0)
     CGM_ALLOC(P->keys, n * 4);
     CGM_ALLOC(P->keys_out, n * 4);
@@ -301,6 +303,8 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
 }
 
 void mix_plan_destroy(MixPlan *P) { delete P; }
+
+int64_t mix_plan_bytes(const MixPlan *P) { return P->bytes; }
 
 // Enqueues the whole mixed pipeline on `stream` (no allocation, no sync).
 int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,

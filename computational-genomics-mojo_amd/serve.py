@@ -1,0 +1,107 @@
+"""Serving engine: continuous batching of alignment requests on one GPU
+(cgm_engine_* in libcgm_align.so; see include/cgm_align.h)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+STAT_NAMES = ["submitted", "completed", "refused", "pairs_completed", "elapsed_s", "pairs_per_s",
+              "requests_per_s", "p50_ms", "p90_ms", "p99_ms", "max_ms", "mean_ms", "batches",
+              "mean_batch_pairs", "bad_results", "submit_phase_s"]
+
+
+class ServingEngine:
+    """n_slots preallocated batch slots within hbm_budget_bytes; one
+    dispatcher thread forms a batch whenever a slot is free."""
+
+    def __init__(self, hbm_budget_bytes: int = 8 << 30, n_slots: int = 4,
+                 max_batch_pairs: int = 16384, max_batch_words: int = 1 << 19,
+                 max_error_cap: int = 8192, match: int = 2, mismatch: int = -4,
+                 gap_open: int = -4, gap_extend: int = -2, x: int = 4, o: int = 6, e: int = 2):
+        h = ctypes.c_void_p()
+        _lib.call("cgm_engine_create", int(hbm_budget_bytes), n_slots, int(max_batch_pairs),
+                  int(max_batch_words), max_error_cap, match, mismatch, gap_open, gap_extend,
+                  x, o, e, ctypes.byref(h))
+        self.handle = h.value
+
+    def submit(self, q_packed, r_packed, q_len, r_len, q_off, r_off, kind, band, max_error,
+               arrival_s: float = 0.0) -> int:
+        """Queues one request; raises CgmError (code CGM_ERR_BUSY) when the
+        engine refuses it."""
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (q_len, r_len, q_off, r_off, kind,
+                                                             band, max_error)]
+        qp = np.ascontiguousarray(q_packed, np.uint64)
+        rp = np.ascontiguousarray(r_packed, np.uint64)
+        rid = ctypes.c_int64()
+        _lib.call("cgm_engine_submit", self.handle, _lib.ptr(qp), _lib.ptr(rp),
+                  *[_lib.ptr(a) for a in arrs[:4]], int(arrs[0].size),
+                  *[_lib.ptr(a) for a in arrs[4:]], float(arrival_s), ctypes.byref(rid))
+        return rid.value
+
+    def submit_pairs(self, w, idx, arrival_s: float = 0.0) -> int:
+        """Queues pairs idx of a gen.Workload as one request."""
+        idx = np.asarray(idx, np.int64)
+        return self.submit(w.q_packed, w.r_packed, w.q_len[idx], w.r_len[idx], w.q_off[idx],
+                           w.r_off[idx], w.kind[idx], w.band[idx], w.max_error[idx], arrival_s)
+
+    def wait(self, request_id: int, n_pairs: int, timeout_ms: int = -1):
+        """(score, query_end, ref_end, latency_ms) of a finished request."""
+        s = np.zeros(n_pairs, np.int32)
+        q = np.zeros(n_pairs, np.int32)
+        r = np.zeros(n_pairs, np.int32)
+        lat = ctypes.c_double()
+        _lib.call("cgm_engine_wait", self.handle, request_id, timeout_ms, _lib.ptr(s),
+                  _lib.ptr(q), _lib.ptr(r), ctypes.byref(lat))
+        return s, q, r, lat.value
+
+    def stats(self) -> dict:
+        out = np.zeros(8, np.int64)
+        _lib.call("cgm_engine_stats", self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        keys = ["batches", "batched_pairs", "done_requests", "done_pairs", "refused",
+                "queued_pairs", "device_bytes", "worker_error"]
+        return dict(zip(keys, out.tolist()))
+
+    def run_load(self, gen_handle, rate_rps: float, duration_s: float,
+                 max_pairs_per_request: int = 64, max_outstanding: int = 512,
+                 seed: int = 1) -> dict:
+        """Native load generator (cgm_serve_run): Poisson arrivals at
+        rate_rps (<= 0: closed loop with max_outstanding requests)."""
+        st = np.zeros(16, np.float64)
+        _lib.call("cgm_serve_run", self.handle, gen_handle, float(rate_rps), float(duration_s),
+                  max_pairs_per_request, max_outstanding, seed, _lib.ptr(st))
+        return dict(zip(STAT_NAMES, st.tolist()))
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().cgm_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GeneratorHandle:
+    """A batch generated inside the library (cgm_gen_create), kept as a
+    handle for the native load generator."""
+
+    def __init__(self, config: int, n_pairs: int, seed: int):
+        h = ctypes.c_void_p()
+        _lib.call("cgm_gen_create", config, int(n_pairs), seed, ctypes.byref(h))
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            _lib.load().cgm_gen_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

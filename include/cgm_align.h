@@ -159,6 +159,50 @@ int cgm_plan_capture(void *plan, const uint64_t *q_packed, const uint64_t *r_pac
                      int32_t *out_r_end, void *stream);
 int cgm_plan_replay(void *plan, void *stream);
 
+/* ------------------------------------------------------ serving engine
+ * Continuous batching for a stream of alignment requests (no reference
+ * counterpart; the reference aligns one pair per call).  The engine
+ * preallocates n_slots batch slots (device buffers, pinned staging, one
+ * mixed-kernel plan each) and fails with CGM_ERR_NOMEM if they exceed
+ * hbm_budget_bytes.  A dispatcher thread launches a new batch whenever a
+ * slot is free, taking every queued request that fits, so requests that
+ * arrive while others run are admitted as slots free up and their kernels
+ * overlap on the GPU.  cgm_engine_submit copies the request (pairs given as
+ * in the batch layout: kind/band/max_error per pair) and returns at once;
+ * it refuses with CGM_ERR_BUSY when the queued pairs would exceed
+ * n_slots * max_batch_pairs.  arrival_s (cgm_engine_now() clock, <= 0 =
+ * now) is the time latency is measured from.  Results do not depend on
+ * which requests share a batch. */
+int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_batch_pairs,
+                      int64_t max_batch_words, int32_t max_error_cap, int32_t match,
+                      int32_t mismatch, int32_t gap_open, int32_t gap_extend, int32_t x, int32_t o,
+                      int32_t e, void **engine);
+int cgm_engine_destroy(void *engine);
+int cgm_engine_submit(void *engine, const uint64_t *q_packed, const uint64_t *r_packed,
+                      const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                      const int32_t *r_off, int32_t n_pairs, const int32_t *kind,
+                      const int32_t *band, const int32_t *max_error, double arrival_s,
+                      int64_t *request_id);
+/* done = 1 once the request's results are ready. */
+int cgm_engine_poll(void *engine, int64_t request_id, int32_t *done);
+/* Blocks (timeout_ms < 0: forever) until the request is done, copies its
+ * n_pairs results (any pointer may be NULL) and releases the request. */
+int cgm_engine_wait(void *engine, int64_t request_id, int32_t timeout_ms, int32_t *score,
+                    int32_t *q_end, int32_t *r_end, double *latency_ms);
+/* out[8]: batches, batched pairs, finished requests, finished pairs,
+ * refused requests, queued pairs, device bytes held, last worker error. */
+int cgm_engine_stats(void *engine, int64_t *out);
+double cgm_engine_now(void);
+/* Load generator: requests of U{1..max_pairs_per_request} consecutive pairs
+ * of a generated batch (cgm_gen_create), Poisson arrivals at rate_rps for
+ * duration_s (rate_rps <= 0: closed loop with max_outstanding requests in
+ * flight).  stats[16]: submitted, completed, refused, pairs completed,
+ * elapsed s, pairs/s, requests/s, p50, p90, p99, max, mean latency (ms),
+ * batches, mean pairs per batch, bad results, submit phase s. */
+int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
+                  int32_t max_pairs_per_request, int32_t max_outstanding, uint64_t seed,
+                  double *stats);
+
 /* -------------------------------------------- host-pointer conveniences
  * Same as the launch entry points but every pointer is a host pointer; the
  * call copies in, runs on an internal stream and copies out before it

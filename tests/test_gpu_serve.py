@@ -1,0 +1,121 @@
+"""Serving engine on the GPU: results equal the oracle, a request's result
+is the same alone and in a mix, admission back-pressure, the HBM budget and
+the native load generator."""
+import numpy as np
+import pytest
+
+from gpu_helpers import CONFIG_SW, assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gen_mod(cgm):
+    from cgm_amd import gen
+    return gen
+
+
+@pytest.fixture(scope="module")
+def serving_pairs(gen_mod):
+    """Config-5 pairs (SW band 64 below 2 kb, WFA above), WFA pairs limited
+    to max_error <= 2500 so the oracle check stays short."""
+    w = gen_mod.generate(5, 6000, seed=17)
+    keep = np.nonzero((w.kind == 0) | (w.max_error <= 2500))[0]
+    return w.subset(keep)
+
+
+def oracle(orc, w):
+    return orc.batch(0, w.q_packed, w.r_packed, w.q_off, w.r_off, w.q_len, w.r_len, kind=w.kind,
+                     band=w.band, max_error=w.max_error, threads=8, **CONFIG_SW)
+
+
+def requests_of(n_pairs, rng, max_size=64):
+    out, i = [], 0
+    while i < n_pairs:
+        k = int(rng.integers(1, max_size + 1))
+        out.append(np.arange(i, min(i + k, n_pairs)))
+        i += k
+    return out
+
+
+def test_engine_results_equal_oracle(cgm, dev, orc, serving_pairs):
+    w = serving_pairs
+    assert (w.kind == 1).any() and (w.kind == 0).any()
+    eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=3, max_batch_pairs=2048,
+                            max_batch_words=1 << 17, max_error_cap=int(w.max_error.max()))
+    reqs = requests_of(w.n_pairs, np.random.default_rng(0))
+    ids = [eng.submit_pairs(w, idx) for idx in reqs]
+    got = [np.zeros(w.n_pairs, np.int32) for _ in range(3)]
+    for idx, rid in zip(reqs, ids):
+        s, q, r, lat = eng.wait(rid, idx.size)
+        assert lat >= 0
+        got[0][idx], got[1][idx], got[2][idx] = s, q, r
+    st = eng.stats()
+    print(f"[case] engine {len(reqs)} requests, stats {st}", flush=True)
+    assert st["done_requests"] == len(reqs) and st["worker_error"] == 0
+    assert_same(tuple(got), oracle(orc, w), "engine vs oracle")
+    eng.close()
+
+
+def test_request_alone_equals_in_mix(cgm, dev, serving_pairs):
+    w = serving_pairs
+    eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=4, max_batch_pairs=4096,
+                            max_batch_words=1 << 17, max_error_cap=int(w.max_error.max()))
+    probe = np.arange(100, 164)
+    rid = eng.submit_pairs(w, probe)
+    alone = eng.wait(rid, probe.size)[:3]
+    # the same request in the middle of a burst of others
+    rng = np.random.default_rng(5)
+    others = requests_of(w.n_pairs, rng)
+    ids = [(idx, eng.submit_pairs(w, idx)) for idx in others[: len(others) // 2]]
+    rid2 = eng.submit_pairs(w, probe)
+    ids += [(idx, eng.submit_pairs(w, idx)) for idx in others[len(others) // 2:]]
+    mixed = eng.wait(rid2, probe.size)[:3]
+    for idx, i in ids:
+        eng.wait(i, idx.size)
+    assert eng.stats()["batches"] >= 2
+    assert_same(mixed, alone, "alone vs mix")
+    eng.close()
+
+
+def test_admission_back_pressure(cgm, dev, serving_pairs):
+    w = serving_pairs
+    eng = cgm.ServingEngine(hbm_budget_bytes=4 << 30, n_slots=1, max_batch_pairs=64,
+                            max_batch_words=1 << 15, max_error_cap=int(w.max_error.max()))
+    accepted, refused = [], 0
+    for k in range(400):
+        idx = np.arange((k * 32) % (w.n_pairs - 32), (k * 32) % (w.n_pairs - 32) + 32)
+        try:
+            accepted.append((idx, eng.submit_pairs(w, idx)))
+        except cgm._lib.CgmError as exc:
+            assert exc.code == cgm._lib.CGM_ERR_BUSY
+            refused += 1
+    for idx, rid in accepted:
+        s = eng.wait(rid, idx.size)[0]
+        assert (s != cgm._lib.CGM_BAD_PAIR).all()
+    print(f"[case] back-pressure: accepted {len(accepted)} refused {refused}", flush=True)
+    assert refused > 0 and len(accepted) > 0
+    assert eng.stats()["refused"] == refused
+    eng.close()
+
+
+def test_engine_respects_hbm_budget(cgm, dev):
+    with pytest.raises(cgm._lib.CgmError) as ei:
+        cgm.ServingEngine(hbm_budget_bytes=1 << 20, n_slots=2, max_batch_pairs=4096,
+                          max_batch_words=1 << 16, max_error_cap=4096)
+    assert ei.value.code == cgm._lib.CGM_ERR_NOMEM
+
+
+def test_native_load_generator(cgm, dev):
+    eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=4, max_batch_pairs=8192,
+                            max_batch_words=1 << 18, max_error_cap=8192)
+    g = cgm.GeneratorHandle(5, 20000, 3)
+    closed = eng.run_load(g.handle, 0.0, 0.5, max_outstanding=256)
+    print(f"[case] closed loop {closed}", flush=True)
+    assert closed["completed"] > 0 and closed["bad_results"] == 0
+    opened = eng.run_load(g.handle, 2000.0, 0.5)
+    print(f"[case] open loop 2000 rps {opened}", flush=True)
+    assert opened["completed"] > 0 and opened["bad_results"] == 0
+    assert 0 < opened["p50_ms"] <= opened["p99_ms"]
+    g.close()
+    eng.close()
