@@ -30,6 +30,10 @@ PKG_DIR = os.path.join(ROOT, "computational-genomics-mojo_amd")
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # int32 lane-ops/s, MI355X spec
 HBM_PEAK_GBS = 8000.0
 SW_OPS_PER_CELL = 11
+# HBM bytes per launch of the persistent kernel on this workload, from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (87999.0 KiB x 2.0 + 20849.8 KiB x
+# 1.0, the scales measured on a 256 MiB device copy).
+TRAFFIC_BYTES = int((87999.0 * 2.0 + 20849.8) * 1024)
 SW = dict(match=2, mismatch=-4, gap_open=-4, gap_extend=-2)
 
 
@@ -187,7 +191,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip plain / WFA extras")
     ap.add_argument("--pairs", type=int, default=200_000)
+    ap.add_argument("--sw-path", choices=["persistent", "buckets"], default="persistent",
+                    help="persistent: the task-queue kernel (the automatic layout); buckets: "
+                         "one register kernel per band bucket")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many bytes (counter scale)")
     args = ap.parse_args()
+    if args.copy_probe:
+        m = load_package()
+        dev = m.GenomicsDevice(0)
+        a = m.DeviceArray(args.copy_probe, np.uint8)
+        b = m.DeviceArray(args.copy_probe, np.uint8)
+        for _ in range(3):
+            m._lib.call("cgm_memcpy_dtod", b.dptr, a.dptr, args.copy_probe, dev.stream)
+        dev.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(pairs are independent: shard a batch across GPUs by calling the "
@@ -207,17 +226,34 @@ def main():
     log(f"[bench] generated {w.n_pairs} pairs, {total_cells / 1e9:.3f} G valid cells "
         f"in {time.perf_counter() - t:.1f}s")
     run = SwRunner(m, dev, w)
+    plan = None
+    if args.sw_path == "buckets" or args.layout != 0:
+        step = lambda: run.launch(args.layout if args.sw_path == "persistent" else 2)  # noqa: E731
+    else:
+        # the automatic layout's path with an explicit plan, so the persistent
+        # kernel's own duration can be read from the plan's events
+        plan = m.MixedPlan(w.n_pairs, 0)
+        bufs = (run.q, run.r, run.ql, run.rl, run.qo, run.ro)
+        step = lambda: plan.align(dev, bufs, w.n_pairs, None, run.band, None,  # noqa: E731
+                                  run.cfg, run.out)
 
-    wall, kms = time_steps(m, dev, lambda: run.launch(args.layout), args.steps, args.warmup)
+    wall, kms = time_steps(m, dev, step, args.steps, args.warmup)
+    kernel_ms = []
+    if plan is not None:
+        # the same steps again, each followed by a read of the kernel's events
+        for _ in range(args.steps):
+            step()
+            kernel_ms.append(plan.kernel_ms())
     gpu_res = run.results()
     ms_step = wall / args.steps * 1e3
     gcups = total_cells * args.steps / wall / 1e9
-    kmean = float(np.mean(kms))
+    step_ms = float(np.mean(kms))
+    kmean = float(np.mean(kernel_ms)) if kernel_ms else step_ms
     achieved_tops = SW_OPS_PER_CELL * total_cells / (kmean * 1e-3) / 1e12
     in_bytes = int(w.q_packed.nbytes + w.r_packed.nbytes + 5 * 4 * w.n_pairs)
     out_bytes = 12 * w.n_pairs
-    log(f"[bench] layout {args.layout}: {ms_step:.3f} ms/step, {gcups:.1f} GCUPS, "
-        f"kernel {kmean:.3f} ms")
+    log(f"[bench] layout {args.layout} {args.sw_path}: {ms_step:.3f} ms/step, {gcups:.1f} GCUPS, "
+        f"step events {step_ms:.3f} ms, persistent kernel {kmean:.3f} ms")
 
     extras = {}
     if not args.no_extras:
@@ -275,10 +311,12 @@ def main():
         "config": {"workload": "batched banded SW: 200,000 pairs, targets 100-300 bp, "
                                "divergence 1-5%, 0.1% N, band 16-64 per pair, 2/-4/-4/-2",
                    "pairs": w.n_pairs, "valid_cells": total_cells, "band_area_cells": band_area,
-                   "layout": args.layout},
+                   "layout": args.layout, "sw_path": args.sw_path},
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": VALU_PEAK_TOPS,
                      "unit": "TOP/s (int32)", "frac": achieved_tops / VALU_PEAK_TOPS,
-                     "traffic": None, "kernel_ms": kmean,
+                     "traffic": TRAFFIC_BYTES if plan is not None else None,
+                     "kernel": "mix_persistent_kernel" if plan is not None else "step",
+                     "kernel_ms": kmean, "step_ms": step_ms,
                      "ops_per_cell": SW_OPS_PER_CELL,
                      "hbm_algorithmic_bytes": in_bytes + out_bytes,
                      "hbm_frac": (in_bytes + out_bytes) / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS},

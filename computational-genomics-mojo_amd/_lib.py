@@ -47,6 +47,7 @@ SIGNATURES = {
     "cgm_host_free": [_vp],
     "cgm_memcpy_htod": [_vp, _vp, _i64, _vp],
     "cgm_memcpy_dtoh": [_vp, _vp, _i64, _vp],
+    "cgm_memcpy_dtod": [_vp, _vp, _i64, _vp],
     "cgm_stream_create": [_pvp],
     "cgm_stream_destroy": [_vp],
     "cgm_stream_synchronize": [_vp],
@@ -68,6 +69,7 @@ SIGNATURES = {
     "cgm_plan_capture": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32,
                          _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
     "cgm_plan_replay": [_vp, _vp],
+    "cgm_plan_kernel_ms": [_vp, _pf],
     "cgm_engine_create": [_i64, _i32, _i64, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                           _pvp],
     "cgm_engine_destroy": [_vp],

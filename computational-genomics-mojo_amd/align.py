@@ -227,6 +227,13 @@ class MixedPlan:
     def replay(self, device: GenomicsDevice) -> None:
         _lib.call("cgm_plan_replay", self.handle, device.stream)
 
+    def kernel_ms(self) -> float:
+        """Duration of the persistent kernel of the last call or replay."""
+        import ctypes
+        ms = ctypes.c_float()
+        _lib.call("cgm_plan_kernel_ms", self.handle, ctypes.byref(ms))
+        return float(ms.value)
+
     def close(self) -> None:
         if self.handle:
             _lib.load().cgm_plan_destroy(self.handle)

@@ -101,6 +101,7 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
                     int lds_bytes, MixPlan **out);
 void mix_plan_destroy(MixPlan *P);
 int64_t mix_plan_bytes(const MixPlan *P);
+int mix_plan_kernel_ms(MixPlan *P, float *ms);
 int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                 SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream);
@@ -108,6 +109,11 @@ int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                 SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream);
 int mix_replay(MixPlan *P, hipStream_t stream);
+// A plan owned by the library for the drop-in launchers, one per (device,
+// stream), recreated larger when a batch exceeds it (the old plan is freed
+// after the stream drains).
+int cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen, MixPlan **out);
+int64_t mix_plan_capacity(const MixPlan *P, int32_t *me_cap, WfaPen *pen);
 
 // Defaults of the persistent mixed kernel (overridden by cgm_plan_create).
 constexpr int kMixDefaultWavesPerCu = 8;
@@ -164,6 +170,8 @@ int launch_sw_plain_range(const Batch &b, const int32_t *band, int32_t band_scal
                           int grid, hipStream_t stream);
 int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                    SwOut out, hipStream_t stream);
+int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                      SwOut out, hipStream_t stream);
 int device_cu_count(int *cus);
 
 // Fork/join of auxiliary streams (default priority, non-blocking) so that

@@ -20,6 +20,9 @@
 //  6. classes 3 and 4 (unusual scorings or bands) run the LDS-lane and
 //     one-workgroup-per-pair kernels over their ranges.
 
+#include <map>
+#include <mutex>
+
 #include <hipcub/hipcub.hpp>
 
 #include "cgm_sw_dev.h"
@@ -232,7 +235,10 @@ struct MixPlan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t bytes = 0;  // device memory held by the plan
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // around the persistent kernel
     ~MixPlan() {
+        if (ev_start) (void)hipEventDestroy(ev_start);
+        if (ev_stop) (void)hipEventDestroy(ev_stop);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         void *ptrs[] = {keys, keys_out, tkeys, tkeys_out, idx, order, tidx, task_order, small,
@@ -298,6 +304,8 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
     if (hipFuncSetAttribute((const void *)mix_persistent_kernel,
                             hipFuncAttributeMaxDynamicSharedMemorySize, P->lds_bytes) != hipSuccess)
         return fail(set_error(CGM_ERR_HIP, "plan: LDS attribute"));
+    if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess)
+        return fail(set_error(CGM_ERR_HIP, "plan: timing events"));
     *out = P;
     return CGM_OK;
 }
@@ -305,6 +313,60 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
 void mix_plan_destroy(MixPlan *P) { delete P; }
 
 int64_t mix_plan_bytes(const MixPlan *P) { return P->bytes; }
+
+int mix_plan_kernel_ms(MixPlan *P, float *ms) {
+    CGM_HIP(hipEventSynchronize(P->ev_stop));
+    CGM_HIP(hipEventElapsedTime(ms, P->ev_start, P->ev_stop));
+    return CGM_OK;
+}
+
+int64_t mix_plan_capacity(const MixPlan *P, int32_t *me_cap, WfaPen *pen) {
+    if (me_cap) *me_cap = P->me_cap;
+    if (pen) *pen = P->pen;
+    return P->cap;
+}
+
+namespace {
+struct PlanKey {
+    int device;
+    hipStream_t stream;
+    bool operator<(const PlanKey &o) const {
+        return device != o.device ? device < o.device : stream < o.stream;
+    }
+};
+std::mutex g_plan_mu;
+std::map<PlanKey, MixPlan *> g_plans;
+}  // namespace
+
+int cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen, MixPlan **out) {
+    int dev = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    MixPlan *&slot = g_plans[PlanKey{dev, stream}];
+    if (slot) {
+        int32_t cap_me = 0;
+        WfaPen cap_pen{};
+        const int64_t cap = mix_plan_capacity(slot, &cap_me, &cap_pen);
+        if (cap >= n_pairs && cap_me >= me_cap && cap_pen.x == pen.x && cap_pen.o == pen.o &&
+            cap_pen.e == pen.e) {
+            *out = slot;
+            return CGM_OK;
+        }
+        CGM_HIP(hipStreamSynchronize(stream));
+        mix_plan_destroy(slot);
+        slot = nullptr;
+    }
+    // grow geometrically so a slowly growing batch size does not rebuild often
+    int64_t cap = 1024;
+    while (cap < n_pairs) cap *= 2;
+    int rc = mix_plan_create(cap, me_cap, pen, kMixDefaultWavesPerCu, kMixDefaultLdsBytes, &slot);
+    if (rc) {
+        slot = nullptr;
+        return rc;
+    }
+    *out = slot;
+    return CGM_OK;
+}
 
 // Enqueues the whole mixed pipeline on `stream` (no allocation, no sync).
 int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
@@ -338,11 +400,13 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->tkeys, P->tkeys_out,
                                                          P->tidx, P->task_order, (int)tn, 0, 32,
                                                          stream));
+    CGM_HIP(hipEventRecord(P->ev_start, stream));
     hipLaunchKernelGGL(mix_persistent_kernel, dim3(P->grid), dim3(kMixThreads),
                        (size_t)P->lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
                        P->gm, out, P->order, P->tasks, P->task_order, n_tasks, queue,
                        P->lds_bytes, P->slab, P->slab_words);
     CGM_HIP(hipGetLastError());
+    CGM_HIP(hipEventRecord(P->ev_stop, stream));
     // uncommon SW classes: LDS lanes, then one workgroup per pair
     int cus = 0;
     if (int rc = device_cu_count(&cus)) return rc;

@@ -189,6 +189,25 @@ int join_streams(hipStream_t main, int n, hipStream_t *aux) {
     return CGM_OK;
 }
 
+// The host-pointer entry points run on one library stream per device and
+// hold that device's lock for the whole (synchronous) call, so concurrent
+// host calls cannot interleave on the stream's cached plan.
+namespace {
+std::mutex g_host_mu[kMaxDevices];
+hipStream_t g_host_stream[kMaxDevices] = {};
+}  // namespace
+
+static int host_stream(hipStream_t *s, std::unique_lock<std::mutex> *lock) {
+    int dev = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(CGM_ERR_INVALID, "device index too large");
+    *lock = std::unique_lock<std::mutex>(g_host_mu[dev]);
+    if (!g_host_stream[dev])
+        CGM_HIP(hipStreamCreateWithFlags(&g_host_stream[dev], hipStreamNonBlocking));
+    *s = g_host_stream[dev];
+    return CGM_OK;
+}
+
 // A device copy of a host batch (for the host-pointer entry points).
 struct DeviceBatch {
     std::vector<void *> allocs;
@@ -290,6 +309,12 @@ int cgm_memcpy_dtoh(void *dst, const void *src, int64_t bytes, void *stream) {
     return CGM_OK;
 }
 
+int cgm_memcpy_dtod(void *dst, const void *src, int64_t bytes, void *stream) {
+    if (bytes <= 0) return CGM_OK;
+    CGM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return CGM_OK;
+}
+
 int cgm_stream_create(void **stream) {
     if (!stream) return set_error(CGM_ERR_INVALID, "null stream pointer");
     hipStream_t s;
@@ -358,8 +383,9 @@ int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed, con
         case CGM_LAYOUT_PLAIN:
             return launch_sw_plain(b, band, band_scalar, sc, out, (hipStream_t)stream);
         case CGM_LAYOUT_AUTO:
-        case CGM_LAYOUT_LANE:
             return launch_sw_auto(b, band, band_scalar, sc, out, (hipStream_t)stream);
+        case CGM_LAYOUT_LANE:
+            return launch_sw_buckets(b, band, band_scalar, sc, out, (hipStream_t)stream);
         default:
             return set_error(CGM_ERR_INVALID, "unknown SW layout");
     }
@@ -457,6 +483,11 @@ int cgm_plan_capture(void *plan, const uint64_t *q_packed, const uint64_t *r_pac
                        max_error_scalar, sc, WfaPen{x, o, e}, out, (hipStream_t)stream);
 }
 
+int cgm_plan_kernel_ms(void *plan, float *ms) {
+    if (!plan || !ms) return set_error(CGM_ERR_INVALID, "null argument");
+    return mix_plan_kernel_ms((MixPlan *)plan, ms);
+}
+
 int cgm_plan_replay(void *plan, void *stream) {
     if (!plan) return set_error(CGM_ERR_INVALID, "null plan");
     return mix_replay((MixPlan *)plan, (hipStream_t)stream);
@@ -471,13 +502,11 @@ int cgm_sw_banded_host(const uint64_t *q_packed, const uint64_t *r_packed, const
     if (n_pairs < 0) return set_error(CGM_ERR_INVALID, "n_pairs < 0");
     if (n_pairs == 0) return CGM_OK;
     DeviceBatch db;
-    CGM_HIP(hipStreamCreateWithFlags(&db.stream, hipStreamNonBlocking));
+    std::unique_lock<std::mutex> host_lock;
+    if (int rc = host_stream(&db.stream, &host_lock)) return rc;
     struct StreamGuard {
         hipStream_t s;
-        ~StreamGuard() {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
+        ~StreamGuard() { (void)hipStreamSynchronize(s); }
     } guard{db.stream};
     uint64_t *qp, *rp;
     int32_t *ql, *rl, *qo, *ro, *bd = nullptr, *os, *oq, *orr;
@@ -509,13 +538,11 @@ int cgm_wfa_affine_host(const uint64_t *q_packed, const uint64_t *r_packed, cons
     if (n_pairs < 0) return set_error(CGM_ERR_INVALID, "n_pairs < 0");
     if (n_pairs == 0) return CGM_OK;
     DeviceBatch db;
-    CGM_HIP(hipStreamCreateWithFlags(&db.stream, hipStreamNonBlocking));
+    std::unique_lock<std::mutex> host_lock;
+    if (int rc = host_stream(&db.stream, &host_lock)) return rc;
     struct StreamGuard {
         hipStream_t s;
-        ~StreamGuard() {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
+        ~StreamGuard() { (void)hipStreamSynchronize(s); }
     } guard{db.stream};
     uint64_t *qp, *rp;
     int32_t *ql, *rl, *qo, *ro, *me = nullptr, *oc;

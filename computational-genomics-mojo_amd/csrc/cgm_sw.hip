@@ -309,10 +309,11 @@ int device_cu_count(int *cus) {
     return CGM_OK;
 }
 
-// Automatic layout: classify + sort on the device, then one persistent
-// launch per class (register buckets, LDS lanes, one workgroup per pair).
-int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
-                   SwOut out, hipStream_t stream) {
+// Band-bucket layout: classify + sort on the device, then one launch per
+// class (register buckets on forked streams, LDS lanes, one workgroup per
+// pair).
+int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                      SwOut out, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
     if (int rc = check_sw_scoring(sc)) return rc;
     if (b.n_pairs == 0) return CGM_OK;
@@ -334,6 +335,20 @@ int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
     CGM_HIP(hipFreeAsync(order, stream));
     CGM_HIP(hipFreeAsync(ranges, stream));
     return rc;
+}
+
+// Automatic layout: the persistent task-queue kernel of the mixed path with
+// every pair an SW pair, on a plan cached per (device, stream) and grown
+// when a larger batch arrives.
+int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                   SwOut out, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    if (int rc = check_sw_scoring(sc)) return rc;
+    if (b.n_pairs == 0) return CGM_OK;
+    MixPlan *P = nullptr;
+    const WfaPen pen = {4, 6, 2};  // unused: the batch holds no WFA pair
+    if (int rc = cached_plan(stream, b.n_pairs, 0, pen, &P)) return rc;
+    return mix_enqueue(P, b, nullptr, 0, band, band_scalar, nullptr, 0, sc, pen, out, stream);
 }
 
 }  // namespace cgm

@@ -50,9 +50,9 @@ extern "C" {
 #define CGM_BAD_PAIR ((int32_t)0x80000000)
 
 /* Kernel layouts for the launch entry points. */
-#define CGM_LAYOUT_AUTO 0   /* pick by the autotuned length/band table */
+#define CGM_LAYOUT_AUTO 0   /* persistent task-queue kernel, work-sorted */
 #define CGM_LAYOUT_PLAIN 1  /* one workgroup per pair (baseline) */
-#define CGM_LAYOUT_LANE 2   /* SW: one pair per lane (short pairs) */
+#define CGM_LAYOUT_LANE 2   /* SW: one pair per lane, one kernel per band bucket */
 #define CGM_LAYOUT_WAVE 3   /* one 64-lane wavefront per pair */
 
 /* ------------------------------------------------------------ errors, info */
@@ -77,6 +77,7 @@ int cgm_host_malloc(void **hptr, int64_t bytes);
 int cgm_host_free(void *hptr);
 int cgm_memcpy_htod(void *dst, const void *src, int64_t bytes, void *stream);
 int cgm_memcpy_dtoh(void *dst, const void *src, int64_t bytes, void *stream);
+int cgm_memcpy_dtod(void *dst, const void *src, int64_t bytes, void *stream);
 int cgm_stream_create(void **stream);
 int cgm_stream_destroy(void *stream);
 int cgm_stream_synchronize(void *stream);
@@ -158,6 +159,9 @@ int cgm_plan_capture(void *plan, const uint64_t *q_packed, const uint64_t *r_pac
                      int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
                      int32_t *out_r_end, void *stream);
 int cgm_plan_replay(void *plan, void *stream);
+/* Duration of the persistent kernel of the plan's last call (or replay),
+ * from events recorded around it on the call's stream; waits for it. */
+int cgm_plan_kernel_ms(void *plan, float *ms);
 
 /* ------------------------------------------------------ serving engine
  * Continuous batching for a stream of alignment requests (no reference

@@ -11,7 +11,7 @@ from gpu_helpers import (CONFIG_SW, DEFAULT_SW, Uploaded, assert_same, sw_gpu, s
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = {"plain": 1, "auto": 0}
+LAYOUTS = {"plain": 1, "lane": 2, "auto": 0}
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "alignment_vectors.json")
 
 
@@ -72,7 +72,7 @@ def test_sw_config2_full_size(cgm, dev, orc, gen_mod):
     up = Uploaded(dev, w)
     print("[case] config2 full 200000 pairs", flush=True)
     ref = sw_gpu(cgm, dev, up, LAYOUTS["plain"])
-    for name in ("auto",):
+    for name in ("lane", "auto"):
         assert_same(sw_gpu(cgm, dev, up, LAYOUTS[name]), ref, f"config2 full {name} vs plain")
     idx = np.random.default_rng(0).choice(w.n_pairs, 2000, replace=False)
     sub = w.subset(idx)
