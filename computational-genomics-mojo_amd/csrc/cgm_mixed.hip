@@ -41,6 +41,10 @@ constexpr int kPackedBand = 128;
 constexpr int kMixThreads = 256;
 constexpr int kMixGroup = kMixThreads;
 
+// Diagonals per thread and pass in WFA tasks: 4 would spill at the 256 VGPRs
+// the register SW layout leaves.
+constexpr int kMixWfaUnroll = 2;
+
 enum TaskType : int32_t { kTaskSwReg = 0, kTaskSwPacked = 1, kTaskWfa = 2 };
 
 struct Task {
@@ -200,8 +204,8 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
             int qlen, rlen;
             load_pair(b, pair, q, r, qlen, rlen);  // validated by the classifier
             const int me = me_arr ? me_arr[pair] : me_scalar;
-            const int res = wfadev::wfa_pair_block(q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes,
-                                                   my_slab, slab_words);
+            const int res = wfadev::wfa_pair_block<kMixWfaUnroll>(
+                q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes, my_slab, slab_words);
             if (threadIdx.x == 0) {
                 out.score[pair] = res;
                 if (out.q_end) out.q_end[pair] = -1;

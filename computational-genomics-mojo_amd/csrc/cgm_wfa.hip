@@ -151,6 +151,7 @@ __global__ __launch_bounds__(kPlainThreads) void wfa_plain_kernel(Batch b, const
 }
 
 constexpr int kTeamThreads = 256;
+constexpr int kTeamUnroll = 4;  // diagonals per thread and pass of a score
 
 __global__ __launch_bounds__(kTeamThreads) void wfa_wave_kernel(
     Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap, WfaPen p, WfaGeom gm,
@@ -172,8 +173,8 @@ __global__ __launch_bounds__(kTeamThreads) void wfa_wave_kernel(
         const int me = me_arr ? me_arr[pair] : me_scalar;
         const int res = (!ok || me < 0 || me > me_cap)
                             ? CGM_BAD_PAIR
-                            : wfa_pair_block(q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes,
-                                             my_slab, slab_words);
+                            : wfa_pair_block<kTeamUnroll>(q, qlen, r, rlen, me, p, gm, lds_raw,
+                                                          lds_bytes, my_slab, slab_words);
         if (threadIdx.x == 0) out[pair] = res;
         __syncthreads();
     }

@@ -24,11 +24,13 @@ namespace wfadev {
 // Extension compares 32 bases per step: two funnel-shifted 64-bit windows
 // XORed, leading zeros / 2 = matching bases, capped at the sequence ends.
 // A whole workgroup works on one pair; sequences and int16 wavefronts sit
-// in LDS when they fit the workgroup's budget, otherwise the sequences are
-// read from global memory and int32 wavefronts live in an HBM slab.
+// in LDS when they fit the workgroup's budget, otherwise the wavefronts live
+// in an HBM slab (int16 or int32 rows).
 // ---------------------------------------------------------------------------
 constexpr int kWaveThreads = 64;
 constexpr int kNegRow = -(1 << 30);
+constexpr int kTileSteps = 32;      // score steps per block of the tiled layout
+constexpr int kTileMinWidth = 256;  // narrowest tile worth the halo
 
 __device__ __forceinline__ uint64_t window64(const uint64_t *s, int pos, int nwords) {
     const int w = pos >> 5, sh = 2 * (pos & 31);
@@ -63,9 +65,44 @@ __device__ __forceinline__ int reach(int s, const WfaPen &p) {
     return s >= p.o + p.e ? (s - p.o) / p.e : 0;
 }
 
-// Storage of wavefront rows: int16 in LDS when every row fits (see
-// rows_fit_int16), int32 in an HBM slab otherwise.  Invalid entries hold the
-// type's most negative value; any stored value < 0 is invalid.
+// Per-score bookkeeping of one pair, advanced one score step (g) at a time
+// with no division: the step index si = s / g, the M slot si % nM and I/D
+// slot si % nI of score s, the reach R(s), and the slots of s - x, s - o - e
+// (M) and s - e (I/D), or -1 before those scores exist.
+struct WfaSteps {
+    int dx, dO, de;  // x / g, (o + e) / g, e / g
+    int si, ms, is, R;
+
+    __device__ __forceinline__ void init(const WfaPen &p, const WfaGeom &gm) {
+        dx = p.x / gm.g;
+        dO = (p.o + p.e) / gm.g;
+        de = p.e / gm.g;
+        si = ms = is = R = 0;
+    }
+    __device__ __forceinline__ void next(const WfaPen &p, const WfaGeom &gm) {
+        ++si;
+        if (++ms == gm.nM) ms = 0;
+        if (++is == gm.nI) is = 0;
+        const int s = si * gm.g;
+        while ((R + 1) * p.e + p.o <= s) ++R;  // at most once: g <= e
+    }
+    __device__ __forceinline__ static int back(int slot, int d, int n) {
+        return slot >= d ? slot - d : slot - d + n;
+    }
+    __device__ __forceinline__ int mx(const WfaGeom &gm) const {
+        return si >= dx ? back(ms, dx, gm.nM) : -1;
+    }
+    __device__ __forceinline__ int mo(const WfaGeom &gm) const {
+        return si >= dO ? back(ms, dO, gm.nM) : -1;
+    }
+    __device__ __forceinline__ int ie(const WfaGeom &gm) const {
+        return si >= de ? back(is, de, gm.nI) : -1;
+    }
+};
+
+// Storage of wavefront rows: int16 when every row fits (see rows_fit_int16),
+// int32 otherwise.  Invalid entries hold the type's most negative value; any
+// stored value < 0 is invalid.
 template <typename T>
 struct RowNeg;
 template <>
@@ -85,9 +122,9 @@ __device__ __forceinline__ bool rows_fit_int16(int qlen, int me, const WfaPen &p
 
 // One pair aligned by the whole workgroup (blockDim.x threads, any number
 // of wavefronts): threads stride over the diagonals of each score, and one
-// __syncthreads_or per score is both the barrier and the termination test.
+// barrier per score also carries the termination test.
 // q / r may point to LDS copies of the sequences or to global memory.
-template <typename T>
+template <int U, typename T>
 __device__ int wfa_pair_team(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
                              const WfaPen &p, const WfaGeom &gm, T *wf, int W, int K) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -100,50 +137,215 @@ __device__ int wfa_pair_team(const uint64_t *q, int qlen, const uint64_t *r, int
     T *I = wf + gm.nM * W;
     T *D = I + gm.nI * W;
     const int target = qlen - rlen;
-    if (tid == 0) M[K] = (T)extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+    __shared__ int done_at;  // score at which the target was reached
+    if (tid == 0) {
+        M[K] = (T)extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+        done_at = -1;
+    }
     __syncthreads();
     if (target == 0 && (int)M[K] >= qlen) return 0;
+    WfaSteps st;
+    st.init(p, gm);
     for (int s = gm.g; s <= me; s += gm.g) {
-        const int si = s / gm.g;
-        const int ms = (si % gm.nM) * W, is = (si % gm.nI) * W;
-        const int Rs = reach(s, p);
+        st.next(p, gm);
+        const int ms = st.ms * W, is = st.is * W;
+        const int Rs = st.R;
         const int lo = max(max(-rlen, -me), -Rs), hi = min(min(qlen, me), Rs);
-        const int mx = s >= p.x ? (((s - p.x) / gm.g) % gm.nM) * W : -1;
-        const int mo = s >= p.o + p.e ? (((s - p.o - p.e) / gm.g) % gm.nM) * W : -1;
-        const int ie = s >= p.e ? (((s - p.e) / gm.g) % gm.nI) * W : -1;
+        const int mx = st.mx(gm) >= 0 ? st.mx(gm) * W : -1;
+        const int mo = st.mo(gm) >= 0 ? st.mo(gm) * W : -1;
+        const int ie = st.ie(gm) >= 0 ? st.ie(gm) * W : -1;
         int found = 0;
-        for (int k = lo + tid; k <= hi; k += nt) {
-            const int di = k + K;
-            int iv = kNegRow, dv = kNegRow, mv = kNegRow;
-            if (mo >= 0) {
-                if (di > 0) iv = (int)M[mo + di - 1] + 1;
-                if (di < W - 1) dv = (int)M[mo + di + 1];
+        // U diagonals per thread and pass: every read of older score slots
+        // is issued before the first write of the current ones (different
+        // slots, so no aliasing), which overlaps the memory latencies of the
+        // diagonals instead of chaining them.
+        for (int k0 = lo + tid; k0 <= hi; k0 += U * nt) {
+            int iv[U], dv[U], mv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * nt, di = k + K;
+                int a = kNegRow, c = kNegRow, bb = kNegRow, d = kNegRow, m = kNegRow;
+                if (k <= hi) {
+                    if (mo >= 0) {
+                        if (di > 0) a = (int)M[mo + di - 1];
+                        if (di < W - 1) bb = (int)M[mo + di + 1];
+                    }
+                    if (ie >= 0) {
+                        if (di > 0) c = (int)I[ie + di - 1];
+                        if (di < W - 1) d = (int)D[ie + di + 1];
+                    }
+                    if (mx >= 0) m = (int)M[mx + di];
+                }
+                iv[u] = max(a, c) + 1;
+                dv[u] = max(bb, d);
+                mv[u] = max(max(m + 1, iv[u]), dv[u]);
             }
-            if (ie >= 0) {
-                if (di > 0) iv = max(iv, (int)I[ie + di - 1] + 1);
-                if (di < W - 1) dv = max(dv, (int)D[ie + di + 1]);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * nt, di = k + K;
+                if (k <= hi) {
+                    I[is + di] = iv[u] < 0 ? neg : (T)iv[u];
+                    D[is + di] = dv[u] < 0 ? neg : (T)dv[u];
+                }
             }
-            if (mx >= 0) mv = (int)M[mx + di] + 1;
-            mv = max(max(mv, iv), dv);
-            I[is + di] = iv < 0 ? neg : (T)iv;
-            D[is + di] = dv < 0 ? neg : (T)dv;
-            if (mv >= 0) {
-                mv = extend_diag(q, qlen, qw, r, rlen, rw, mv, k);
-                M[ms + di] = (T)mv;
-                found |= (k == target && mv >= qlen);
-            } else {
-                M[ms + di] = neg;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * nt, di = k + K;
+                if (k <= hi) {
+                    int m = mv[u];
+                    if (m >= 0) {
+                        m = extend_diag(q, qlen, qw, r, rlen, rw, m, k);
+                        found |= (k == target && m >= qlen);
+                    }
+                    M[ms + di] = m < 0 ? neg : (T)m;
+                }
             }
         }
-        if (__syncthreads_or(found)) return s;
+        // One barrier ends the score: a thread that reached the target
+        // records s before it; scores differ per step, so the flag needs
+        // no reset, and the pair ends at the first s recorded.
+        if (found) done_at = s;
+        __syncthreads();
+        if (done_at == s) return s;
+    }
+    return -1;
+}
+
+// Trapezoid tiling of the team layout for pairs whose wavefronts exceed
+// LDS.  Scores advance in blocks of up to TB steps.  Each block's widest
+// range is cut into tiles of TW = LW - 2 TB diagonals; a tile copies every
+// slot of its diagonals plus TB on each side from slab set `cur` into LDS
+// (entries outside [-K, K] read as invalid), runs the block's steps there on
+// a range that narrows by one diagonal per step on each side (a cell reads
+// diagonals k-1..k+1 of earlier scores only, and e >= g), and writes its own
+// diagonals of every slot to slab set `cur ^ 1`.  The slab is thus read and
+// written once per TB scores instead of every score.  The first step whose
+// M row reaches the target ends the pair, in whichever tile computes it.
+// Entries a step does not compute keep what the slot held, which is invalid
+// outside the score's range because ranges only grow.  `slab` holds two sets
+// of slots * W int16 entries; `loc` holds slots * LW int16 entries.
+template <int U>
+__device__ int wfa_pair_tiled(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
+                              const WfaPen &p, const WfaGeom &gm, int16_t *slab, int W, int K,
+                              int16_t *loc, int LW, int TB) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int slots = gm.nM + 2 * gm.nI;
+    const int16_t neg = RowNeg<int16_t>::v;
+    const int64_t set_len = (int64_t)slots * W;
+    for (int64_t i = tid; i < 2 * set_len; i += nt) slab[i] = neg;
+    __syncthreads();
+    const int target = qlen - rlen;
+    __shared__ int done_at;  // score at which the target was reached
+    if (tid == 0) {
+        slab[K] = (int16_t)extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+        done_at = -1;
+    }
+    __syncthreads();
+    if (target == 0 && (int)slab[K] >= qlen) return 0;
+    const int TW = LW - 2 * TB;
+    WfaSteps blk;  // state at the start of the current block of scores
+    blk.init(p, gm);
+    int16_t *M = loc;
+    int16_t *I = loc + gm.nM * LW;
+    int16_t *D = I + gm.nI * LW;
+    int cur = 0;
+    for (int s0 = 0; s0 + gm.g <= me;) {
+        const int nsteps = min(TB, (me - s0) / gm.g);
+        const int s_end = s0 + nsteps * gm.g;
+        const int Re = reach(s_end, p);
+        const int lo_e = max(max(-rlen, -me), -Re), hi_e = min(min(qlen, me), Re);
+        const int16_t *src = slab + cur * set_len;
+        int16_t *dst = slab + (cur ^ 1) * set_len;
+        for (int a = lo_e; a <= hi_e; a += TW) {
+            const int b = min(a + TW - 1, hi_e);
+            const int base = a - TB;  // diagonal of local index 0
+            for (int sl = 0; sl < slots; ++sl)
+                for (int j = tid; j < LW; j += nt) {
+                    const int k = base + j;
+                    loc[sl * LW + j] = (k >= -K && k <= K) ? src[(int64_t)sl * W + k + K] : neg;
+                }
+            __syncthreads();
+            WfaSteps st = blk;
+            for (int t = 1; t <= nsteps; ++t) {
+                const int s = s0 + t * gm.g;
+                st.next(p, gm);
+                const int ms = st.ms * LW, is = st.is * LW;
+                const int Rs = st.R;
+                const int shrink = nsteps - t;
+                const int klo = max(max(max(-rlen, -me), -Rs), a - shrink);
+                const int khi = min(min(min(qlen, me), Rs), b + shrink);
+                const int mx = st.mx(gm) >= 0 ? st.mx(gm) * LW : -1;
+                const int mo = st.mo(gm) >= 0 ? st.mo(gm) * LW : -1;
+                const int ie = st.ie(gm) >= 0 ? st.ie(gm) * LW : -1;
+                int found = 0;
+                // klo - 1 >= base and khi + 1 < base + LW: no bounds tests
+                for (int k0 = klo + tid; k0 <= khi; k0 += U * nt) {
+                    int iv[U], dv[U], mv[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + u * nt, di = k - base;
+                        int a_ = kNegRow, c = kNegRow, bb = kNegRow, d = kNegRow, m = kNegRow;
+                        if (k <= khi) {
+                            if (mo >= 0) {
+                                a_ = M[mo + di - 1];
+                                bb = M[mo + di + 1];
+                            }
+                            if (ie >= 0) {
+                                c = I[ie + di - 1];
+                                d = D[ie + di + 1];
+                            }
+                            if (mx >= 0) m = M[mx + di];
+                        }
+                        iv[u] = max(a_, c) + 1;
+                        dv[u] = max(bb, d);
+                        mv[u] = max(max(m + 1, iv[u]), dv[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + u * nt, di = k - base;
+                        if (k <= khi) {
+                            I[is + di] = iv[u] < 0 ? neg : (int16_t)iv[u];
+                            D[is + di] = dv[u] < 0 ? neg : (int16_t)dv[u];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + u * nt, di = k - base;
+                        if (k <= khi) {
+                            int m = mv[u];
+                            if (m >= 0) {
+                                m = extend_diag(q, qlen, qw, r, rlen, rw, m, k);
+                                found |= (k == target && m >= qlen);
+                            }
+                            M[ms + di] = m < 0 ? neg : (int16_t)m;
+                        }
+                    }
+                }
+                // the first score that reaches the target ends the pair
+                // (the flag is set once, then every thread returns)
+                if (found) done_at = s;
+                __syncthreads();
+                if (done_at == s) return s;
+            }
+            const int wd = b - a + 1;
+            for (int sl = 0; sl < slots; ++sl)
+                for (int j = tid; j < wd; j += nt)
+                    dst[(int64_t)sl * W + a + j + K] = loc[sl * LW + (a + j - base)];
+            __syncthreads();
+        }
+        for (int t = 0; t < nsteps; ++t) blk.next(p, gm);
+        cur ^= 1;
+        s0 = s_end;
     }
     return -1;
 }
 
 // Everything one workgroup needs to align a WFA pair: sequences staged to
-// LDS and int16 wavefronts in LDS when both fit `lds_bytes`, otherwise the
-// sequences are read from global memory and the wavefronts live in the
-// workgroup's HBM slab (int32).  Returns the cost, -1, or CGM_BAD_PAIR.
+// LDS when they fit `lds_bytes`; int16 wavefronts in LDS when they fit the
+// rest, otherwise in the workgroup's HBM slab (int16 rows when they fit
+// int16, int32 otherwise).  Returns the cost, -1, or CGM_BAD_PAIR.
+template <int U>
 __device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, const uint64_t *rg,
                                               int rlen, int me, const WfaPen &p,
                                               const WfaGeom &gm, uint8_t *lds, int lds_bytes,
@@ -157,15 +359,30 @@ __device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, cons
     const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
     const int64_t seq_bytes = 8 * (int64_t)(qw + rw);
     const int64_t wf16_bytes = 2 * (int64_t)slots * W;
-    if (rows_fit_int16(qlen, me, p) && seq_bytes + wf16_bytes <= lds_bytes) {
+    const bool fit16 = rows_fit_int16(qlen, me, p);
+    const uint64_t *q = qg, *r = rg;
+    if (seq_bytes <= lds_bytes) {
+        // Sequences go to LDS whenever they fit, also when the wavefronts
+        // do not: the extension reads them on every diagonal.
         uint64_t *seq = (uint64_t *)lds;
         for (int i = threadIdx.x; i < qw + rw; i += blockDim.x) seq[i] = i < qw ? qg[i] : rg[i - qw];
+        q = seq;
+        r = seq + qw;
         // the team function's first barrier orders these copies
-        return wfa_pair_team(seq, qlen, seq + qw, rlen, me, p, gm,
-                             (int16_t *)(lds + seq_bytes), W, K);
+        if (fit16 && seq_bytes + wf16_bytes <= lds_bytes)
+            return wfa_pair_team<U>(q, qlen, r, rlen, me, p, gm, (int16_t *)(lds + seq_bytes), W, K);
+        // Tiled: two int16 slab sets (slots * W int32 words) and LDS tiles.
+        const int LW = (int)((lds_bytes - seq_bytes) / (2 * slots));
+        if (fit16 && LW - 2 * kTileSteps >= kTileMinWidth && (int64_t)slots * W <= slab_words)
+            return wfa_pair_tiled<U>(q, qlen, r, rlen, me, p, gm, (int16_t *)slab, W, K,
+                                     (int16_t *)(lds + seq_bytes), LW, kTileSteps);
     }
+    // Untiled HBM slab of slab_words int32 words: int16 rows when they fit
+    // (half the traffic), int32 otherwise.
+    if (fit16 && (int64_t)slots * W <= 2 * slab_words)
+        return wfa_pair_team<U>(q, qlen, r, rlen, me, p, gm, (int16_t *)slab, W, K);
     if ((int64_t)slots * W <= slab_words)
-        return wfa_pair_team(qg, qlen, rg, rlen, me, p, gm, slab, W, K);
+        return wfa_pair_team<U>(q, qlen, r, rlen, me, p, gm, slab, W, K);
     return CGM_BAD_PAIR;
 }
 

@@ -82,3 +82,47 @@ def test_wfa_is_deterministic(cgm, dev, gen_mod):
     w = gen_mod.generate(3, 400, seed=4)
     up = Uploaded(dev, w)
     assert_same(wfa_gpu(cgm, dev, up, 0), wfa_gpu(cgm, dev, up, 0), "run twice")
+
+
+def big_wfa_pairs(cgm, gen_mod, orc):
+    """Config-4 WFA pairs whose wavefronts exceed LDS (max_error 2.6k-13k,
+    5-20 kb), the same pairs with max_error just below their cost, and a
+    34 kb pair whose rows do not fit int16."""
+    w = gen_mod.generate(4, 20000, seed=11)
+    wf = np.nonzero(w.kind == 1)[0]
+    me = w.max_error[wf]
+    idx = np.concatenate([wf[(me >= lo) & (me < hi)][:n]
+                          for lo, hi, n in ((2600, 4000, 6), (4000, 7000, 4), (7000, 13000, 1))])
+    big = w.subset(idx)
+    below = w.subset(idx)
+    below.max_error = np.maximum(wfa_oracle(orc, big) - 2, 0).astype(np.int32)
+    rng = np.random.default_rng(12)
+    t = "".join(rng.choice(list("ACGT"), 34000))
+    qs = list(t)
+    for pos in rng.choice(34000, 40, replace=False):
+        qs[pos] = "ACGT"[("ACGT".index(qs[pos]) + 1) % 4]
+    huge = workload_from_strings([("".join(qs[:20000] + qs[20005:]), t)], max_error=[400])
+    huge.kind[:] = 1
+    return gen_mod.concat([big, below, huge])
+
+
+def test_wfa_large_pairs_tiled_and_slab(cgm, dev, orc, gen_mod):
+    w = big_wfa_pairs(cgm, gen_mod, orc)
+    print(f"[case] large WFA pairs n={w.n_pairs} max_error max={w.max_error.max()}", flush=True)
+    exp = wfa_oracle(orc, w)
+    assert (exp >= 0).sum() > 10 and (exp == -1).sum() > 5
+    up = Uploaded(dev, w)
+    for layout in ("wave", "auto"):
+        print(f"[case] large WFA layout={layout}", flush=True)
+        assert_same(wfa_gpu(cgm, dev, up, LAYOUTS[layout]), exp, f"large {layout}")
+    kind = dev.upload(w.kind)
+    cfg = cgm.AlignConfig(2, -4, -4, -2, 0)
+    for lds in (24 * 1024, 12 * 1024, 0):
+        print(f"[case] large WFA mixed plan lds_bytes={lds}", flush=True)
+        plan = cgm.MixedPlan(w.n_pairs, int(w.max_error.max()), lds_bytes=lds)
+        out = cgm.SwResult(w.n_pairs)
+        plan.align(dev, (up.q, up.r, up.ql, up.rl, up.qo, up.ro), w.n_pairs, kind, up.band,
+                   up.me, cfg, out)
+        dev.synchronize()
+        assert_same(dev.download(out.scores, w.n_pairs), exp, f"large mixed lds={lds}")
+        plan.close()

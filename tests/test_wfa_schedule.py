@@ -80,3 +80,126 @@ def test_wave_schedule_equals_oracle():
         x, o, e = rng.choice(pens)
         me = rng.choice([0, 2, 5, 12, 30, 80, 200])
         assert wfa_sched(a, b, x, o, e, me) == O.wfa_affine(a, b, x, o, e, me), (a, b, x, o, e, me)
+
+
+def wfa_tiled(q, r, x, o, e, me, TB, TW):
+    """The trapezoid-tiled schedule of the team kernel for pairs whose
+    wavefronts exceed LDS: blocks of up to TB score steps; per block the
+    widest range is cut into tiles of TW diagonals; a tile copies every slot
+    of its diagonals plus TB on each side from slab set `cur` (entries
+    outside [-K, K] read as invalid), runs the block's steps on a range that
+    narrows by one diagonal per step on each side, returns at the first step
+    that reaches the target, and writes its own diagonals of every slot to
+    slab set `cur ^ 1`."""
+    ql, rl = len(q), len(r)
+    if ql == 0 and rl == 0: return 0
+    target = ql - rl
+    g = math.gcd(math.gcd(x, o), e)
+    R = lambda s: (s - o) // e if s >= o + e else 0
+    if abs(target) > min(me, R(me)) and target != 0: return -1
+    K = min(me, max(ql, rl), R(me))
+    W = 2 * K + 1
+    nM = max(x, o + e) // g + 1
+    nI = e // g + 1
+    slots = nM + 2 * nI
+    slab = [[[NEG] * W for _ in range(slots)] for _ in range(2)]
+    def ext(row, k):
+        col = row - k
+        while row < ql and 0 <= col < rl and q[row] == r[col]:
+            row += 1; col += 1
+        return row
+    slab[0][0][K] = ext(0, 0)
+    if target == 0 and slab[0][0][K] >= ql: return 0
+    LW = TW + 2 * TB
+    cur, s0 = 0, 0
+    while s0 + g <= me:
+        nsteps = min(TB, (me - s0) // g)
+        s_end = s0 + nsteps * g
+        Re = R(s_end)
+        lo_e, hi_e = max(-rl, -me, -Re), min(ql, me, Re)
+        src, dst = slab[cur], slab[cur ^ 1]
+        for a in range(lo_e, hi_e + 1, TW):
+            b = min(a + TW - 1, hi_e)
+            base = a - TB
+            loc = [[src[sl][k + K] if -K <= k <= K else NEG for k in range(base, base + LW)]
+                   for sl in range(slots)]
+            M, I, D = loc[:nM], loc[nM:nM + nI], loc[nM + nI:]
+            for t in range(1, nsteps + 1):
+                s = s0 + t * g
+                si = s // g
+                ms, is_ = si % nM, si % nI
+                Rs = R(s)
+                shrink = nsteps - t
+                klo = max(-rl, -me, -Rs, a - shrink)
+                khi = min(ql, me, Rs, b + shrink)
+                mx = ((s - x) // g) % nM if s >= x else -1
+                mo = ((s - o - e) // g) % nM if s >= o + e else -1
+                ie = ((s - e) // g) % nI if s >= e else -1
+                found = False
+                for k in range(klo, khi + 1):
+                    di = k - base
+                    assert 1 <= di <= LW - 2
+                    A = M[mo][di - 1] if mo >= 0 else NEG
+                    Bv = M[mo][di + 1] if mo >= 0 else NEG
+                    C = I[ie][di - 1] if ie >= 0 else NEG
+                    Dv = D[ie][di + 1] if ie >= 0 else NEG
+                    m = M[mx][di] if mx >= 0 else NEG
+                    iv = max(A, C) + 1
+                    dv = max(Bv, Dv)
+                    mv = max(m + 1, iv, dv)
+                    I[is_][di] = iv if iv >= 0 else NEG
+                    D[is_][di] = dv if dv >= 0 else NEG
+                    if mv >= 0:
+                        mv = ext(mv, k)
+                        found |= (k == target and mv >= ql)
+                    M[ms][di] = mv if mv >= 0 else NEG
+                if found: return s
+            for sl in range(slots):
+                for k in range(a, b + 1):
+                    dst[sl][k + K] = loc[sl][k - base]
+        cur ^= 1
+        s0 = s_end
+    return -1
+
+
+def test_tiled_schedule_equals_oracle():
+    rng = random.Random(2)
+    pens = [(4, 6, 2), (1, 1, 1), (3, 2, 1), (5, 0, 3), (2, 7, 1), (6, 4, 2), (9, 3, 3)]
+    for _ in range(2500):
+        a = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 30)))
+        if rng.random() < 0.5:
+            b = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 30)))
+        else:
+            b = "".join(c if rng.random() > 0.15 else rng.choice("ACGT")
+                        for c in a if rng.random() > 0.1)
+        x, o, e = rng.choice(pens)
+        me = rng.choice([0, 2, 5, 12, 30, 80, 200])
+        TB, TW = rng.randint(1, 6), rng.randint(1, 9)
+        assert wfa_tiled(a, b, x, o, e, me, TB, TW) == O.wfa_affine(a, b, x, o, e, me), \
+            (a, b, x, o, e, me, TB, TW)
+
+
+def test_incremental_step_state_equals_division_form():
+    """The kernels advance slot indices and the reach per score step with no
+    division (WfaSteps in csrc/cgm_wfa_dev.h); restated here, they must equal
+    si % nM, si % nI, ((s - x) / g) % nM, ((s - o - e) / g) % nM,
+    ((s - e) / g) % nI and R(s) at every score."""
+    for x, o, e in [(4, 6, 2), (1, 1, 1), (3, 2, 1), (5, 0, 3), (2, 7, 1), (6, 4, 2), (9, 3, 3),
+                    (8, 12, 4), (4, 0, 2), (7, 5, 7)]:
+        g = math.gcd(math.gcd(x, o), e)
+        nM, nI = max(x, o + e) // g + 1, e // g + 1
+        dx, dO, de = x // g, (o + e) // g, e // g
+        back = lambda slot, d, n: slot - d if slot >= d else slot - d + n  # noqa: E731
+        si = ms = is_ = R = 0
+        for s in range(g, 2000, g):
+            si += 1
+            ms = 0 if ms + 1 == nM else ms + 1
+            is_ = 0 if is_ + 1 == nI else is_ + 1
+            while (R + 1) * e + o <= si * g:
+                R += 1
+            assert si == s // g and ms == si % nM and is_ == si % nI
+            assert R == ((s - o) // e if s >= o + e else 0)
+            assert (back(ms, dx, nM) if si >= dx else -1) == (((s - x) // g) % nM if s >= x else -1)
+            assert (back(ms, dO, nM) if si >= dO else -1) == \
+                (((s - o - e) // g) % nM if s >= o + e else -1)
+            assert (back(is_, de, nI) if si >= de else -1) == (((s - e) // g) % nI if s >= e else -1)
