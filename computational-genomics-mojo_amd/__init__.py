@@ -9,9 +9,8 @@ Python identifier; load it with importlib (see tests/conftest.py).
 """
 from . import _lib
 from .align import (AlignConfig, AlignResult, MixedPlan, SwResult, WFAConfig, align_mixed_batch,
-                    align_mixed_plain,
-                    default_config,
-                    default_wfa_config, launch_banded_sw, launch_wfa_affine,
+                    align_mixed_plain, default_config, default_tuning, default_wfa_config,
+                    launch_banded_sw, launch_wfa_affine, set_default_tuning,
                     smith_waterman_banded, sw_banded_batch, wfa_affine, wfa_affine_batch,
                     wfa_affine_cpu)
 from .device import DeviceArray, GenomicsDevice, Timer
@@ -20,9 +19,11 @@ from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_
                        get_base_from_word, get_view, pack_codes)
 
 __all__ = [
-    "AlignConfig", "AlignResult", "MixedPlan", "align_mixed_batch", "align_mixed_plain", "SwResult", "WFAConfig", "default_config",
-    "default_wfa_config", "launch_banded_sw", "launch_wfa_affine", "smith_waterman_banded",
+    "AlignConfig", "AlignResult", "MixedPlan", "align_mixed_batch", "align_mixed_plain",
+    "SwResult", "WFAConfig", "default_config", "default_tuning", "default_wfa_config",
+    "launch_banded_sw", "launch_wfa_affine", "set_default_tuning", "smith_waterman_banded",
     "sw_banded_batch", "wfa_affine", "wfa_affine_batch", "wfa_affine_cpu", "DeviceArray",
-    "GenomicsDevice", "Timer", "GeneratorHandle", "ServingEngine", "BASES_PER_WORD", "Base", "SequenceBatch", "SequenceView",
-    "ascii_to_2bit", "get_base_from_word", "get_view", "pack_codes",
+    "GenomicsDevice", "Timer", "GeneratorHandle", "ServingEngine", "BASES_PER_WORD", "Base",
+    "SequenceBatch", "SequenceView", "ascii_to_2bit", "get_base_from_word", "get_view",
+    "pack_codes",
 ]

@@ -20,6 +20,10 @@ CGM_ERR_NODEV = 5
 CGM_ERR_NOTFOUND = 6
 CGM_BAD_PAIR = -(2 ** 31)
 
+# cgm_tuning fields (CGM_TUNE_* order)
+TUNE_FIELDS = ("blocks_per_cu", "lds_bytes", "tile_steps", "wave_max_width", "reg_max_band",
+               "lane_max_len")
+
 LAYOUT_AUTO = 0
 LAYOUT_PLAIN = 1
 LAYOUT_LANE = 2
@@ -70,6 +74,13 @@ SIGNATURES = {
                          _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
     "cgm_plan_replay": [_vp, _vp],
     "cgm_plan_kernel_ms": [_vp, _pf],
+    "cgm_plan_get_tuning": [_vp, _vp],
+    "cgm_plan_set_tuning": [_vp, _vp],
+    "cgm_tuning_get_default": [_vp],
+    "cgm_tuning_set_default": [_vp],
+    "cgm_plan_autotune": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _vp,
+                          _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp,
+                          _vp, _i32, _vp, _pf, _pf, _vp],
     "cgm_engine_create": [_i64, _i32, _i64, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                           _pvp],
     "cgm_engine_destroy": [_vp],

@@ -234,6 +234,35 @@ class MixedPlan:
         _lib.call("cgm_plan_kernel_ms", self.handle, ctypes.byref(ms))
         return float(ms.value)
 
+    def tuning(self) -> dict:
+        """The plan's layout choices (cgm_plan_get_tuning) by field name."""
+        a = np.zeros(len(_lib.TUNE_FIELDS), np.int32)
+        _lib.call("cgm_plan_get_tuning", self.handle, _lib.ptr(a))
+        return dict(zip(_lib.TUNE_FIELDS, (int(v) for v in a)))
+
+    def set_tuning(self, **fields) -> None:
+        """Changes some layout choices (others keep their value)."""
+        t = self.tuning()
+        for k, v in fields.items():
+            if k not in t:
+                raise KeyError(k)
+            t[k] = int(v)
+        a = np.array([t[k] for k in _lib.TUNE_FIELDS], np.int32)
+        _lib.call("cgm_plan_set_tuning", self.handle, _lib.ptr(a))
+
+    def autotune(self, device: GenomicsDevice, bufs, n: int, kind, band, max_error,
+                 cfg: AlignConfig, out: SwResult, reps: int = 2):
+        """cgm_plan_autotune on one batch: returns (best tuning dict, best
+        ms, starting ms); the plan keeps the best tuning."""
+        import ctypes
+        a = np.zeros(len(_lib.TUNE_FIELDS), np.int32)
+        best, start = ctypes.c_float(), ctypes.c_float()
+        _lib.call("cgm_plan_autotune", self.handle,
+                  *_mixed_args(bufs, n, kind, band, max_error, cfg, out, self.pen), int(reps),
+                  _lib.ptr(a), ctypes.byref(best), ctypes.byref(start), device.stream)
+        return dict(zip(_lib.TUNE_FIELDS, (int(v) for v in a))), float(best.value), \
+            float(start.value)
+
     def close(self) -> None:
         if self.handle:
             _lib.load().cgm_plan_destroy(self.handle)
@@ -244,6 +273,25 @@ class MixedPlan:
             self.close()
         except Exception:
             pass
+
+
+def default_tuning() -> dict:
+    """The tuning new plans start from (cgm_tuning_get_default)."""
+    a = np.zeros(len(_lib.TUNE_FIELDS), np.int32)
+    _lib.call("cgm_tuning_get_default", _lib.ptr(a))
+    return dict(zip(_lib.TUNE_FIELDS, (int(v) for v in a)))
+
+
+def set_default_tuning(**fields) -> None:
+    """Changes the tuning of plans created afterwards (including the
+    library's own plans and serving engines)."""
+    t = default_tuning()
+    for k, v in fields.items():
+        if k not in t:
+            raise KeyError(k)
+        t[k] = int(v)
+    _lib.call("cgm_tuning_set_default", _lib.ptr(np.array([t[k] for k in _lib.TUNE_FIELDS],
+                                                            np.int32)))
 
 
 def align_mixed_plain(device: GenomicsDevice, bufs, n: int, kind, band, max_error,

@@ -273,8 +273,7 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
             return fail(set_error(CGM_ERR_HIP, "engine: stream/event"));
-        int rc = mix_plan_create(max_batch_pairs, max_error_cap, pen, kMixDefaultWavesPerCu,
-                                 kMixDefaultLdsBytes, &s.plan);
+        int rc = mix_plan_create(max_batch_pairs, max_error_cap, pen, 0, 0, &s.plan);
         if (rc) return fail(rc);
         E->device_bytes += mix_plan_bytes(s.plan);
         const int64_t wb = max_batch_words * 8, ab = 7 * max_batch_pairs * 4,

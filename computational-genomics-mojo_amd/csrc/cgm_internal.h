@@ -109,15 +109,38 @@ int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                 SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream);
 int mix_replay(MixPlan *P, hipStream_t stream);
+
+// Layout choices of the persistent mixed kernel, per plan (values and
+// indices as CGM_TUNE_* in cgm_align.h).  None changes a result: every
+// layout computes the same recurrences exactly.
+struct MixTuning {
+    int32_t blocks_per_cu = 2;         // four-wavefront workgroups per CU (1 or 2)
+    int32_t lds_bytes = 64 * 1024;     // LDS per workgroup
+    int32_t tile_steps = 32;           // WFA scores per block of the tiled layout
+    int32_t wave_max_width = 1 << 30;  // WFA pairs up to this many diagonals: one per wavefront
+    int32_t reg_max_band = 64;         // SW bands up to this: register lanes, wider: LDS lanes
+    int32_t lane_max_len = 1 << 30;    // SW pairs longer than this: one workgroup per pair
+};
+constexpr int kMixTuneN = 6;
+void tuning_to_array(const MixTuning &t, int32_t *a);
+int tuning_from_array(const int32_t *a, MixTuning *t);  // validates
+MixTuning mix_default_tuning();
+void mix_set_default_tuning(const MixTuning &t);
+MixTuning mix_plan_tuning(const MixPlan *P);
+void mix_plan_set_tuning(MixPlan *P, const MixTuning &t);
+// Measures candidate tunings on one batch (coordinate descent over the
+// fields, `reps` timed enqueues each, best kept), leaves the best in the
+// plan and returns it with its time and the starting tuning's time.
+int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
+                 SwScoring sc, WfaPen pen, SwOut out, int reps, hipStream_t stream,
+                 MixTuning *best, float *best_ms, float *start_ms);
 // A plan owned by the library for the drop-in launchers, one per (device,
 // stream), recreated larger when a batch exceeds it (the old plan is freed
 // after the stream drains).
 int cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen, MixPlan **out);
 int64_t mix_plan_capacity(const MixPlan *P, int32_t *me_cap, WfaPen *pen);
 
-// Defaults of the persistent mixed kernel (overridden by cgm_plan_create).
-constexpr int kMixDefaultWavesPerCu = 8;
-constexpr int kMixDefaultLdsBytes = 64 * 1024;
 
 // WFA layout: LDS budget per 256-thread workgroup and workgroups per CU.
 constexpr int kWfaLdsBytes = 52 * 1024;

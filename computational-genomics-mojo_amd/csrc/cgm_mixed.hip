@@ -6,20 +6,26 @@
 //
 // Pipeline (all device work, one stream):
 //  1. classify each pair (kind, effective band, lengths, max_error) into
-//     0 SW register lanes (band <= 64), 1 SW packed register lanes (band 128),
-//     2 WFA wavefront, 3 SW LDS lanes, 4 SW one-workgroup-per-pair, 5 bad;
-//     bad pairs get their outputs written here;
+//     0 SW register lanes (band <= reg_max_band <= 64), 1 SW packed register
+//     lanes (band 128), 2 WFA one workgroup per pair, 3 SW LDS lanes, 4 SW
+//     one workgroup per pair (also SW pairs longer than lane_max_len), 5 bad,
+//     6 WFA one wavefront per pair (at most wave_max_width diagonals and
+//     fitting a quarter of the workgroup's LDS); bad pairs get their outputs
+//     written here;
 //  2. radix-sort pair indices by (class, band, length) or (class, WFA work);
 //  3. build tasks: 256 consecutive SW pairs of one register class per task
-//     (one 64-lane group per wavefront of the workgroup), one WFA pair per
-//     task (the whole workgroup), each with an estimated cost;
+//     (one 64-lane group per wavefront of the workgroup), one class-2 WFA
+//     pair per task (the whole workgroup), four class-6 WFA pairs per task
+//     (one per wavefront), each with an estimated cost;
 //  4. radix-sort tasks by cost, largest first (longest-processing-time
 //     order, so the tail of the launch is made of small tasks);
-//  5. the persistent kernel: CUs x waves_per_cu/4 four-wavefront workgroups
+//  5. the persistent kernel: CUs x blocks_per_cu four-wavefront workgroups
 //     take task indices from an atomic counter until the queue is empty;
-//  6. classes 3 and 4 (unusual scorings or bands) run the LDS-lane and
-//     one-workgroup-per-pair kernels over their ranges.
+//  6. classes 3 and 4 (unusual scorings, bands or lengths) run the LDS-lane
+//     and one-workgroup-per-pair kernels over their ranges.
+// The class limits and the kernel's shape are the plan's MixTuning.
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -32,9 +38,12 @@ namespace cgm {
 
 namespace {
 
-constexpr int kMixClasses = 6;
+constexpr int kMixClasses = 7;
 constexpr int kMixSwReg = 0, kMixSwPacked = 1, kMixWfa = 2, kMixSwLds = 3, kMixSwPlain = 4,
-              kMixBad = 5;
+              kMixBad = 5, kMixWfaWave = 6;
+constexpr int kMixWaves = 4;              // wavefronts per workgroup
+constexpr int kMixMaxBlocksPerCu = 2;     // 256 VGPRs: two workgroups per CU at most
+constexpr int kMixMaxLds = 160 * 1024;    // LDS of a CU
 constexpr int kPackedBand = 128;
 // One workgroup = four wavefronts: an SW task holds up to 4 x 64 pairs (one
 // lane-group per wavefront), a WFA task is one pair for the whole workgroup.
@@ -45,7 +54,12 @@ constexpr int kMixGroup = kMixThreads;
 // the register SW layout leaves.
 constexpr int kMixWfaUnroll = 2;
 
-enum TaskType : int32_t { kTaskSwReg = 0, kTaskSwPacked = 1, kTaskWfa = 2 };
+enum TaskType : int32_t { kTaskSwReg = 0, kTaskSwPacked = 1, kTaskWfa = 2, kTaskWfaWave = 3 };
+
+// LDS part of one wavefront in class-6 tasks.
+__host__ __device__ __forceinline__ int wave_part_bytes(int lds_bytes) {
+    return (lds_bytes / kMixWaves) & ~15;
+}
 
 struct Task {
     int32_t type, start, count, pad;
@@ -63,8 +77,9 @@ __device__ __forceinline__ uint32_t wfa_work_estimate(int me, int qlen, int rlen
 
 __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scalar,
                                  const int32_t *band, int32_t band_scalar, const int32_t *me_arr,
-                                 int32_t me_scalar, int32_t me_cap, SwScoring sc, WfaPen p, int g,
-                                 SwOut out, uint32_t *keys, int32_t *idx, int32_t *counts) {
+                                 int32_t me_scalar, int32_t me_cap, SwScoring sc, WfaPen p,
+                                 wfadev::WfaGeom gm, MixTuning tune, SwOut out, uint32_t *keys,
+                                 int32_t *idx, int32_t *counts) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool in = i < b.n_pairs;
     const uint64_t *q, *r;
@@ -80,9 +95,12 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
                 const int64_t hmax = (int64_t)max(sc.match, 0) * min(ql, rl);
                 const int oe = sc.gap_open + sc.gap_extend;
                 const bool reg_scoring = sc.mismatch <= 0 && sc.gap_extend <= 0 && oe <= 0;
+                const bool long_pair = max(ql, rl) > tune.lane_max_len;
                 if (hmax > kMaxScoreProduct) {
                     cls = kMixBad;
-                } else if (reg_scoring && be <= 64 && hmax < (1 << 23)) {
+                } else if (long_pair && be <= kPlainMaxBand) {
+                    cls = kMixSwPlain;
+                } else if (reg_scoring && be <= tune.reg_max_band && hmax < (1 << 23)) {
                     cls = kMixSwReg;
                 } else if (reg_scoring && be == kPackedBand && hmax + oe <= 32767 && oe >= -32768) {
                     cls = kMixSwPacked;
@@ -96,8 +114,13 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
         } else if (k == 1) {
             const int me = me_arr ? me_arr[i] : me_scalar;
             if (me >= 0 && me <= me_cap) {
-                cls = kMixWfa;
-                payload = wfa_work_estimate(me, ql, rl, p, g);
+                const int K = min(min(me, max(ql, rl)), wfadev::reach(me, p));
+                const bool wave = 2 * K + 1 <= tune.wave_max_width &&
+                                  wfadev::rows_fit_int16(ql, me, p) &&
+                                  wfadev::wave_pair_bytes(ql, rl, me, p, gm) <=
+                                      wave_part_bytes(tune.lds_bytes);
+                cls = wave ? kMixWfaWave : kMixWfa;
+                payload = wfa_work_estimate(me, ql, rl, p, gm.g);
             }
         }
     }
@@ -125,7 +148,8 @@ __global__ void mix_ranges_kernel(const int32_t *counts, int32_t *ranges, int32_
         ranges[2 * c + 1] = s;
     }
     *n_tasks = (counts[kMixSwReg] + kMixGroup - 1) / kMixGroup +
-               (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup + counts[kMixWfa];
+               (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup + counts[kMixWfa] +
+               (counts[kMixWfaWave] + kMixWaves - 1) / kMixWaves;
 }
 
 __global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
@@ -152,10 +176,18 @@ __global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
             const uint32_t band = payload >> 17, len = payload & 0x1ffffu;
             // ~10 VALU ops x 4 cycles per wave-step of 64 cells, units of 16 cycles
             key = (uint32_t)min((uint64_t)len * (2 * band + 1) * 3 + 1, (uint64_t)0xffffffffu);
-        } else {
+        } else if (t < n0 + n1 + (ranges[2 * kMixWfa + 1] - ranges[2 * kMixWfa])) {
             task.type = kTaskWfa;
             task.start = ranges[2 * kMixWfa] + (int32_t)(t - n0 - n1);
             task.count = 1;
+            key = (sorted_keys[task.start] & 0x0fffffffu) + 1;
+        } else {
+            // four class-6 pairs, one per wavefront; the first is the largest
+            const int64_t g = t - n0 - n1 - (ranges[2 * kMixWfa + 1] - ranges[2 * kMixWfa]);
+            const int32_t cnt = ranges[2 * kMixWfaWave + 1] - ranges[2 * kMixWfaWave];
+            task.type = kTaskWfaWave;
+            task.start = ranges[2 * kMixWfaWave] + (int32_t)(kMixWaves * g);
+            task.count = (int32_t)min((int64_t)kMixWaves, cnt - kMixWaves * g);
             key = (sorted_keys[task.start] & 0x0fffffffu) + 1;
         }
     }
@@ -164,17 +196,35 @@ __global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
     tidx[t] = (int32_t)t;
 }
 
+// The WFA paths run once per task, so they are called out of line: inlined
+// beside the register SW layout they would push the kernel past 256 VGPRs.
+__device__ __noinline__ int mix_wfa_block(const uint64_t *q, int qlen, const uint64_t *r, int rlen,
+                                          int me, WfaPen p, wfadev::WfaGeom gm, uint8_t *lds,
+                                          int lds_bytes, int32_t *slab, int64_t slab_words,
+                                          int tile_steps) {
+    return wfadev::wfa_pair_block<kMixWfaUnroll>(q, qlen, r, rlen, me, p, gm, lds, lds_bytes,
+                                                 slab, slab_words, tile_steps);
+}
+
+__device__ __noinline__ int mix_wfa_wave(const uint64_t *q, int qlen, const uint64_t *r, int rlen,
+                                         int me, WfaPen p, wfadev::WfaGeom gm, uint8_t *part) {
+    return wfadev::wfa_pair_wave<kMixWfaUnroll>(q, qlen, r, rlen, me, p, gm, part);
+}
+
 __global__ void mix_zero_kernel(int32_t *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
 // The persistent kernel: four-wavefront workgroups take tasks in order.
+// WITH_WFA = false drops the WFA task code for batches known to hold SW
+// pairs only, so the register SW layout keeps its registers to itself.
+template <bool WITH_WFA>
 __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, const int32_t *me_arr,
                       int32_t me_scalar, SwScoring sc, WfaPen p, wfadev::WfaGeom gm, SwOut out,
                       const int32_t *order, const Task *tasks, const int32_t *task_order,
-                      const int32_t *n_tasks, int32_t *queue, int32_t lds_bytes, int32_t *slab,
-                      int64_t slab_words) {
+                      const int32_t *n_tasks, int32_t *queue, int32_t lds_bytes,
+                      int32_t tile_steps, int32_t *slab, int64_t slab_words) {
     extern __shared__ uint8_t lds_raw[];
     __shared__ int next_task;
     int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
@@ -198,15 +248,31 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
                     swdev::sw_reg_group<kPackedBand, kPackedBand, true>(
                         b, band_arr, band_scalar, sc, out, order, start, count);
             }
-        } else {
+        } else if (!WITH_WFA) {
+            // unreachable: WFA tasks only exist when the batch has WFA pairs
+        } else if (task.type == kTaskWfa) {
             const int64_t pair = order[task.start];
             const uint64_t *q, *r;
             int qlen, rlen;
             load_pair(b, pair, q, r, qlen, rlen);  // validated by the classifier
             const int me = me_arr ? me_arr[pair] : me_scalar;
-            const int res = wfadev::wfa_pair_block<kMixWfaUnroll>(
-                q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes, my_slab, slab_words);
+            const int res = mix_wfa_block(q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes, my_slab,
+                                          slab_words, tile_steps);
             if (threadIdx.x == 0) {
+                out.score[pair] = res;
+                if (out.q_end) out.q_end[pair] = -1;
+                if (out.r_end) out.r_end[pair] = -1;
+            }
+        } else if (wave < task.count) {
+            // class 6: each wavefront aligns its own pair in its LDS part
+            const int64_t pair = order[task.start + wave];
+            const uint64_t *q, *r;
+            int qlen, rlen;
+            load_pair(b, pair, q, r, qlen, rlen);  // validated by the classifier
+            const int me = me_arr ? me_arr[pair] : me_scalar;
+            const int res =
+                mix_wfa_wave(q, qlen, r, rlen, me, p, gm, lds_raw + wave * wave_part_bytes(lds_bytes));
+            if ((threadIdx.x & 63) == 0) {
                 out.score[pair] = res;
                 if (out.q_end) out.q_end[pair] = -1;
                 if (out.r_end) out.r_end[pair] = -1;
@@ -225,8 +291,8 @@ struct MixPlan {
     int32_t me_cap = 0;
     WfaPen pen{};
     wfadev::WfaGeom gm{};
-    int grid = 0;
-    int lds_bytes = 0;
+    int cus = 0;
+    MixTuning tune{};
     // workspace
     uint32_t *keys = nullptr, *keys_out = nullptr, *tkeys = nullptr, *tkeys_out = nullptr;
     int32_t *idx = nullptr, *order = nullptr, *tidx = nullptr, *task_order = nullptr;
@@ -252,15 +318,74 @@ struct MixPlan {
     }
 };
 
+namespace {
+std::mutex g_tune_mu;
+MixTuning g_default_tuning{};
+}  // namespace
+
+void tuning_to_array(const MixTuning &t, int32_t *a) {
+    a[0] = t.blocks_per_cu;
+    a[1] = t.lds_bytes;
+    a[2] = t.tile_steps;
+    a[3] = t.wave_max_width;
+    a[4] = t.reg_max_band;
+    a[5] = t.lane_max_len;
+}
+
+int tuning_from_array(const int32_t *a, MixTuning *t) {
+    MixTuning v;
+    v.blocks_per_cu = a[0];
+    v.lds_bytes = a[1] & ~255;
+    v.tile_steps = a[2];
+    v.wave_max_width = a[3];
+    v.reg_max_band = a[4];
+    v.lane_max_len = a[5];
+    if (v.blocks_per_cu < 1 || v.blocks_per_cu > kMixMaxBlocksPerCu)
+        return set_error(CGM_ERR_INVALID, "tuning: blocks_per_cu must be 1 or 2");
+    // static LDS of the kernel (a few ints) needs room beside the dynamic part
+    if (v.lds_bytes < 4096 || (int64_t)v.blocks_per_cu * (v.lds_bytes + 256) > kMixMaxLds)
+        return set_error(CGM_ERR_INVALID, "tuning: lds_bytes out of range for blocks_per_cu");
+    if (v.tile_steps < 1 || v.tile_steps > 1024)
+        return set_error(CGM_ERR_INVALID, "tuning: tile_steps out of range");
+    if (v.wave_max_width < 0) return set_error(CGM_ERR_INVALID, "tuning: wave_max_width < 0");
+    if (v.reg_max_band < 0 || v.reg_max_band > 64)
+        return set_error(CGM_ERR_INVALID, "tuning: reg_max_band out of range");
+    if (v.lane_max_len < 1) return set_error(CGM_ERR_INVALID, "tuning: lane_max_len < 1");
+    *t = v;
+    return CGM_OK;
+}
+
+MixTuning mix_default_tuning() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_default_tuning;
+}
+
+void mix_set_default_tuning(const MixTuning &t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_default_tuning = t;
+}
+
+MixTuning mix_plan_tuning(const MixPlan *P) { return P->tune; }
+
+void mix_plan_set_tuning(MixPlan *P, const MixTuning &t) { P->tune = t; }
+
+// waves_per_cu / lds_bytes > 0 override the default tuning's shape.
 int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per_cu,
                     int lds_bytes, MixPlan **out) {
     if (max_pairs <= 0 || max_pairs > (int64_t(1) << 31) - 1)
         return set_error(CGM_ERR_INVALID, "plan: max_pairs out of range");
     if (me_cap < 0 || me_cap > kMaxWfaError) return set_error(CGM_ERR_INVALID, "plan: bad max_error cap");
     if (int rc = check_wfa_pen(pen)) return rc;
-    if (waves_per_cu < 1 || waves_per_cu > 32) return set_error(CGM_ERR_INVALID, "plan: waves_per_cu");
-    if (lds_bytes < 0 || lds_bytes > 160 * 1024) return set_error(CGM_ERR_INVALID, "plan: lds bytes");
+    if (waves_per_cu < 0 || waves_per_cu > 32) return set_error(CGM_ERR_INVALID, "plan: waves_per_cu");
+    if (lds_bytes < 0 || lds_bytes > kMixMaxLds) return set_error(CGM_ERR_INVALID, "plan: lds bytes");
+    MixTuning tune = mix_default_tuning();
+    if (waves_per_cu > 0)
+        tune.blocks_per_cu = std::min(kMixMaxBlocksPerCu, (waves_per_cu + kMixWaves - 1) / kMixWaves);
+    if (lds_bytes > 0) tune.lds_bytes = std::max(4096, lds_bytes & ~255);
+    if ((int64_t)tune.blocks_per_cu * (tune.lds_bytes + 256) > kMixMaxLds) tune.blocks_per_cu = 1;
+    if ((int64_t)tune.lds_bytes + 256 > kMixMaxLds) tune.lds_bytes = kMixMaxLds - 256;
     MixPlan *P = new MixPlan();
+    P->tune = tune;
     auto fail = [&](int rc) {
         delete P;
         return rc;
@@ -271,10 +396,7 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
     P->pen = pen;
     wfa_geometry(pen, &P->gm.g, &P->gm.nM, &P->gm.nI);
     if (hipGetDevice(&P->device) != hipSuccess) return fail(set_error(CGM_ERR_HIP, "hipGetDevice"));
-    int cus = 0;
-    if (int rc = device_cu_count(&cus)) return fail(rc);
-    P->grid = cus * ((waves_per_cu + 3) / 4);
-    P->lds_bytes = lds_bytes & ~255;
+    if (int rc = device_cu_count(&P->cus)) return fail(rc);
     const int64_t n = max_pairs, nt = P->task_cap;
 #define CGM_ALLOC(ptr, nbytes)                                                     \
     do {                                                                           \
@@ -300,13 +422,18 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
         return fail(set_error(CGM_ERR_HIP, "plan: sort size query"));
     P->sort_tmp_bytes = b1 > b2 ? b1 : b2;
     CGM_ALLOC(P->sort_tmp, P->sort_tmp_bytes + 256);
-    // HBM wavefront slab per workgroup for WFA pairs too large for LDS.
+    // HBM wavefront slab per workgroup (for the largest grid any tuning
+    // launches) for WFA pairs too large for LDS.
     const int K = me_cap < reach_host(me_cap, pen) ? me_cap : reach_host(me_cap, pen);
     P->slab_words = (int64_t)(P->gm.nM + 2 * P->gm.nI) * (2 * (int64_t)K + 1);
-    CGM_ALLOC(P->slab, (int64_t)P->grid * P->slab_words * 4);
+    CGM_ALLOC(P->slab, (int64_t)P->cus * kMixMaxBlocksPerCu * P->slab_words * 4);
 #undef CGM_ALLOC
-    if (hipFuncSetAttribute((const void *)mix_persistent_kernel,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, P->lds_bytes) != hipSuccess)
+    if (hipFuncSetAttribute((const void *)mix_persistent_kernel<true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kMixMaxLds - 256) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void *)mix_persistent_kernel<false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kMixMaxLds - 256) !=
+            hipSuccess)
         return fail(set_error(CGM_ERR_HIP, "plan: LDS attribute"));
     if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess)
         return fail(set_error(CGM_ERR_HIP, "plan: timing events"));
@@ -363,7 +490,7 @@ int cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen,
     // grow geometrically so a slowly growing batch size does not rebuild often
     int64_t cap = 1024;
     while (cap < n_pairs) cap *= 2;
-    int rc = mix_plan_create(cap, me_cap, pen, kMixDefaultWavesPerCu, kMixDefaultLdsBytes, &slot);
+    int rc = mix_plan_create(cap, me_cap, pen, 0, 0, &slot);
     if (rc) {
         slot = nullptr;
         return rc;
@@ -389,9 +516,10 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
             *queue = P->small + 25;
     const int64_t n = b.n_pairs, nt = P->task_cap;
     hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(64), 0, stream, P->small, 64);
+    const MixTuning tune = P->tune;
     hipLaunchKernelGGL(mix_class_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, b,
                        kind, kind_scalar, band, band_scalar, me, me_scalar, P->me_cap, sc, pen,
-                       P->gm.g, out, P->keys, P->idx, counts);
+                       P->gm, tune, out, P->keys, P->idx, counts);
     hipLaunchKernelGGL(mix_ranges_kernel, dim3(1), dim3(64), 0, stream, counts, ranges, n_tasks);
     size_t tb = P->sort_tmp_bytes;
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->keys, P->keys_out,
@@ -405,20 +533,20 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                                                          P->tidx, P->task_order, (int)tn, 0, 32,
                                                          stream));
     CGM_HIP(hipEventRecord(P->ev_start, stream));
-    hipLaunchKernelGGL(mix_persistent_kernel, dim3(P->grid), dim3(kMixThreads),
-                       (size_t)P->lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
+    const bool sw_only = kind == nullptr && kind_scalar == 0;
+    hipLaunchKernelGGL(sw_only ? mix_persistent_kernel<false> : mix_persistent_kernel<true>,
+                       dim3(P->cus * tune.blocks_per_cu), dim3(kMixThreads),
+                       (size_t)tune.lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
                        P->gm, out, P->order, P->tasks, P->task_order, n_tasks, queue,
-                       P->lds_bytes, P->slab, P->slab_words);
+                       tune.lds_bytes, tune.tile_steps, P->slab, P->slab_words);
     CGM_HIP(hipGetLastError());
     CGM_HIP(hipEventRecord(P->ev_stop, stream));
     // uncommon SW classes: LDS lanes, then one workgroup per pair
-    int cus = 0;
-    if (int rc = device_cu_count(&cus)) return rc;
     if (int rc = launch_sw_lds_lane(b, band, band_scalar, sc, out, P->order, ranges + 2 * kMixSwLds,
-                                    cus * 2, stream))
+                                    P->cus * 2, stream))
         return rc;
     return launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
-                                 ranges + 2 * kMixSwPlain, cus * 2, stream);
+                                 ranges + 2 * kMixSwPlain, P->cus * 2, stream);
 }
 
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
@@ -451,6 +579,110 @@ int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
 int mix_replay(MixPlan *P, hipStream_t stream) {
     if (!P->exec) return set_error(CGM_ERR_INVALID, "no captured graph in this plan");
     CGM_HIP(hipGraphLaunch(P->exec, stream));
+    return CGM_OK;
+}
+
+int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
+                 SwScoring sc, WfaPen pen, SwOut out, int reps, hipStream_t stream,
+                 MixTuning *best, float *best_ms, float *start_ms) {
+    if (reps < 1 || reps > 100) return set_error(CGM_ERR_INVALID, "autotune: reps out of range");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CGM_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(CGM_ERR_HIP, "autotune: events");
+    }
+    const MixTuning start = P->tune;
+    // best of `reps` timed enqueues of the whole pipeline under tuning t
+    auto measure = [&](const MixTuning &t, float *ms) -> int {
+        P->tune = t;
+        float lo = 3.4e38f;
+        for (int i = 0; i < reps; ++i) {
+            CGM_HIP(hipEventRecord(e0, stream));
+            if (int rc = mix_enqueue(P, b, kind, kind_scalar, band, band_scalar, me, me_scalar, sc,
+                                     pen, out, stream))
+                return rc;
+            CGM_HIP(hipEventRecord(e1, stream));
+            CGM_HIP(hipEventSynchronize(e1));
+            float x = 0;
+            CGM_HIP(hipEventElapsedTime(&x, e0, e1));
+            lo = std::min(lo, x);
+        }
+        *ms = lo;
+        return CGM_OK;
+    };
+    int rc = CGM_OK;
+    MixTuning cur = start;
+    float cur_ms = 0;
+    // warm-up enqueue (first-touch of the plan's buffers), then the start
+    if (!rc) rc = measure(cur, &cur_ms);
+    if (!rc) rc = measure(cur, &cur_ms);
+    if (!rc && start_ms) *start_ms = cur_ms;
+    // Coordinate descent: each field in turn over its candidates, keeping a
+    // change only when it is at least 2 % faster.
+    auto try_all = [&](const MixTuning *cands, int nc) {
+        for (int i = 0; i < nc && !rc; ++i) {
+            float ms = 0;
+            rc = measure(cands[i], &ms);
+            if (!rc && ms < 0.98f * cur_ms) {
+                cur = cands[i];
+                cur_ms = ms;
+            }
+        }
+    };
+    {   // workgroup shape: workgroups per CU and LDS per workgroup
+        const int shapes[][2] = {{2, 48 * 1024}, {2, 64 * 1024}, {2, 79 * 1024},
+                                 {1, 96 * 1024}, {1, 128 * 1024}, {1, 159 * 1024}};
+        MixTuning c[6];
+        int nc = 0;
+        for (auto &sh : shapes) {
+            if (sh[0] == cur.blocks_per_cu && sh[1] == cur.lds_bytes) continue;
+            c[nc] = cur;
+            c[nc].blocks_per_cu = sh[0];
+            c[nc].lds_bytes = sh[1];
+            ++nc;
+        }
+        try_all(c, nc);
+    }
+    {   // WFA tile height (scores per block of the tiled layout)
+        const int v[] = {16, 32, 64};
+        MixTuning c[3];
+        int nc = 0;
+        for (int x : v)
+            if (x != cur.tile_steps) (c[nc] = cur, c[nc++].tile_steps = x);
+        try_all(c, nc);
+    }
+    {   // switch-over between one wavefront and one workgroup per WFA pair
+        const int v[] = {0, 256, 512, 1024, 2048, 1 << 30};
+        MixTuning c[6];
+        int nc = 0;
+        for (int x : v)
+            if (x != cur.wave_max_width) (c[nc] = cur, c[nc++].wave_max_width = x);
+        try_all(c, nc);
+    }
+    {   // band split between register lanes and LDS lanes
+        const int v[] = {32, 48, 64};
+        MixTuning c[3];
+        int nc = 0;
+        for (int x : v)
+            if (x != cur.reg_max_band) (c[nc] = cur, c[nc++].reg_max_band = x);
+        try_all(c, nc);
+    }
+    {   // switch-over between one lane and one workgroup per SW pair
+        const int v[] = {1024, 4096, 1 << 30};
+        MixTuning c[3];
+        int nc = 0;
+        for (int x : v)
+            if (x != cur.lane_max_len) (c[nc] = cur, c[nc++].lane_max_len = x);
+        try_all(c, nc);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->tune = rc ? start : cur;
+    if (rc) return rc;
+    *best = cur;
+    if (best_ms) *best_ms = cur_ms;
     return CGM_OK;
 }
 

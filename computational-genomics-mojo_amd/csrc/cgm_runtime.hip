@@ -418,9 +418,8 @@ int cgm_plan_create(int64_t max_pairs, int32_t max_error_cap, int32_t x, int32_t
                     int32_t waves_per_cu, int32_t lds_bytes, void **plan) {
     if (!plan) return set_error(CGM_ERR_INVALID, "null plan pointer");
     MixPlan *P = nullptr;
-    int rc = mix_plan_create(max_pairs, max_error_cap, WfaPen{x, o, e},
-                             waves_per_cu > 0 ? waves_per_cu : kMixDefaultWavesPerCu,
-                             lds_bytes > 0 ? lds_bytes : kMixDefaultLdsBytes, &P);
+    int rc = mix_plan_create(max_pairs, max_error_cap, WfaPen{x, o, e}, waves_per_cu, lds_bytes,
+                             &P);
     if (rc) return rc;
     *plan = P;
     return CGM_OK;
@@ -491,6 +490,58 @@ int cgm_plan_kernel_ms(void *plan, float *ms) {
 int cgm_plan_replay(void *plan, void *stream) {
     if (!plan) return set_error(CGM_ERR_INVALID, "null plan");
     return mix_replay((MixPlan *)plan, (hipStream_t)stream);
+}
+
+int cgm_plan_get_tuning(void *plan, int32_t *params) {
+    if (!plan || !params) return set_error(CGM_ERR_INVALID, "null argument");
+    tuning_to_array(mix_plan_tuning((MixPlan *)plan), params);
+    return CGM_OK;
+}
+
+int cgm_plan_set_tuning(void *plan, const int32_t *params) {
+    if (!plan || !params) return set_error(CGM_ERR_INVALID, "null argument");
+    MixTuning t;
+    if (int rc = tuning_from_array(params, &t)) return rc;
+    mix_plan_set_tuning((MixPlan *)plan, t);
+    return CGM_OK;
+}
+
+int cgm_tuning_get_default(int32_t *params) {
+    if (!params) return set_error(CGM_ERR_INVALID, "null argument");
+    tuning_to_array(mix_default_tuning(), params);
+    return CGM_OK;
+}
+
+int cgm_tuning_set_default(const int32_t *params) {
+    if (!params) return set_error(CGM_ERR_INVALID, "null argument");
+    MixTuning t;
+    if (int rc = tuning_from_array(params, &t)) return rc;
+    mix_set_default_tuning(t);
+    return CGM_OK;
+}
+
+int cgm_plan_autotune(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                      const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                      const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                      const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                      int32_t band_scalar, const int32_t *max_error, int32_t max_error_scalar,
+                      int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                      int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                      int32_t *out_r_end, int32_t reps, int32_t *best_params, float *best_ms,
+                      float *start_ms, void *stream) {
+    if (!plan || !best_params) return set_error(CGM_ERR_INVALID, "null argument");
+    Batch b = {q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words, r_words};
+    SwScoring sc = {match, mismatch, gap_open, gap_extend};
+    SwOut out = {out_score, out_q_end, out_r_end};
+    if (n_pairs > 0 && (!out_score || !out_q_end || !out_r_end))
+        return set_error(CGM_ERR_INVALID, "null output array");
+    MixTuning best;
+    if (int rc = mix_autotune((MixPlan *)plan, b, kind, kind_scalar, band, band_scalar, max_error,
+                              max_error_scalar, sc, WfaPen{x, o, e}, out, reps,
+                              (hipStream_t)stream, &best, best_ms, start_ms))
+        return rc;
+    tuning_to_array(best, best_params);
+    return CGM_OK;
 }
 
 int cgm_sw_banded_host(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_len,

@@ -29,7 +29,7 @@ namespace wfadev {
 // ---------------------------------------------------------------------------
 constexpr int kWaveThreads = 64;
 constexpr int kNegRow = -(1 << 30);
-constexpr int kTileSteps = 32;      // score steps per block of the tiled layout
+constexpr int kTileSteps = 32;      // default score steps per block of the tiled layout
 constexpr int kTileMinWidth = 256;  // narrowest tile worth the halo
 
 __device__ __forceinline__ uint64_t window64(const uint64_t *s, int pos, int nwords) {
@@ -341,6 +341,111 @@ __device__ int wfa_pair_tiled(const uint64_t *q, int qlen, const uint64_t *r, in
     return -1;
 }
 
+// Orders the LDS accesses of one wavefront's lanes: a wavefront's LDS
+// operations complete in issue order, so only the compiler must be kept
+// from moving accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS bytes the one-wavefront layout needs for a pair: both sequences and
+// the int16 wavefront slots (0 when the pair ends before any score).
+__device__ __forceinline__ int64_t wave_pair_bytes(int qlen, int rlen, int me, const WfaPen &p,
+                                                   const WfaGeom &gm) {
+    const int K = min(min(me, max(qlen, rlen)), reach(me, p));
+    const int64_t W = 2 * (int64_t)K + 1;
+    return 8 * (int64_t)(((qlen + 31) >> 5) + ((rlen + 31) >> 5)) +
+           2 * (int64_t)(gm.nM + 2 * gm.nI) * W;
+}
+
+// One pair per wavefront: the 64 lanes stride over the diagonals of each
+// score with the team recurrence, sequences and int16 wavefronts in the
+// wavefront's own LDS part (`part`, at least wave_pair_bytes; the caller
+// checks rows_fit_int16).  Steps are ordered by wave_sync and the pair ends
+// on the wavefront's vote, so small pairs pay no workgroup barrier.
+template <int U>
+__device__ int wfa_pair_wave(const uint64_t *qg, int qlen, const uint64_t *rg, int rlen, int me,
+                             const WfaPen &p, const WfaGeom &gm, uint8_t *part) {
+    const int lane = threadIdx.x & 63;
+    if (qlen == 0 && rlen == 0) return 0;
+    const int target = qlen - rlen;
+    const int K = min(min(me, max(qlen, rlen)), reach(me, p));
+    const int W = 2 * K + 1;
+    if (target != 0 && (target > K || -target > K)) return -1;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int slots = gm.nM + 2 * gm.nI;
+    uint64_t *seq = (uint64_t *)part;
+    for (int i = lane; i < qw + rw; i += kWaveThreads) seq[i] = i < qw ? qg[i] : rg[i - qw];
+    const uint64_t *q = seq, *r = seq + qw;
+    int16_t *M = (int16_t *)(part + 8 * (qw + rw));
+    int16_t *I = M + gm.nM * W;
+    int16_t *D = I + gm.nI * W;
+    const int16_t neg = RowNeg<int16_t>::v;
+    for (int i = lane; i < slots * W; i += kWaveThreads) M[i] = neg;
+    wave_sync();
+    if (lane == 0) M[K] = (int16_t)extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+    wave_sync();
+    if (target == 0 && (int)M[K] >= qlen) return 0;
+    WfaSteps st;
+    st.init(p, gm);
+    for (int s = gm.g; s <= me; s += gm.g) {
+        st.next(p, gm);
+        const int ms = st.ms * W, is = st.is * W;
+        const int lo = max(max(-rlen, -me), -st.R), hi = min(min(qlen, me), st.R);
+        const int mx = st.mx(gm) >= 0 ? st.mx(gm) * W : -1;
+        const int mo = st.mo(gm) >= 0 ? st.mo(gm) * W : -1;
+        const int ie = st.ie(gm) >= 0 ? st.ie(gm) * W : -1;
+        int found = 0;
+        for (int k0 = lo + lane; k0 <= hi; k0 += U * kWaveThreads) {
+            int iv[U], dv[U], mv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * kWaveThreads, di = k + K;
+                int a = kNegRow, c = kNegRow, bb = kNegRow, d = kNegRow, m = kNegRow;
+                if (k <= hi) {
+                    if (mo >= 0) {
+                        if (di > 0) a = M[mo + di - 1];
+                        if (di < W - 1) bb = M[mo + di + 1];
+                    }
+                    if (ie >= 0) {
+                        if (di > 0) c = I[ie + di - 1];
+                        if (di < W - 1) d = D[ie + di + 1];
+                    }
+                    if (mx >= 0) m = M[mx + di];
+                }
+                iv[u] = max(a, c) + 1;
+                dv[u] = max(bb, d);
+                mv[u] = max(max(m + 1, iv[u]), dv[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * kWaveThreads, di = k + K;
+                if (k <= hi) {
+                    I[is + di] = iv[u] < 0 ? neg : (int16_t)iv[u];
+                    D[is + di] = dv[u] < 0 ? neg : (int16_t)dv[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * kWaveThreads, di = k + K;
+                if (k <= hi) {
+                    int m = mv[u];
+                    if (m >= 0) {
+                        m = extend_diag(q, qlen, qw, r, rlen, rw, m, k);
+                        found |= (k == target && m >= qlen);
+                    }
+                    M[ms + di] = m < 0 ? neg : (int16_t)m;
+                }
+            }
+        }
+        if (__ballot(found)) return s;
+        wave_sync();
+    }
+    return -1;
+}
+
 // Everything one workgroup needs to align a WFA pair: sequences staged to
 // LDS when they fit `lds_bytes`; int16 wavefronts in LDS when they fit the
 // rest, otherwise in the workgroup's HBM slab (int16 rows when they fit
@@ -349,7 +454,8 @@ template <int U>
 __device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, const uint64_t *rg,
                                               int rlen, int me, const WfaPen &p,
                                               const WfaGeom &gm, uint8_t *lds, int lds_bytes,
-                                              int32_t *slab, int64_t slab_words) {
+                                              int32_t *slab, int64_t slab_words,
+                                              int tile_steps = kTileSteps) {
     if (qlen == 0 && rlen == 0) return 0;
     const int target = qlen - rlen;
     const int K = min(min(me, max(qlen, rlen)), reach(me, p));
@@ -373,9 +479,9 @@ __device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, cons
             return wfa_pair_team<U>(q, qlen, r, rlen, me, p, gm, (int16_t *)(lds + seq_bytes), W, K);
         // Tiled: two int16 slab sets (slots * W int32 words) and LDS tiles.
         const int LW = (int)((lds_bytes - seq_bytes) / (2 * slots));
-        if (fit16 && LW - 2 * kTileSteps >= kTileMinWidth && (int64_t)slots * W <= slab_words)
+        if (fit16 && LW - 2 * tile_steps >= kTileMinWidth && (int64_t)slots * W <= slab_words)
             return wfa_pair_tiled<U>(q, qlen, r, rlen, me, p, gm, (int16_t *)slab, W, K,
-                                     (int16_t *)(lds + seq_bytes), LW, kTileSteps);
+                                     (int16_t *)(lds + seq_bytes), LW, tile_steps);
     }
     // Untiled HBM slab of slab_words int32 words: int16 rows when they fit
     // (half the traffic), int32 otherwise.

@@ -135,8 +135,9 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
  * cgm_align_mixed performs no allocation and no host synchronisation and
  * can be captured: cgm_plan_capture records one call into a hipGraph kept
  * in the plan, cgm_plan_replay launches it again (same pointers; refill
- * their contents between replays).  waves_per_cu / lds_bytes: 0 = tuned
- * defaults.  A plan serves one device and one call at a time. */
+ * their contents between replays).  waves_per_cu / lds_bytes: 0 = the
+ * default tuning's (below), else they set its workgroup shape.  A plan
+ * serves one device and one call at a time. */
 int cgm_plan_create(int64_t max_pairs, int32_t max_error_cap, int32_t x, int32_t o, int32_t e,
                     int32_t waves_per_cu, int32_t lds_bytes, void **plan);
 int cgm_plan_destroy(void *plan);
@@ -162,6 +163,49 @@ int cgm_plan_replay(void *plan, void *stream);
 /* Duration of the persistent kernel of the plan's last call (or replay),
  * from events recorded around it on the call's stream; waits for it. */
 int cgm_plan_kernel_ms(void *plan, float *ms);
+
+/* ------------------------------------------------------------ autotuning
+ * Layout choices of the persistent kernel, as an int32 array of
+ * CGM_TUNE_N entries (no reference counterpart: the reference has one
+ * launch shape per kernel, align_gpu.mojo:24-38).  None changes a result.
+ *   BLOCKS_PER_CU   four-wavefront workgroups per CU (1 or 2)
+ *   LDS_BYTES       LDS per workgroup (blocks_per_cu x (lds + 256) <= 160 KiB)
+ *   TILE_STEPS      WFA scores per block of the tiled layout (pairs whose
+ *                   wavefronts exceed LDS)
+ *   WAVE_MAX_WIDTH  switch-over: WFA pairs with at most this many diagonals
+ *                   (2 R(max_error) + 1) that fit a quarter of the LDS run one
+ *                   per wavefront, the others one per workgroup (0: never)
+ *   REG_MAX_BAND    band split: SW bands up to this (<= 64) run in register
+ *                   lanes, wider ones in LDS lanes (band 128: packed lanes)
+ *   LANE_MAX_LEN    switch-over: SW pairs longer than this run one workgroup
+ *                   per pair instead of one lane per pair
+ * cgm_plan_autotune times the whole pipeline on the given batch for
+ * candidate values of each field in turn (coordinate descent, best of
+ * `reps` runs per candidate), keeps a change only if it is >= 2 % faster,
+ * leaves the best tuning in the plan and returns it with its time and the
+ * starting tuning's time; the outputs hold the batch's results.  New plans
+ * (and the library's own plans behind CGM_LAYOUT_AUTO and the serving
+ * engine) start from the process default tuning. */
+#define CGM_TUNE_BLOCKS_PER_CU 0
+#define CGM_TUNE_LDS_BYTES 1
+#define CGM_TUNE_TILE_STEPS 2
+#define CGM_TUNE_WAVE_MAX_WIDTH 3
+#define CGM_TUNE_REG_MAX_BAND 4
+#define CGM_TUNE_LANE_MAX_LEN 5
+#define CGM_TUNE_N 6
+int cgm_plan_get_tuning(void *plan, int32_t *params);
+int cgm_plan_set_tuning(void *plan, const int32_t *params);
+int cgm_tuning_get_default(int32_t *params);
+int cgm_tuning_set_default(const int32_t *params);
+int cgm_plan_autotune(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                      const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                      const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                      const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                      int32_t band_scalar, const int32_t *max_error, int32_t max_error_scalar,
+                      int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                      int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                      int32_t *out_r_end, int32_t reps, int32_t *best_params, float *best_ms,
+                      float *start_ms, void *stream);
 
 /* ------------------------------------------------------ serving engine
  * Continuous batching for a stream of alignment requests (no reference

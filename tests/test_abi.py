@@ -7,6 +7,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from conftest import ROOT, load_oracle
 
@@ -134,3 +135,24 @@ def test_subset_repacks(cgm):
         a = w.q_packed[w.q_off[i]: w.q_off[i] + (w.q_len[i] + 31) // 32]
         b = s.q_packed[s.q_off[k]: s.q_off[k] + (s.q_len[k] + 31) // 32]
         np.testing.assert_array_equal(a, b)
+
+
+def test_default_tuning_roundtrip_and_validation(cgm):
+    """The process default tuning (host state only) reads back what was set
+    and rejects shapes the persistent kernel cannot launch."""
+    t0 = cgm.default_tuning()
+    assert set(t0) == set(cgm._lib.TUNE_FIELDS)
+    try:
+        cgm.set_default_tuning(wave_max_width=700, tile_steps=48, reg_max_band=48)
+        t = cgm.default_tuning()
+        assert (t["wave_max_width"], t["tile_steps"], t["reg_max_band"]) == (700, 48, 48)
+        bad = [dict(blocks_per_cu=3), dict(blocks_per_cu=2, lds_bytes=100 * 1024),
+               dict(blocks_per_cu=1, lds_bytes=160 * 1024), dict(tile_steps=0),
+               dict(reg_max_band=65), dict(lane_max_len=0), dict(wave_max_width=-1)]
+        for b in bad:
+            with pytest.raises(cgm._lib.CgmError):
+                cgm.set_default_tuning(**b)
+        assert cgm.default_tuning() == t
+    finally:
+        cgm.set_default_tuning(**t0)
+    assert cgm.default_tuning() == t0

@@ -103,3 +103,69 @@ def test_mixed_is_deterministic(cgm, dev, gen_mod):
     a = cgm.align_mixed_batch(dev, w, sw_cfg(cgm))
     b = cgm.align_mixed_batch(dev, w, sw_cfg(cgm))
     assert_same(a, b, "run twice")
+
+
+TUNINGS = [
+    {},
+    {"wave_max_width": 0},
+    {"wave_max_width": 1 << 30, "blocks_per_cu": 2, "lds_bytes": 79 * 1024},
+    {"blocks_per_cu": 1, "lds_bytes": 159 * 1024, "tile_steps": 64},
+    {"reg_max_band": 32, "wave_max_width": 512},
+    {"lane_max_len": 1024, "tile_steps": 16, "lds_bytes": 24 * 1024},
+]
+
+
+def upload_mixed(dev, w):
+    nz = lambda a: a if a.size else np.zeros(1, a.dtype)  # noqa: E731
+    bufs = tuple(dev.upload(nz(a)) for a in (w.q_packed, w.r_packed, w.q_len, w.r_len, w.q_off,
+                                             w.r_off))
+    return bufs, tuple(dev.upload(nz(a)) for a in (w.kind, w.band, w.max_error))
+
+
+def run_plan(cgm, dev, plan, bufs, arrs, n):
+    out = cgm.SwResult(n)
+    plan.align(dev, bufs, n, *arrs, sw_cfg(cgm), out)
+    dev.synchronize()
+    return tuple(dev.download(a, n) for a in (out.scores, out.query_end, out.ref_end))
+
+
+def test_mixed_every_tuning_equals_oracle(cgm, dev, orc, gen_mod):
+    """Every layout choice (per-wavefront vs per-workgroup WFA, LDS budget,
+    tile height, band split, per-lane vs per-workgroup SW) gives the
+    oracle's results on a batch that exercises all of them."""
+    big = gen_mod.generate(4, 4000, seed=21)
+    wf = np.nonzero(big.kind == 1)[0]
+    big = big.subset(wf[np.argsort(big.max_error[wf])][[0, 5, 20, 60, 150, 250]])
+    w = gen_mod.concat([mixed_sample(gen_mod, seed=13), big])
+    exp = oracle_mixed(orc, w)
+    bufs, arrs = upload_mixed(dev, w)
+    plan = cgm.MixedPlan(w.n_pairs, int(w.max_error.max()))
+    base = plan.tuning()
+    for t in TUNINGS:
+        print(f"[case] tuning {t}", flush=True)
+        plan.set_tuning(**{**base, **t})
+        assert plan.tuning() == {**base, **t}
+        assert_same(run_plan(cgm, dev, plan, bufs, arrs, w.n_pairs), exp, f"tuning {t}")
+    plan.close()
+
+
+def test_mixed_autotune_keeps_results_and_tuning(cgm, dev, orc, gen_mod):
+    w = mixed_sample(gen_mod, seed=17)
+    exp = oracle_mixed(orc, w)
+    bufs, arrs = upload_mixed(dev, w)
+    plan = cgm.MixedPlan(w.n_pairs, int(w.max_error.max()))
+    out = cgm.SwResult(w.n_pairs)
+    print("[case] autotune mixed sample", flush=True)
+    best, best_ms, start_ms = plan.autotune(dev, bufs, w.n_pairs, *arrs, sw_cfg(cgm), out, reps=1)
+    dev.synchronize()
+    got = tuple(dev.download(a, w.n_pairs) for a in (out.scores, out.query_end, out.ref_end))
+    assert_same(got, exp, "autotune outputs")
+    assert plan.tuning() == best
+    assert 0 < best_ms <= start_ms * 1.5
+    assert_same(run_plan(cgm, dev, plan, bufs, arrs, w.n_pairs), exp, "after autotune")
+    with pytest.raises(cgm._lib.CgmError):
+        plan.set_tuning(blocks_per_cu=2, lds_bytes=100 * 1024)
+    with pytest.raises(cgm._lib.CgmError):
+        plan.set_tuning(reg_max_band=65)
+    assert plan.tuning() == best
+    plan.close()
