@@ -82,11 +82,30 @@ class SwRunner:
 
 def wfa_extra(m, dev, gen, n_pairs=20_000):
     """Gap-affine WFA on 20,000 pairs of 1-10 kb (x=4, o=6, e=2): pairs/s of
-    the wave layout and of the one-workgroup-per-pair layout."""
+    the mixed persistent layout on an autotuned plan, of the automatic
+    launcher (the WFA persistent workgroup-per-pair kernel) and of the plain
+    one-workgroup-per-pair baseline."""
     w = gen.generate(3, n_pairs)
     up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
-                                                  "q_off", "r_off", "max_error")}
+                                                  "q_off", "r_off", "max_error", "kind", "band")}
     out = m.DeviceArray(w.n_pairs, np.int32)
+    res = {}
+    costs = {}
+    n = w.n_pairs
+    bufs = tuple(up[k] for k in ("q_packed", "r_packed", "q_len", "r_len", "q_off", "r_off"))
+    cfg = m.AlignConfig(SW["match"], SW["mismatch"], SW["gap_open"], SW["gap_extend"], 0)
+    plan = m.MixedPlan(n, int(w.max_error.max()))
+    pout = m.SwResult(n)
+    tuning, best_ms, start_ms = plan.autotune(dev, bufs, n, up["kind"], up["band"],
+                                              up["max_error"], cfg, pout, reps=2)
+    wall, kms = time_steps(m, dev, lambda: plan.align(dev, bufs, n, up["kind"], up["band"],
+                                                      up["max_error"], cfg, pout), 3, 1)
+    costs["persistent"] = dev.download(pout.scores, n)
+    res["persistent"] = {"pairs_per_s": n * 3 / wall, "ms_per_step": wall / 3 * 1e3,
+                         "tuning": tuning, "autotune_start_ms": start_ms,
+                         "autotune_best_ms": best_ms}
+    log(f"[bench] wfa persistent (autotuned): {res['persistent']}")
+    plan.close()
 
     def launch(layout):
         m._lib.call("cgm_wfa_affine_launch", up["q_packed"].dptr, up["r_packed"].dptr,
@@ -94,25 +113,24 @@ def wfa_extra(m, dev, gen, n_pairs=20_000):
                     w.n_pairs, up["q_packed"].count, up["r_packed"].count, up["max_error"].dptr,
                     0, 4, 6, 2, out.dptr, layout, dev.stream)
 
-    res = {}
-    costs = {}
-    for name, layout, steps in (("wave", 3, 3), ("plain", 1, 1)):
+    for name, layout, steps in (("auto", 0, 3), ("plain", 1, 1)):
         wall, kms = time_steps(m, dev, lambda: launch(layout), steps, 1)
         costs[name] = dev.download(out, w.n_pairs)
         res[name] = {"pairs_per_s": w.n_pairs * steps / wall, "ms_per_step": wall / steps * 1e3}
         log(f"[bench] wfa {name}: {res[name]}")
-    res["identical"] = bool(np.array_equal(costs["wave"], costs["plain"]))
-    res["speedup_wave_vs_plain"] = res["wave"]["pairs_per_s"] / res["plain"]["pairs_per_s"]
-    res["mean_cost"] = float(costs["wave"].mean())
+    res["identical"] = all(bool(np.array_equal(costs[k], costs["plain"]))
+                           for k in ("persistent", "auto"))
+    res["speedup_auto_vs_plain"] = res["auto"]["pairs_per_s"] / res["plain"]["pairs_per_s"]
+    res["mean_cost"] = float(costs["plain"].mean())
     res["pairs"] = w.n_pairs
     return res
 
 
 def mixed_extra(m, dev, gen, n_pairs=100_000):
     """Mixed batch (50% SW 100-300 bp band 32, 40% SW 1-2 kb band 128, 10%
-    WFA 5-20 kb): pairs/s of the persistent task-queue kernel replayed from
-    a hipGraph, of the same enqueued eagerly, and of the one-workgroup-per-
-    pair kernels."""
+    WFA 5-20 kb): pairs/s of the persistent task-queue kernel on an
+    autotuned plan replayed from a hipGraph, of the same enqueued eagerly,
+    and of the one-workgroup-per-pair kernels."""
     w = gen.generate(4, n_pairs)
     n = w.n_pairs
     bufs = tuple(dev.upload(a) for a in (w.q_packed, w.r_packed, w.q_len, w.r_len, w.q_off,
@@ -122,6 +140,11 @@ def mixed_extra(m, dev, gen, n_pairs=100_000):
     out = m.SwResult(n)
     plan = m.MixedPlan(n, int(w.max_error.max()))
     res = {"pairs": n}
+    t0 = time.perf_counter()
+    tuning, best_ms, start_ms = plan.autotune(dev, bufs, n, kind, band, me, cfg, out, reps=1)
+    res["autotune"] = {"tuning": tuning, "start_ms": start_ms, "best_ms": best_ms,
+                       "seconds": time.perf_counter() - t0}
+    log(f"[bench] mixed autotune: {res['autotune']}")
     plan.capture(dev, bufs, n, kind, band, me, cfg, out)
     wall, kms = time_steps(m, dev, lambda: plan.replay(dev), 3, 1)
     graph_res = [dev.download(a, n) for a in (out.scores, out.query_end, out.ref_end)]
@@ -227,6 +250,7 @@ def main():
         f"in {time.perf_counter() - t:.1f}s")
     run = SwRunner(m, dev, w)
     plan = None
+    autotune = None
     if args.sw_path == "buckets" or args.layout != 0:
         step = lambda: run.launch(args.layout if args.sw_path == "persistent" else 2)  # noqa: E731
     else:
@@ -236,6 +260,13 @@ def main():
         bufs = (run.q, run.r, run.ql, run.rl, run.qo, run.ro)
         step = lambda: plan.align(dev, bufs, w.n_pairs, None, run.band, None,  # noqa: E731
                                   run.cfg, run.out)
+        # autotuned layout (untimed): workgroup shape, band split, switch-over
+        t = time.perf_counter()
+        tuning, best_ms, start_ms = plan.autotune(dev, bufs, w.n_pairs, None, run.band, None,
+                                                  run.cfg, run.out, reps=3)
+        autotune = {"tuning": tuning, "start_ms": start_ms, "best_ms": best_ms,
+                    "seconds": time.perf_counter() - t}
+        log(f"[bench] autotune: {autotune}")
 
     wall, kms = time_steps(m, dev, step, args.steps, args.warmup)
     kernel_ms = []
@@ -311,7 +342,7 @@ def main():
         "config": {"workload": "batched banded SW: 200,000 pairs, targets 100-300 bp, "
                                "divergence 1-5%, 0.1% N, band 16-64 per pair, 2/-4/-4/-2",
                    "pairs": w.n_pairs, "valid_cells": total_cells, "band_area_cells": band_area,
-                   "layout": args.layout, "sw_path": args.sw_path},
+                   "layout": args.layout, "sw_path": args.sw_path, "autotune": autotune},
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": VALU_PEAK_TOPS,
                      "unit": "TOP/s (int32)", "frac": achieved_tops / VALU_PEAK_TOPS,
                      "traffic": TRAFFIC_BYTES if plan is not None else None,

@@ -292,6 +292,7 @@ struct MixPlan {
     WfaPen pen{};
     wfadev::WfaGeom gm{};
     int cus = 0;
+    int max_grid = 0;  // workgroups that have a slab (the launch grid's cap)
     MixTuning tune{};
     // workspace
     uint32_t *keys = nullptr, *keys_out = nullptr, *tkeys = nullptr, *tkeys_out = nullptr;
@@ -423,10 +424,13 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
     P->sort_tmp_bytes = b1 > b2 ? b1 : b2;
     CGM_ALLOC(P->sort_tmp, P->sort_tmp_bytes + 256);
     // HBM wavefront slab per workgroup (for the largest grid any tuning
-    // launches) for WFA pairs too large for LDS.
+    // launches, within 4 GiB of slabs) for WFA pairs too large for LDS.
     const int K = me_cap < reach_host(me_cap, pen) ? me_cap : reach_host(me_cap, pen);
     P->slab_words = (int64_t)(P->gm.nM + 2 * P->gm.nI) * (2 * (int64_t)K + 1);
-    CGM_ALLOC(P->slab, (int64_t)P->cus * kMixMaxBlocksPerCu * P->slab_words * 4);
+    P->max_grid = P->cus * kMixMaxBlocksPerCu;
+    while (P->max_grid > 1 && (int64_t)P->max_grid * P->slab_words * 4 > (int64_t(4) << 30))
+        P->max_grid /= 2;
+    CGM_ALLOC(P->slab, (int64_t)P->max_grid * P->slab_words * 4);
 #undef CGM_ALLOC
     if (hipFuncSetAttribute((const void *)mix_persistent_kernel<true>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, kMixMaxLds - 256) !=
@@ -534,8 +538,9 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                                                          stream));
     CGM_HIP(hipEventRecord(P->ev_start, stream));
     const bool sw_only = kind == nullptr && kind_scalar == 0;
+    const int grid = std::min(P->cus * tune.blocks_per_cu, P->max_grid);
     hipLaunchKernelGGL(sw_only ? mix_persistent_kernel<false> : mix_persistent_kernel<true>,
-                       dim3(P->cus * tune.blocks_per_cu), dim3(kMixThreads),
+                       dim3(grid), dim3(kMixThreads),
                        (size_t)tune.lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
                        P->gm, out, P->order, P->tasks, P->task_order, n_tasks, queue,
                        tune.lds_bytes, tune.tile_steps, P->slab, P->slab_words);
@@ -547,6 +552,27 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
         return rc;
     return launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
                                  ranges + 2 * kMixSwPlain, P->cus * 2, stream);
+}
+
+// Automatic WFA layout: the persistent kernel with every pair a WFA pair,
+// on the plan cached per (device, stream), sized for the batch's largest
+// max_error (read back from the device when given per pair).
+int launch_wfa_auto(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
+                    int32_t *out_cost, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    if (int rc = check_wfa_pen(p)) return rc;
+    if (b.n_pairs == 0) return CGM_OK;
+    int32_t me_max = max_error_scalar;
+    if (max_error) {
+        if (int rc = device_max_i32(max_error, b.n_pairs, stream, &me_max)) return rc;
+    }
+    me_max = std::max(0, std::min(me_max, kMaxWfaError));
+    MixPlan *P = nullptr;
+    if (int rc = cached_plan(stream, b.n_pairs, me_max, p, &P)) return rc;
+    const SwScoring sc = {2, -3, -5, -2};  // unused: the batch holds no SW pair
+    const SwOut out = {out_cost, nullptr, nullptr};
+    return mix_enqueue(P, b, nullptr, 1, nullptr, 0, max_error, max_error_scalar, sc, p, out,
+                       stream);
 }
 
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,

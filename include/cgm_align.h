@@ -50,10 +50,13 @@ extern "C" {
 #define CGM_BAD_PAIR ((int32_t)0x80000000)
 
 /* Kernel layouts for the launch entry points. */
-#define CGM_LAYOUT_AUTO 0   /* persistent task-queue kernel, work-sorted */
+#define CGM_LAYOUT_AUTO 0   /* SW: persistent task-queue kernel, work-sorted, one pair per
+                               lane, on a plan the library keeps per (device, stream);
+                               WFA: as CGM_LAYOUT_WAVE */
 #define CGM_LAYOUT_PLAIN 1  /* one workgroup per pair (baseline) */
 #define CGM_LAYOUT_LANE 2   /* SW: one pair per lane, one kernel per band bucket */
-#define CGM_LAYOUT_WAVE 3   /* one 64-lane wavefront per pair */
+#define CGM_LAYOUT_WAVE 3   /* WFA: persistent kernel, one four-wavefront workgroup per pair
+                               (LDS, or tiled over an HBM slab), longest pairs first */
 
 /* ------------------------------------------------------------ errors, info */
 
@@ -113,7 +116,9 @@ int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed,
  * out_cost = minimum global cost (mismatch x, gap of length l = o + l*e,
  * match 0) or -1 when it exceeds max_error.  max_error: per-pair device
  * array or NULL for max_error_scalar.  Requires x >= 1, e >= 1, o >= 0 (the
- * reference GPU kernel asserts e >= 1, align_gpu.mojo:288). */
+ * reference GPU kernel asserts e >= 1, align_gpu.mojo:288).  With a per-pair
+ * max_error array, CGM_LAYOUT_AUTO and CGM_LAYOUT_WAVE read its maximum back
+ * to size their wavefront storage (one host synchronisation). */
 int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                           const int32_t *q_len, const int32_t *r_len,
                           const int32_t *q_off, const int32_t *r_off,
