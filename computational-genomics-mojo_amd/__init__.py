@@ -16,7 +16,8 @@ from .align import (AlignConfig, AlignResult, MixedPlan, SwResult, WFAConfig, al
 from .device import DeviceArray, GenomicsDevice, Timer
 from .serve import GeneratorHandle, ServingEngine
 from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_to_2bit,
-                       get_base_from_word, get_view, pack_codes)
+                       concat_ascii, decode_ascii, encode_ascii_launch, encode_batch_gpu, get_base_from_word,
+                       get_view, pack_codes)
 
 __all__ = [
     "AlignConfig", "AlignResult", "MixedPlan", "align_mixed_batch", "align_mixed_plain",
@@ -25,5 +26,5 @@ __all__ = [
     "sw_banded_batch", "wfa_affine", "wfa_affine_batch", "wfa_affine_cpu", "DeviceArray",
     "GenomicsDevice", "Timer", "GeneratorHandle", "ServingEngine", "BASES_PER_WORD", "Base",
     "SequenceBatch", "SequenceView", "ascii_to_2bit", "get_base_from_word", "get_view",
-    "pack_codes",
+    "pack_codes", "concat_ascii", "decode_ascii", "encode_ascii_launch", "encode_batch_gpu",
 ]

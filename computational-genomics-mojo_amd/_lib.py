@@ -95,6 +95,8 @@ SIGNATURES = {
                            _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32],
     "cgm_wfa_affine_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                             _i32, _i32, _i32, _vp, _i32],
+    "cgm_encode_ascii_launch": [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "cgm_encode_ascii_host": [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _pi64],
     "cgm_gen_create": [_i32, _i64, _u64, _pvp],
     "cgm_gen_sizes": [_vp, _pi64, _pi64, _pi64],
     "cgm_gen_copy": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -161,6 +161,69 @@ class SequenceBatch:
         return words_for_length(int(self.lengths[idx]))
 
 
+def concat_ascii(seqs):
+    """Sequences (str or bytes) back to back as one uint8 buffer, with the
+    int64 byte start and int32 length of each: the input of the device
+    encoder (cgm_encode_ascii_*)."""
+    parts = [q.encode() if isinstance(q, str) else bytes(q) for q in seqs]
+    lens = np.fromiter((len(p) for p in parts), dtype=np.int64, count=len(parts))
+    starts = np.zeros(len(parts), np.int64)
+    if len(parts) > 1:
+        starts[1:] = np.cumsum(lens)[:-1]
+    buf = np.frombuffer(b"".join(parts), dtype=np.uint8).copy()
+    return buf, starts, lens.astype(np.int32)
+
+
+def decode_ascii(packed, nmask, offsets, lengths) -> list:
+    """Sequences of a packed batch back to ASCII: bases as A/C/G/T
+    (base_to_ascii, dna.mojo:43-52), NMask-flagged positions as 'N'."""
+    out = []
+    lut = np.frombuffer(b"ACGT", dtype=np.uint8)
+    for o, n in zip(offsets, lengths):
+        o, n = int(o), int(n)
+        nw = words_for_length(n)
+        w = np.asarray(packed[o:o + nw], np.uint64)
+        m = np.asarray(nmask[o:o + nw], np.uint64) if nmask is not None else np.zeros(nw, np.uint64)
+        i = np.arange(n)
+        codes = (w[i // 32] >> (62 - 2 * (i % 32)).astype(np.uint64)) & np.uint64(3)
+        flags = (m[i // 32] >> (i % 32).astype(np.uint64)) & np.uint64(1)
+        s = lut[codes.astype(np.int64)]
+        s[flags.astype(bool)] = ord("N")
+        out.append(s.tobytes())
+    return out
+
+
+def encode_batch_gpu(seqs) -> SequenceBatch:
+    """SequenceBatch of `seqs` encoded on the GPU (cgm_encode_ascii_host):
+    the batch add_sequence (sequence.mojo:38-66) would build by appending
+    them one by one."""
+    from . import _lib
+    buf, starts, lens = concat_ascii(seqs)
+    n = len(lens)
+    cap = n + int(buf.size) // BASES_PER_WORD + 1
+    packed = np.zeros(cap, np.uint64)
+    masks = np.zeros(cap, np.uint64)
+    offs = np.zeros(n, np.int32)
+    lout = np.zeros(n, np.int32)
+    import ctypes
+    total = ctypes.c_int64(0)
+    _lib.call("cgm_encode_ascii_host", _lib.ptr(buf if buf.size else np.zeros(1, np.uint8)),
+              int(buf.size), _lib.ptr(starts), _lib.ptr(lens), n, _lib.ptr(packed),
+              _lib.ptr(masks), cap, _lib.ptr(offs), _lib.ptr(lout), ctypes.byref(total))
+    nw = int(total.value)
+    return SequenceBatch.from_arrays(packed[:nw], masks[:nw], offs, lout)
+
+
+def encode_ascii_launch(device, ascii, seq_start, seq_len, n_seqs: int, words_cap: int,
+                        packed, nmask, word_off, lengths, total_words=None) -> None:
+    """Device-pointer form (cgm_encode_ascii_launch) on the device's stream;
+    every array is a DeviceArray (nmask / total_words may be None)."""
+    from . import _lib
+    _lib.call("cgm_encode_ascii_launch", _lib.ptr(ascii), int(ascii.count), _lib.ptr(seq_start),
+              _lib.ptr(seq_len), int(n_seqs), _lib.ptr(packed), _lib.ptr(nmask), int(words_cap),
+              _lib.ptr(word_off), _lib.ptr(lengths), _lib.ptr(total_words), device.stream)
+
+
 def get_view(batch: SequenceBatch, idx: int) -> SequenceView:
     """sequence.mojo:123-137 -- copy sequence idx into a SequenceView."""
     off = int(batch.offsets[idx])

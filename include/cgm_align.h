@@ -278,6 +278,33 @@ int cgm_wfa_affine_host(const uint64_t *q_packed, const uint64_t *r_packed,
                         int32_t x, int32_t o, int32_t e,
                         int32_t *out_cost, int32_t layout);
 
+/* ------------------------------------------------------- ingest on the device
+ * Replaces SequenceBatch.add_sequence (genomics/core/sequence.mojo:38-66)
+ * with ascii_to_2bit (genomics/core/dna.mojo:26-40) for a whole batch at
+ * once: sequence i is the seq_len[i] ASCII bytes at byte offset
+ * seq_start[i] of `ascii` (ascii_bytes long).  Writes, in the order the
+ * reference appends them: word_off[i] = running sum of ceil(len / 32)
+ * (int32, as GenomicsDevice narrows offsets, genomics/gpu/device.mojo:
+ * 82-108), lengths[i], and the packed / NMask words (nmask may be NULL).
+ * A/a=0 C/c=1 G/g=2 T/t=3; every other byte is stored as 0 and flagged in
+ * NMask (LSB = first base of the word); bits past a sequence's end are 0.
+ * A sequence whose bytes fall outside the buffer gets lengths[i] = -1 and no
+ * words.  *total_words (device int64, may be NULL) receives the number of
+ * words the batch needs; only words below words_cap are written, so a call
+ * with *total_words > words_cap did not fit (words_cap = n_seqs +
+ * ascii_bytes / 32 always fits sequences that do not overlap).  Uses a
+ * stream-ordered workspace (hipMallocAsync / hipFreeAsync on `stream`). */
+int cgm_encode_ascii_launch(const uint8_t *ascii, int64_t ascii_bytes, const int64_t *seq_start,
+                            const int32_t *seq_len, int64_t n_seqs, uint64_t *packed,
+                            uint64_t *nmask, int64_t words_cap, int32_t *word_off,
+                            int32_t *lengths, int64_t *total_words, void *stream);
+/* Host-pointer form of the above (copies in and out; total_words is a
+ * host pointer here). */
+int cgm_encode_ascii_host(const uint8_t *ascii, int64_t ascii_bytes, const int64_t *seq_start,
+                          const int32_t *seq_len, int64_t n_seqs, uint64_t *packed,
+                          uint64_t *nmask, int64_t words_cap, int32_t *word_off, int32_t *lengths,
+                          int64_t *total_words);
+
 /* ------------------------------------------------------- synthetic batches
  * Seeded workload generator (replaces make_random_batch,
  * benchmarks/harness.mojo:55-67, whose queries equal their references).

@@ -156,3 +156,24 @@ def test_default_tuning_roundtrip_and_validation(cgm):
     finally:
         cgm.set_default_tuning(**t0)
     assert cgm.default_tuning() == t0
+
+
+def test_decode_ascii_inverts_oracle_packing(cgm, orc):
+    """decode_ascii (packed words + NMask -> ACGT/N text) is the inverse of the
+    oracle's packing on upper-case ACGTN input."""
+    rng = np.random.default_rng(2)
+    seqs = ["".join(rng.choice(list("ACGTN"), int(n))) for n in rng.integers(0, 100, 200)]
+    words, masks, offs = [], [], []
+    nw = 0
+    for s in seqs:
+        w, m = orc.pack_ascii(s)
+        offs.append(nw)
+        nw += w.size
+        words.append(w)
+        masks.append(m)
+    packed = np.concatenate(words)
+    nmask = np.concatenate(masks)
+    back = cgm.decode_ascii(packed, nmask, offs, [len(s) for s in seqs])
+    assert [b.decode() for b in back] == seqs
+    buf, starts, lens = cgm.concat_ascii(seqs)
+    assert [buf[a:a + b].tobytes().decode() for a, b in zip(starts, lens)] == seqs
