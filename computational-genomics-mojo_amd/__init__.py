@@ -12,7 +12,8 @@ from .align import (AlignConfig, AlignResult, MixedPlan, SwResult, WFAConfig, al
                     align_mixed_plain, default_config, default_tuning, default_wfa_config,
                     launch_banded_sw, launch_wfa_affine, set_default_tuning,
                     smith_waterman_banded, sw_banded_batch, wfa_affine, wfa_affine_batch,
-                    wfa_affine_cpu)
+                    wfa_affine_cpu, WFAResult, cigar_capacity, runs_to_cigar,
+                    wfa_affine_cigar_batch, wfa_affine_cigar_cpu)
 from .device import DeviceArray, GenomicsDevice, Timer
 from .serve import GeneratorHandle, ServingEngine
 from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_to_2bit,
@@ -26,5 +27,6 @@ __all__ = [
     "sw_banded_batch", "wfa_affine", "wfa_affine_batch", "wfa_affine_cpu", "DeviceArray",
     "GenomicsDevice", "Timer", "GeneratorHandle", "ServingEngine", "BASES_PER_WORD", "Base",
     "SequenceBatch", "SequenceView", "ascii_to_2bit", "get_base_from_word", "get_view",
-    "pack_codes", "concat_ascii", "decode_ascii", "encode_ascii_launch", "encode_batch_gpu",
+    "pack_codes", "concat_ascii", "decode_ascii", "WFAResult", "cigar_capacity",
+    "runs_to_cigar", "wfa_affine_cigar_batch", "wfa_affine_cigar_cpu", "encode_ascii_launch", "encode_batch_gpu",
 ]

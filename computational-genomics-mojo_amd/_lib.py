@@ -95,6 +95,10 @@ SIGNATURES = {
                            _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32],
     "cgm_wfa_affine_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                             _i32, _i32, _i32, _vp, _i32],
+    "cgm_wfa_cigar_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _i32,
+                             _i32, _i32, _vp, _vp, _vp, _vp, _vp],
+    "cgm_wfa_cigar_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _i32, _i32,
+                           _i32, _vp, _vp, _vp, _vp],
     "cgm_encode_ascii_launch": [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "cgm_encode_ascii_host": [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _pi64],
     "cgm_gen_create": [_i32, _i64, _u64, _pvp],

@@ -109,6 +109,63 @@ def wfa_affine_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: WFACon
     return cost
 
 
+CIGAR_OPS = "=XID"
+
+
+@dataclass
+class WFAResult:
+    """align_cpu.mojo:40-43: the cost (or -1 above max_error) and the
+    run-length extended CIGAR ('=', 'X', 'I', 'D')."""
+    score: int
+    cigar: str
+
+
+def runs_to_cigar(runs) -> str:
+    """Run-length operations (count << 2 | op; op 0 '=', 1 'X', 2 'I', 3 'D')
+    as a CIGAR string, e.g. [12, 5, 4] -> '3=1X1='."""
+    return "".join(f"{int(v) >> 2}{CIGAR_OPS[int(v) & 3]}" for v in runs)
+
+
+def cigar_capacity(max_error, cfg: WFAConfig) -> np.ndarray:
+    """Runs a CIGAR of cost <= max_error can have: every run other than '='
+    costs at least min(x, o + e), and '=' runs alternate with them."""
+    me = np.maximum(np.asarray(max_error, np.int64), 0)
+    return 2 * (me // max(1, min(cfg.x, cfg.o + cfg.e))) + 1
+
+
+def wfa_affine_cigar_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: WFAConfig,
+                           max_error=None):
+    """Cost and CIGAR of every pair (wfa_affine_cigar_cpu, align_cpu.mojo:
+    380-623, on the GPU); host arrays in, (costs, list of CIGAR strings) out."""
+    q_packed = np.ascontiguousarray(q_packed, np.uint64)
+    r_packed = np.ascontiguousarray(r_packed, np.uint64)
+    ql, rl, qo, ro, me = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(max_error)
+    n = int(ql.size)
+    cost = np.zeros(n, np.int32)
+    if not n:
+        return cost, []
+    cap = cigar_capacity(me if me is not None else np.full(n, cfg.max_error), cfg)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(cap)
+    runs = np.zeros(max(int(off[-1]), 1), np.uint32)
+    nr = np.zeros(n, np.int32)
+    _lib.call("cgm_wfa_cigar_host", _lib.ptr(_nonempty(q_packed)), _lib.ptr(_nonempty(r_packed)),
+              _lib.ptr(ql), _lib.ptr(rl), _lib.ptr(qo), _lib.ptr(ro), n, q_packed.size,
+              r_packed.size, _lib.ptr(me), cfg.max_error, cfg.x, cfg.o, cfg.e, _lib.ptr(cost),
+              _lib.ptr(runs), _lib.ptr(off), _lib.ptr(nr))
+    if np.any(nr < 0):
+        raise _lib.CgmError(_lib.CGM_ERR_INVALID, "CIGAR capacity exceeded")
+    return cost, [runs_to_cigar(runs[off[i]:off[i] + nr[i]]) for i in range(n)]
+
+
+def wfa_affine_cigar_cpu(query: SequenceView, ref_seq: SequenceView, cfg: WFAConfig) -> WFAResult:
+    """align_cpu.mojo:380-623 (same name and signature), computed on the GPU."""
+    qp, qo, ql = _view_arrays(query)
+    rp, ro, rl = _view_arrays(ref_seq)
+    c, cig = wfa_affine_cigar_batch(qp, qo, ql, rp, ro, rl, cfg)
+    return WFAResult(int(c[0]), cig[0])
+
+
 def _view_arrays(v: SequenceView):
     return np.ascontiguousarray(v.packed, np.uint64), np.zeros(1, np.int32), \
         np.array([v.length], np.int32)

@@ -1,0 +1,443 @@
+// cgm_cigar.hip -- gap-affine WFA with traceback (CIGAR) for pair batches.
+//
+// Semantics of wfa_affine_cigar_cpu (genomics/cpu/align_cpu.mojo:380-623):
+// the optimal global cost (as wfa_affine_cpu) and the CIGAR its backtrace
+// picks, as run-length operations run = (count << 2) | op with op 0 '=',
+// 1 'X', 2 'I' (query base), 3 'D' (reference base), first operation first.
+//
+// Pipeline:
+//  1. costs: the automatic WFA layout (persistent kernel, cached plan);
+//  2. per pair, the size of its full wavefront history: one row of
+//     2K + 1 diagonals (K = the reach of its cost) per score step, for M
+//     (after extension), I and D, int16 when every row offset fits;
+//  3. pairs in chunks whose histories fit the workspace budget; per chunk,
+//     one 256-thread workgroup per pair recomputes the forward pass (the
+//     same exact reductions as the cost kernels: only multiples of
+//     gcd(x, o, e), only diagonals |k| <= (s - o) / e) writing every row to
+//     its HBM history, then thread 0 walks the reference's backtrace.
+// A diagonal or score outside the stored rows holds no reachable state, so
+// reading it as invalid gives the reference's NEG_INF there.  M before
+// extension (the reference's M_pre) is recomputed from the stored rows with
+// the forward formula.
+
+#include <algorithm>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "cgm_wfa_dev.h"
+
+namespace cgm {
+
+namespace {
+
+constexpr int kCigarThreads = 256;
+constexpr int kCigarSeqLds = 48 * 1024;  // sequences staged to LDS up to this
+constexpr int64_t kCigarBudget = int64_t(8) << 30;
+
+struct CigarGeom {
+    int64_t K, W, rows, esize;  // K = 0 and rows = 0: nothing to store
+};
+
+__host__ __device__ __forceinline__ int64_t reach64(int64_t s, const WfaPen &p) {
+    return s >= p.o + p.e ? (s - p.o) / p.e : 0;
+}
+
+__host__ __device__ __forceinline__ CigarGeom cigar_geom(int qlen, int rlen, int me, int cost,
+                                                         const WfaPen &p, int g) {
+    CigarGeom c{0, 0, 0, 0};
+    if (cost < 0 || (qlen == 0 && rlen == 0)) return c;
+    c.K = min(min((int64_t)me, (int64_t)max(qlen, rlen)), reach64(cost, p));
+    c.W = 2 * c.K + 1;
+    c.rows = cost / g + 1;
+    // rows grow past qlen only by +1 steps (mismatch x, insertion e)
+    c.esize = ((int64_t)qlen + cost / min(p.x, p.e) + 2 <= 32767) ? 2 : 4;
+    return c;
+}
+
+__global__ void cigar_size_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p,
+                                  int g, const int32_t *cost, int64_t *bytes) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= b.n_pairs) return;
+    const uint64_t *q, *r;
+    int qlen = 0, rlen = 0;
+    int64_t v = 0;
+    if (load_pair(b, i, q, r, qlen, rlen)) {
+        const CigarGeom c = cigar_geom(qlen, rlen, me_arr ? me_arr[i] : me_scalar, cost[i], p, g);
+        v = 3 * c.rows * c.W * c.esize;
+        v = (v + 255) & ~(int64_t)255;
+    }
+    bytes[i] = v;
+}
+
+template <typename T>
+__device__ __forceinline__ int hist_get(const T *row, int64_t K, int64_t k) {
+    if (k < -K || k > K) return -1;
+    return (int)row[k + K];
+}
+
+// Backtrace of align_cpu.mojo:510-600 over the stored rows; writes the runs
+// (reversed, then flipped) and returns their number, or -2 past `cap`.
+template <typename T>
+__device__ int cigar_backtrace(const T *M, const T *I, const T *D, int64_t K, int64_t W,
+                               int qlen, int rlen, int cost, const WfaPen &p, int g,
+                               uint32_t *out, int64_t cap) {
+    const int dx = p.x / g, dO = (p.o + p.e) / g, de = p.e / g;
+    auto row_of = [&](const T *A, int si) { return A + (int64_t)si * W; };
+    // M before extension at (si, k): the forward formula over stored rows
+    auto m_pre = [&](int si, int64_t k) -> int {
+        if (si == 0) return k == 0 ? 0 : -1;
+        int v = -1;
+        if (si >= dx) {
+            const int a = hist_get(row_of(M, si - dx), K, k);
+            if (a >= 0) v = max(v, a + 1);
+        }
+        const int iv = hist_get(row_of(I, si), K, k);
+        if (iv >= 0) v = max(v, iv);
+        const int dv = hist_get(row_of(D, si), K, k);
+        if (dv >= 0) v = max(v, dv);
+        return v;
+    };
+    int64_t nr = 0;
+    int run_op = -1;
+    int64_t run_cnt = 0;
+    auto emit = [&](int op, int64_t cnt) {
+        if (cnt <= 0) return;
+        if (op == run_op) {
+            run_cnt += cnt;
+            return;
+        }
+        if (run_op >= 0) {
+            if (nr < cap) out[nr] = (uint32_t)((run_cnt << 2) | run_op);
+            ++nr;
+        }
+        run_op = op;
+        run_cnt = cnt;
+    };
+    int si = cost / g;
+    int64_t cur_d = qlen - rlen;
+    int cur_matrix = 0;
+    int64_t cur_row = hist_get(row_of(M, si), K, cur_d);
+    for (;;) {
+        if (cur_matrix == 0) {
+            const int pre_r = m_pre(si, cur_d);
+            emit(0, cur_row - pre_r);
+            cur_row = pre_r;
+            if (si == 0 && cur_d == 0) break;
+            if (si >= dx) {
+                const int prev_m = hist_get(row_of(M, si - dx), K, cur_d);
+                if (prev_m >= 0 && prev_m + 1 == cur_row) {
+                    emit(1, 1);
+                    si -= dx;
+                    cur_row = prev_m;
+                    continue;
+                }
+            }
+            const int i_here = hist_get(row_of(I, si), K, cur_d);
+            if (i_here >= 0 && i_here == cur_row) {
+                cur_matrix = 1;
+                continue;
+            }
+            const int d_here = hist_get(row_of(D, si), K, cur_d);
+            if (d_here >= 0 && d_here == cur_row) {
+                cur_matrix = 2;
+                continue;
+            }
+            break;
+        } else if (cur_matrix == 1) {
+            emit(2, 1);
+            const int64_t prev_d = cur_d - 1;
+            if (si >= dO) {
+                const int mv = hist_get(row_of(M, si - dO), K, prev_d);
+                if (mv >= 0 && mv + 1 == cur_row) {
+                    si -= dO;
+                    cur_d = prev_d;
+                    cur_row = mv;
+                    cur_matrix = 0;
+                    continue;
+                }
+            }
+            if (si >= de) {
+                const int iv = hist_get(row_of(I, si - de), K, prev_d);
+                if (iv >= 0 && iv + 1 == cur_row) {
+                    si -= de;
+                    cur_d = prev_d;
+                    cur_row = iv;
+                    continue;
+                }
+            }
+            break;
+        } else {
+            emit(3, 1);
+            const int64_t prev_d = cur_d + 1;
+            if (si >= dO) {
+                const int mv = hist_get(row_of(M, si - dO), K, prev_d);
+                if (mv >= 0 && mv == cur_row) {
+                    si -= dO;
+                    cur_d = prev_d;
+                    cur_row = mv;
+                    cur_matrix = 0;
+                    continue;
+                }
+            }
+            if (si >= de) {
+                const int dv = hist_get(row_of(D, si - de), K, prev_d);
+                if (dv >= 0 && dv == cur_row) {
+                    si -= de;
+                    cur_d = prev_d;
+                    cur_row = dv;
+                    continue;
+                }
+            }
+            break;
+        }
+    }
+    if (run_op >= 0) {
+        if (nr < cap) out[nr] = (uint32_t)((run_cnt << 2) | run_op);
+        ++nr;
+    }
+    if (nr > cap) return -2;
+    for (int64_t a = 0, z = nr - 1; a < z; ++a, --z) {
+        const uint32_t t = out[a];
+        out[a] = out[z];
+        out[z] = t;
+    }
+    return (int)nr;
+}
+
+// Forward pass with full history, one workgroup per pair; rows of score
+// step si hold diagonals -K..K (entries outside the step's range invalid).
+template <typename T>
+__device__ void cigar_forward(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
+                              int cost, const WfaPen &p, int g, T *M, T *I, T *D, int64_t K,
+                              int64_t W) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const T neg = wfadev::RowNeg<T>::v;
+    const int dx = p.x / g, dO = (p.o + p.e) / g, de = p.e / g;
+    for (int64_t i = tid; i < W; i += nt) {
+        M[i] = neg;
+        I[i] = neg;
+        D[i] = neg;
+    }
+    __syncthreads();
+    if (tid == 0) M[K] = (T)wfadev::extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+    __syncthreads();
+    const int nrows = cost / g + 1;
+    for (int si = 1; si < nrows; ++si) {
+        const int64_t s = (int64_t)si * g;
+        const int64_t Rs = reach64(s, p);
+        const int64_t lo = max(max((int64_t)-rlen, (int64_t)-me), -Rs);
+        const int64_t hi = min(min((int64_t)qlen, (int64_t)me), Rs);
+        T *Mr = M + si * W, *Ir = I + si * W, *Dr = D + si * W;
+        const T *Mx = si >= dx ? M + (si - dx) * W : nullptr;
+        const T *Mo = si >= dO ? M + (si - dO) * W : nullptr;
+        const T *Ie = si >= de ? I + (si - de) * W : nullptr;
+        const T *De = si >= de ? D + (si - de) * W : nullptr;
+        for (int64_t k = -K + tid; k <= K; k += nt) {
+            const int64_t di = k + K;
+            if (k < lo || k > hi) {
+                Mr[di] = neg;
+                Ir[di] = neg;
+                Dr[di] = neg;
+                continue;
+            }
+            int a = -1, c = -1, bb = -1, d = -1, m = -1;
+            if (Mo) {
+                if (di > 0) a = (int)Mo[di - 1];
+                if (di < W - 1) bb = (int)Mo[di + 1];
+            }
+            if (Ie) {
+                if (di > 0) c = (int)Ie[di - 1];
+                if (di < W - 1) d = (int)De[di + 1];
+            }
+            if (Mx) m = (int)Mx[di];
+            const int iv = max(a < 0 ? -1 : a + 1, c < 0 ? -1 : c + 1);
+            const int dv = max(bb, d);
+            int mv = max(max(m < 0 ? -1 : m + 1, iv), dv);
+            Ir[di] = iv < 0 ? neg : (T)iv;
+            Dr[di] = dv < 0 ? neg : (T)dv;
+            if (mv >= 0) mv = wfadev::extend_diag(q, qlen, qw, r, rlen, rw, mv, (int)k);
+            Mr[di] = mv < 0 ? neg : (T)mv;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
+    Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p, int g, const int32_t *cost_arr,
+    const int64_t *hist_off, uint8_t *ws, int64_t p0, uint32_t *cigar, const int64_t *cigar_off,
+    int32_t *cigar_len) {
+    __shared__ uint64_t seq[kCigarSeqLds / 8];
+    const int64_t pair = p0 + blockIdx.x;
+    const int cost = cost_arr[pair];
+    const uint64_t *qg, *rg;
+    int qlen = 0, rlen = 0;
+    if (cost < 0 || !load_pair(b, pair, qg, rg, qlen, rlen) || (qlen == 0 && rlen == 0)) {
+        if (threadIdx.x == 0) cigar_len[pair] = 0;
+        return;
+    }
+    const int me = me_arr ? me_arr[pair] : me_scalar;
+    const CigarGeom cg = cigar_geom(qlen, rlen, me, cost, p, g);
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const uint64_t *q = qg, *r = rg;
+    if (8 * (qw + rw) <= kCigarSeqLds) {
+        for (int i = threadIdx.x; i < qw + rw; i += blockDim.x) seq[i] = i < qw ? qg[i] : rg[i - qw];
+        q = seq;
+        r = seq + qw;
+        __syncthreads();
+    }
+    uint8_t *h = ws + hist_off[pair];
+    const int64_t plane = cg.rows * cg.W;
+    uint32_t *out = cigar + cigar_off[pair];
+    const int64_t cap = cigar_off[pair + 1] - cigar_off[pair];
+    if (cg.esize == 2) {
+        int16_t *M = (int16_t *)h;
+        cigar_forward<int16_t>(q, qlen, r, rlen, me, cost, p, g, M, M + plane, M + 2 * plane, cg.K,
+                               cg.W);
+        if (threadIdx.x == 0)
+            cigar_len[pair] = cigar_backtrace<int16_t>(M, M + plane, M + 2 * plane, cg.K, cg.W,
+                                                       qlen, rlen, cost, p, g, out, cap);
+    } else {
+        int32_t *M = (int32_t *)h;
+        cigar_forward<int32_t>(q, qlen, r, rlen, me, cost, p, g, M, M + plane, M + 2 * plane, cg.K,
+                               cg.W);
+        if (threadIdx.x == 0)
+            cigar_len[pair] = cigar_backtrace<int32_t>(M, M + plane, M + 2 * plane, cg.K, cg.W,
+                                                       qlen, rlen, cost, p, g, out, cap);
+    }
+}
+
+}  // namespace
+
+}  // namespace cgm
+
+using namespace cgm;
+
+extern "C" {
+
+int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
+                         const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                         const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                         const int32_t *max_error, int32_t max_error_scalar, int32_t x, int32_t o,
+                         int32_t e, int32_t *out_cost, uint32_t *cigar, const int64_t *cigar_off,
+                         int32_t *cigar_len, void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    const Batch b = {q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words, r_words};
+    const WfaPen p = {x, o, e};
+    if (int rc = check_batch(b)) return rc;
+    if (int rc = check_wfa_pen(p)) return rc;
+    if (n_pairs == 0) return CGM_OK;
+    if (!out_cost || !cigar || !cigar_off || !cigar_len)
+        return set_error(CGM_ERR_INVALID, "cigar: null output buffer");
+    if (!max_error && (max_error_scalar < 0 || max_error_scalar > kMaxWfaError))
+        return set_error(CGM_ERR_INVALID, "cigar: max_error out of range");
+    int g = 0, nM = 0, nI = 0;
+    if (int rc = wfa_geometry(p, &g, &nM, &nI)) return rc;
+    // 1. costs
+    if (int rc = launch_wfa_auto(b, max_error, max_error_scalar, p, out_cost, stream)) return rc;
+    // 2. history sizes -> host, chunks within the budget
+    int64_t *d_bytes = nullptr, *d_off = nullptr;
+    CGM_HIP(hipMallocAsync((void **)&d_bytes, (size_t)n_pairs * 8, stream));
+    hipLaunchKernelGGL(cigar_size_kernel, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0,
+                       stream, b, max_error, max_error_scalar, p, g, out_cost, d_bytes);
+    std::vector<int64_t> bytes((size_t)n_pairs), off((size_t)n_pairs);
+    hipError_t err = hipMemcpyAsync(bytes.data(), d_bytes, (size_t)n_pairs * 8,
+                                    hipMemcpyDeviceToHost, stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(stream);
+    (void)hipFreeAsync(d_bytes, stream);
+    if (err != hipSuccess) return hip_error(err, "cigar: history sizes");
+    // chunks [p0, p1) of consecutive pairs; a pair larger than the budget
+    // forms a chunk of its own
+    std::vector<std::pair<int64_t, int64_t>> chunks;
+    int64_t ws_need = 0;
+    for (int64_t i = 0; i < n_pairs;) {
+        int64_t j = i, acc = 0;
+        while (j < n_pairs && (j == i || acc + bytes[j] <= kCigarBudget)) {
+            off[j] = acc;
+            acc += bytes[j];
+            ++j;
+        }
+        chunks.push_back({i, j});
+        ws_need = std::max(ws_need, acc);
+        i = j;
+    }
+    CGM_HIP(hipMallocAsync((void **)&d_off, (size_t)n_pairs * 8, stream));
+    uint8_t *ws = nullptr;
+    err = hipMallocAsync((void **)&ws, (size_t)std::max<int64_t>(ws_need, 256), stream);
+    if (err != hipSuccess) {
+        (void)hipFreeAsync(d_off, stream);
+        return hip_error(err, "cigar: history workspace");
+    }
+    int rc = CGM_OK;
+    err = hipMemcpyAsync(d_off, off.data(), (size_t)n_pairs * 8, hipMemcpyHostToDevice, stream);
+    if (err != hipSuccess) rc = hip_error(err, "cigar: offsets");
+    // 3. forward with history + backtrace, chunk by chunk (stream order
+    // keeps a chunk's workspace intact until its kernel is done)
+    for (size_t c = 0; !rc && c < chunks.size(); ++c) {
+        const int64_t p0 = chunks[c].first, np = chunks[c].second - p0;
+        hipLaunchKernelGGL(wfa_cigar_kernel, dim3((unsigned)np), dim3(kCigarThreads), 0, stream, b,
+                           max_error, max_error_scalar, p, g, out_cost, d_off, ws, p0, cigar,
+                           cigar_off, cigar_len);
+        err = hipGetLastError();
+        if (err != hipSuccess) rc = hip_error(err, "cigar: kernel launch");
+    }
+    // the host copy of `off` must outlive the asynchronous upload
+    err = hipStreamSynchronize(stream);
+    if (!rc && err != hipSuccess) rc = hip_error(err, "cigar: synchronize");
+    (void)hipFreeAsync(ws, stream);
+    (void)hipFreeAsync(d_off, stream);
+    return rc;
+}
+
+int cgm_wfa_cigar_host(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_len,
+                       const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                       int64_t n_pairs, int64_t q_words, int64_t r_words,
+                       const int32_t *max_error, int32_t max_error_scalar, int32_t x, int32_t o,
+                       int32_t e, int32_t *out_cost, uint32_t *cigar, const int64_t *cigar_off,
+                       int32_t *cigar_len) {
+    if (n_pairs < 0) return set_error(CGM_ERR_INVALID, "n_pairs < 0");
+    if (n_pairs == 0) return CGM_OK;
+    if (!cigar_off) return set_error(CGM_ERR_INVALID, "cigar: null cigar_off");
+    const int64_t runs = cigar_off[n_pairs];
+    if (runs < 0) return set_error(CGM_ERR_INVALID, "cigar: negative capacity");
+    hipStream_t s = nullptr;
+    CGM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct Guard {
+        hipStream_t s;
+        std::vector<void *> bufs;
+        ~Guard() {
+            (void)hipStreamSynchronize(s);
+            for (void *p : bufs) (void)hipFree(p);
+            (void)hipStreamDestroy(s);
+        }
+    } guard{s, {}};
+    auto up = [&](const void *src, size_t bytes, void **d) -> int {
+        CGM_HIP(hipMalloc(d, bytes > 0 ? bytes : 8));
+        guard.bufs.push_back(*d);
+        if (src && bytes) CGM_HIP(hipMemcpyAsync(*d, src, bytes, hipMemcpyHostToDevice, s));
+        return CGM_OK;
+    };
+    void *qp, *rp, *ql, *rl, *qo, *ro, *me = nullptr, *oc, *cg, *co, *cl;
+    int rc = CGM_OK;
+    const size_t n4 = (size_t)n_pairs * 4;
+    if ((rc = up(q_packed, (size_t)q_words * 8, &qp)) || (rc = up(r_packed, (size_t)r_words * 8, &rp)) ||
+        (rc = up(q_len, n4, &ql)) || (rc = up(r_len, n4, &rl)) || (rc = up(q_off, n4, &qo)) ||
+        (rc = up(r_off, n4, &ro)) || (max_error && (rc = up(max_error, n4, &me))) ||
+        (rc = up(nullptr, n4, &oc)) || (rc = up(nullptr, (size_t)runs * 4, &cg)) ||
+        (rc = up(cigar_off, (size_t)(n_pairs + 1) * 8, &co)) || (rc = up(nullptr, n4, &cl)))
+        return rc;
+    if ((rc = cgm_wfa_cigar_launch((const uint64_t *)qp, (const uint64_t *)rp, (const int32_t *)ql,
+                                   (const int32_t *)rl, (const int32_t *)qo, (const int32_t *)ro,
+                                   n_pairs, q_words, r_words, (const int32_t *)me,
+                                   max_error_scalar, x, o, e, (int32_t *)oc, (uint32_t *)cg,
+                                   (const int64_t *)co, (int32_t *)cl, s)))
+        return rc;
+    CGM_HIP(hipMemcpyAsync(out_cost, oc, n4, hipMemcpyDeviceToHost, s));
+    CGM_HIP(hipMemcpyAsync(cigar_len, cl, n4, hipMemcpyDeviceToHost, s));
+    if (runs > 0) CGM_HIP(hipMemcpyAsync(cigar, cg, (size_t)runs * 4, hipMemcpyDeviceToHost, s));
+    CGM_HIP(hipStreamSynchronize(s));
+    return CGM_OK;
+}
+
+}  // extern "C"

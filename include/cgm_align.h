@@ -127,6 +127,34 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                           int32_t x, int32_t o, int32_t e,
                           int32_t *out_cost, int32_t layout, void *stream);
 
+/* ------------------------------------------- gap-affine WFA with traceback
+ * Replaces wfa_affine_cigar_cpu (genomics/cpu/align_cpu.mojo:380-623) for a
+ * batch: out_cost as cgm_wfa_affine_launch, plus the CIGAR that function's
+ * backtrace builds (in M: the match run, then mismatch before insertion
+ * before deletion; in I and D: gap opening before extension), as
+ * run-length operations run = (count << 2) | op, op 0 '=', 1 'X', 2 'I'
+ * (query base), 3 'D' (reference base), first operation first, written to
+ * cigar[cigar_off[i] .. cigar_off[i + 1]) (cigar_off: n_pairs + 1 int64 on
+ * the device).  cigar_len[i] = number of runs; 0 for cost -1, an invalid
+ * pair or two empty sequences; -2 when the pair's capacity was too small.
+ * A CIGAR of cost c has at most 2 c / min(x, o + e) + 1 runs.  Computes
+ * the costs first, then re-runs each pair with its whole wavefront history
+ * in HBM (chunks of at most 8 GiB) and walks it back; allocates its
+ * workspace on `stream` and synchronises with the host before returning. */
+int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
+                         const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                         const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                         const int32_t *max_error, int32_t max_error_scalar, int32_t x, int32_t o,
+                         int32_t e, int32_t *out_cost, uint32_t *cigar, const int64_t *cigar_off,
+                         int32_t *cigar_len, void *stream);
+/* Host-pointer form (cigar_off is a host array here). */
+int cgm_wfa_cigar_host(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_len,
+                       const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                       int64_t n_pairs, int64_t q_words, int64_t r_words,
+                       const int32_t *max_error, int32_t max_error_scalar, int32_t x, int32_t o,
+                       int32_t e, int32_t *out_cost, uint32_t *cigar, const int64_t *cigar_off,
+                       int32_t *cigar_len);
+
 /* ------------------------------------------------ mixed SW + WFA batches
  * The batched persistent form of the two launchers above for a batch that
  * mixes SW pairs (kind[i] == 0) and WFA pairs (kind[i] == 1): one call

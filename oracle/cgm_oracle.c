@@ -10,6 +10,7 @@
  *   or_pack_ascii        genomics/core/sequence.mojo:38-66, genomics/core/dna.mojo:26-40
  *   or_sw_banded         genomics/cpu/align_cpu.mojo:229-309  (smith_waterman_banded)
  *   or_wfa_affine        genomics/cpu/align_cpu.mojo:75-226   (wfa_affine_cpu)
+ *   or_wfa_affine_cigar  genomics/cpu/align_cpu.mojo:380-623  (wfa_affine_cigar_cpu)
  * Independent cross-checks (not restatements; used to pin the above):
  *   or_sw_antidiag       the same SW recurrence swept by anti-diagonals with
  *                        an explicit (H desc, qi asc, ri asc) reduction
@@ -22,6 +23,13 @@
  *   or_wfa_affine: reference known answers test_alignment.mojo:87-138, closed
  *     forms, equality with or_gotoh_global (itself pinned by exhaustive
  *     alignment enumeration on tiny inputs) on random and exhaustive inputs.
+ *   or_wfa_affine_cigar: reference known answers test_alignment.mojo:201-279
+ *     (exact CIGAR strings), its cost equal to or_wfa_affine, and per CIGAR:
+ *     cost recomputed from the operations, query / reference bases consumed,
+ *     '=' on equal and 'X' on different bases.  Which of several co-optimal
+ *     CIGARs it returns (mismatch before insertion before deletion in M,
+ *     gap opening before extension in I and D) follows align_cpu.mojo:
+ *     520-600 line by line and is pinned by the known answers only.
  *   or_sw_banded: reference known answers test_alignment.mojo:24-58, closed
  *     forms (identical pair, no common base, empty input, band 0, one
  *     substitution), invariants (band monotonicity, single-diagonal Kadane
@@ -254,6 +262,226 @@ int64_t or_wfa_affine(const uint64_t *q, int64_t qlen, const uint64_t *r, int64_
 done:
     free(M); free(I); free(D);
     return result; /* :226 returns -1 when max_err is exceeded */
+}
+
+/* ---------------------------------------------- gap-affine WFA with CIGAR */
+
+/* align_cpu.mojo:380-623.  Full wavefront history (scores 0..max_err, every
+ * diagonal in +-max_err), then the reference's backtrace.  The CIGAR is
+ * written as run-length operations run = (count << 2) | op with op 0 '=',
+ * 1 'X', 2 'I' (query base), 3 'D' (reference base), first operation first;
+ * *n_runs receives the number of runs, or -2 if more than cap were needed.
+ * Returns the cost, or -1 (no CIGAR, *n_runs = 0) above max_err. */
+int64_t or_wfa_affine_cigar(const uint64_t *q, int64_t qlen, const uint64_t *r, int64_t rlen,
+                            int64_t x, int64_t o, int64_t e, int64_t max_err, uint32_t *runs,
+                            int64_t cap, int64_t *n_runs) {
+    *n_runs = 0;
+    if (qlen == 0 && rlen == 0) return 0; /* :396-397, empty CIGAR */
+    const int64_t n_diags = 2 * max_err + 1, offset = max_err; /* :404-405 */
+    const int64_t total = (max_err + 1) * n_diags;           /* :410 */
+    int32_t *M_pre = (int32_t *)malloc((size_t)total * 4);
+    int32_t *M_full = (int32_t *)malloc((size_t)total * 4);
+    int32_t *I_arr = (int32_t *)malloc((size_t)total * 4);
+    int32_t *D_arr = (int32_t *)malloc((size_t)total * 4);
+    const int32_t NEG = (int32_t)OR_NEG_INF;
+    for (int64_t i = 0; i < total; ++i) M_pre[i] = M_full[i] = I_arr[i] = D_arr[i] = NEG;
+    M_pre[offset] = 0; /* :423-428 */
+    int64_t r0 = 0;
+    while (r0 < qlen && r0 < rlen && or_get_base(q, r0) == or_get_base(r, r0)) ++r0;
+    M_full[offset] = (int32_t)r0;
+    const int64_t target_d = qlen - rlen; /* :430 */
+    int64_t opt_score = -1;
+    if (target_d >= -max_err && target_d <= max_err)
+        if (M_full[offset + target_d] >= qlen) opt_score = 0;
+    if (opt_score == -1) { /* :437-505 */
+        for (int64_t s = 1; s <= max_err; ++s) {
+            int64_t lo = or_max(-rlen, -s), hi = or_min(qlen, s);
+            if (target_d < lo) lo = or_min(lo, target_d);
+            if (target_d > hi) hi = or_max(hi, target_d);
+            lo = or_max(lo, -max_err);
+            hi = or_min(hi, max_err);
+            for (int64_t d = lo; d <= hi; ++d) {
+                const int64_t di = d + offset;
+                int64_t i_val = OR_NEG_INF;
+                if (di > 0) {
+                    if (s >= o + e) {
+                        int64_t mv = M_full[(s - o - e) * n_diags + di - 1];
+                        if (mv != OR_NEG_INF) i_val = or_max(i_val, mv + 1);
+                    }
+                    if (s >= e) {
+                        int64_t iv = I_arr[(s - e) * n_diags + di - 1];
+                        if (iv != OR_NEG_INF) i_val = or_max(i_val, iv + 1);
+                    }
+                }
+                I_arr[s * n_diags + di] = (int32_t)i_val;
+                int64_t d_val = OR_NEG_INF;
+                if (di < n_diags - 1) {
+                    if (s >= o + e) {
+                        int64_t mv = M_full[(s - o - e) * n_diags + di + 1];
+                        if (mv != OR_NEG_INF) d_val = or_max(d_val, mv);
+                    }
+                    if (s >= e) {
+                        int64_t dv = D_arr[(s - e) * n_diags + di + 1];
+                        if (dv != OR_NEG_INF) d_val = or_max(d_val, dv);
+                    }
+                }
+                D_arr[s * n_diags + di] = (int32_t)d_val;
+                int64_t m_val = OR_NEG_INF;
+                if (s >= x) {
+                    int64_t mv = M_full[(s - x) * n_diags + di];
+                    if (mv != OR_NEG_INF) m_val = or_max(m_val, mv + 1);
+                }
+                if (i_val != OR_NEG_INF) m_val = or_max(m_val, i_val);
+                if (d_val != OR_NEG_INF) m_val = or_max(m_val, d_val);
+                M_pre[s * n_diags + di] = (int32_t)m_val;
+                if (m_val != OR_NEG_INF) { /* :491-499 extend */
+                    int64_t row = m_val, col = row - d;
+                    while (row < qlen && col >= 0 && col < rlen) {
+                        if (or_get_base(q, row) != or_get_base(r, col)) break;
+                        ++row; ++col;
+                    }
+                    M_full[s * n_diags + di] = (int32_t)row;
+                }
+            }
+            if (target_d >= lo && target_d <= hi) /* :502-505 */
+                if (M_full[s * n_diags + offset + target_d] >= qlen) { opt_score = s; break; }
+        }
+    }
+    if (opt_score == -1) goto out; /* :507-508 */
+    { /* :510-600 backtrack, operations collected last first */
+        int64_t ops_cap = qlen + rlen + 2 * (opt_score + 1) + 16, n_ops = 0;
+        uint8_t *ops = (uint8_t *)malloc((size_t)ops_cap);
+        int64_t cur_s = opt_score, cur_d = target_d, cur_row = M_full[opt_score * n_diags + target_d + offset];
+        int cur_matrix = 0;
+#define OR_PUSH(op) do { if (n_ops < ops_cap) ops[n_ops] = (uint8_t)(op); ++n_ops; } while (0)
+        for (;;) {
+            const int64_t di = cur_d + offset;
+            if (cur_matrix == 0) { /* M */
+                const int64_t pre_r = M_pre[cur_s * n_diags + di];
+                for (int64_t k = 0; k < cur_row - pre_r; ++k) OR_PUSH(0);
+                cur_row = pre_r;
+                if (cur_s == 0 && cur_d == 0) break;
+                if (cur_s >= x) {
+                    const int64_t prev_m = M_full[(cur_s - x) * n_diags + di];
+                    if (prev_m != OR_NEG_INF && prev_m + 1 == cur_row) {
+                        OR_PUSH(1);
+                        cur_s -= x;
+                        cur_row = prev_m;
+                        continue;
+                    }
+                }
+                const int64_t i_here = I_arr[cur_s * n_diags + di];
+                if (i_here != OR_NEG_INF && i_here == cur_row) { cur_matrix = 1; continue; }
+                const int64_t d_here = D_arr[cur_s * n_diags + di];
+                if (d_here != OR_NEG_INF && d_here == cur_row) { cur_matrix = 2; continue; }
+                break;
+            } else if (cur_matrix == 1) { /* I */
+                OR_PUSH(2);
+                const int64_t prev_d = cur_d - 1, prev_di = prev_d + offset;
+                if (cur_s >= o + e && prev_di >= 0) {
+                    const int64_t mv = M_full[(cur_s - o - e) * n_diags + prev_di];
+                    if (mv != OR_NEG_INF && mv + 1 == cur_row) {
+                        cur_s -= o + e; cur_d = prev_d; cur_row = mv; cur_matrix = 0;
+                        continue;
+                    }
+                }
+                if (cur_s >= e && prev_di >= 0) {
+                    const int64_t iv = I_arr[(cur_s - e) * n_diags + prev_di];
+                    if (iv != OR_NEG_INF && iv + 1 == cur_row) {
+                        cur_s -= e; cur_d = prev_d; cur_row = iv; cur_matrix = 1;
+                        continue;
+                    }
+                }
+                break;
+            } else { /* D */
+                OR_PUSH(3);
+                const int64_t prev_d = cur_d + 1, prev_di = prev_d + offset;
+                if (cur_s >= o + e && prev_di < n_diags) {
+                    const int64_t mv = M_full[(cur_s - o - e) * n_diags + prev_di];
+                    if (mv != OR_NEG_INF && mv == cur_row) {
+                        cur_s -= o + e; cur_d = prev_d; cur_row = mv; cur_matrix = 0;
+                        continue;
+                    }
+                }
+                if (cur_s >= e && prev_di < n_diags) {
+                    const int64_t dv = D_arr[(cur_s - e) * n_diags + prev_di];
+                    if (dv != OR_NEG_INF && dv == cur_row) {
+                        cur_s -= e; cur_d = prev_d; cur_row = dv; cur_matrix = 2;
+                        continue;
+                    }
+                }
+                break;
+            }
+        }
+#undef OR_PUSH
+        /* :602-621 run-length encode, first operation first */
+        int64_t nr = 0;
+        for (int64_t i = n_ops - 1; i >= 0;) {
+            const uint8_t op = ops[i];
+            int64_t cnt = 1;
+            while (i - cnt >= 0 && ops[i - cnt] == op) ++cnt;
+            if (nr < cap) runs[nr] = (uint32_t)((cnt << 2) | op);
+            ++nr;
+            i -= cnt;
+        }
+        *n_runs = nr <= cap && n_ops <= ops_cap ? nr : -2;
+        free(ops);
+    }
+out:
+    free(M_pre); free(M_full); free(I_arr); free(D_arr);
+    return opt_score;
+}
+
+typedef struct {
+    const uint64_t *qp, *rp;
+    const int32_t *qo, *ro, *ql, *rl, *me;
+    int32_t me_scalar;
+    int64_t x, o, e, n;
+    int32_t *score;
+    uint32_t *runs;
+    const int64_t *run_off;
+    int32_t *n_runs;
+    int64_t next; pthread_mutex_t *lock;
+} or_cigar_job;
+
+static void *or_cigar_worker(void *arg) {
+    or_cigar_job *j = (or_cigar_job *)arg;
+    for (;;) {
+        pthread_mutex_lock(j->lock);
+        int64_t i = j->next++;
+        pthread_mutex_unlock(j->lock);
+        if (i >= j->n) break;
+        int64_t nr = 0;
+        int64_t me = j->me ? j->me[i] : j->me_scalar;
+        j->score[i] = (int32_t)or_wfa_affine_cigar(j->qp + j->qo[i], j->ql[i], j->rp + j->ro[i],
+                                                   j->rl[i], j->x, j->o, j->e, me,
+                                                   j->runs + j->run_off[i],
+                                                   j->run_off[i + 1] - j->run_off[i], &nr);
+        j->n_runs[i] = (int32_t)nr;
+    }
+    return NULL;
+}
+
+/* or_wfa_affine_cigar over a batch (layout as or_batch); pair i's runs go to
+ * runs[run_off[i] .. run_off[i + 1]). */
+int or_wfa_cigar_batch(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_off,
+                       const int32_t *r_off, const int32_t *q_len, const int32_t *r_len,
+                       int64_t n_pairs, const int32_t *max_err, int32_t max_err_scalar,
+                       int32_t x, int32_t o, int32_t e, int32_t *score, uint32_t *runs,
+                       const int64_t *run_off, int32_t *n_runs, int n_threads) {
+    pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
+    or_cigar_job j;
+    memset(&j, 0, sizeof(j));
+    j.qp = q_packed; j.rp = r_packed; j.qo = q_off; j.ro = r_off; j.ql = q_len; j.rl = r_len;
+    j.me = max_err; j.me_scalar = max_err_scalar; j.x = x; j.o = o; j.e = e; j.n = n_pairs;
+    j.score = score; j.runs = runs; j.run_off = run_off; j.n_runs = n_runs;
+    j.next = 0; j.lock = &lock;
+    if (n_threads < 1) n_threads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
+    for (int t = 0; t < n_threads; ++t) pthread_create(&th[t], NULL, or_cigar_worker, &j);
+    for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    return 0;
 }
 
 /* ---------------------------------------------------- textbook Gotoh (check) */

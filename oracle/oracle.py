@@ -53,6 +53,12 @@ def lib():
             f = getattr(L, name)
             f.restype = _i64
             f.argtypes = [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64]
+        L.or_wfa_affine_cigar.restype = _i64
+        L.or_wfa_affine_cigar.argtypes = [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _i64,
+                                          ctypes.POINTER(_i64)]
+        L.or_wfa_cigar_batch.restype = ctypes.c_int
+        L.or_wfa_cigar_batch.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _i32,
+                                         _i32, _vp, _vp, _vp, _vp, ctypes.c_int]
         L.or_batch.restype = ctypes.c_int
         L.or_batch.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
                                _vp, _i32, _i32, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _i32,
@@ -142,3 +148,49 @@ def batch(mode, q_packed, r_packed, q_off, r_off, q_len, r_len, *, kind=None,
     if rc != 0:
         raise RuntimeError("oracle batch failed")
     return score, qe, re_
+
+
+CIGAR_OPS = "=XID"
+
+
+def runs_to_cigar(runs) -> str:
+    """Run-length operations (count << 2 | op) as the reference's CIGAR
+    string (align_cpu.mojo:602-621): '=' match, 'X' mismatch, 'I', 'D'."""
+    return "".join(f"{int(v) >> 2}{CIGAR_OPS[int(v) & 3]}" for v in runs)
+
+
+def wfa_affine_cigar(q, r, x=4, o=6, e=2, max_error=200):
+    """wfa_affine_cigar_cpu (align_cpu.mojo:380-623): (cost or -1, CIGAR)."""
+    qw, ql = _words(q)
+    rw, rl = _words(r)
+    cap = ql + rl + 8
+    runs = np.zeros(cap, np.uint32)
+    n = ctypes.c_int64(0)
+    c = lib().or_wfa_affine_cigar(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl, x, o, e,
+                                  max_error, _ptr(runs), cap, ctypes.byref(n))
+    assert n.value >= 0
+    return int(c), runs_to_cigar(runs[:n.value])
+
+
+def wfa_cigar_batch(q_packed, r_packed, q_off, r_off, q_len, r_len, *, max_error, x=4, o=6, e=2,
+                    threads=8):
+    """or_wfa_cigar_batch: (costs int32[n], list of CIGAR strings)."""
+    n = int(len(q_len))
+    q_len = np.ascontiguousarray(q_len, np.int32)
+    r_len = np.ascontiguousarray(r_len, np.int32)
+    cap = q_len.astype(np.int64) + r_len + 8
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(cap)
+    runs = np.zeros(max(int(off[-1]), 1), np.uint32)
+    nr = np.zeros(n, np.int32)
+    score = np.zeros(n, np.int32)
+    me = np.ascontiguousarray(max_error, np.int32) if np.ndim(max_error) else None
+    lib().or_wfa_cigar_batch(_ptr(_nonempty(np.ascontiguousarray(q_packed, np.uint64))),
+                             _ptr(_nonempty(np.ascontiguousarray(r_packed, np.uint64))),
+                             _ptr(np.ascontiguousarray(q_off, np.int32)),
+                             _ptr(np.ascontiguousarray(r_off, np.int32)), _ptr(q_len),
+                             _ptr(r_len), n, _ptr(me) if me is not None else None,
+                             int(max_error) if me is None else 0, x, o, e, _ptr(score),
+                             _ptr(runs), _ptr(off), _ptr(nr), threads)
+    assert np.all(nr >= 0)
+    return score, [runs_to_cigar(runs[off[i]:off[i] + nr[i]]) for i in range(n)]

@@ -351,3 +351,95 @@ def test_sw_band_wider_than_both_sequences_is_clamped_exactly():
         sc = rng.choice([DEFAULT_SW, CONFIG_SW])
         eff = min(band, max(len(q), len(r)))
         assert O.sw_banded(q, r, band=band, **sc) == O.sw_banded(q, r, band=eff, **sc)
+
+
+# ----------------------------------------------------------- WFA with CIGAR
+
+def cigar_ops(cigar):
+    """'3=1X2I' -> [(3, '='), (1, 'X'), (2, 'I')]"""
+    import re
+    return [(int(n), op) for n, op in re.findall(r"(\d+)([=XID])", cigar)]
+
+
+def check_cigar(q, r, cigar, x=4, o=6, e=2):
+    """Cost of the CIGAR (mismatch x, gap of l bases o + l e), and that it
+    walks both sequences exactly: '=' on equal bases, 'X' on different."""
+    i = j = cost = 0
+    for n, op in cigar_ops(cigar):
+        assert n > 0
+        if op in "=X":
+            for _ in range(n):
+                assert (q[i] == r[j]) == (op == "="), (q, r, cigar)
+                i += 1
+                j += 1
+            cost += x * n if op == "X" else 0
+        elif op == "I":
+            i += n
+            cost += o + e * n
+        else:
+            j += n
+            cost += o + e * n
+    assert (i, j) == (len(q), len(r)), (q, r, cigar)
+    return cost
+
+
+def test_wfa_cigar_reference_known_answers():
+    """test_alignment.mojo:201-279: exact CIGAR strings and scores."""
+    assert O.wfa_affine_cigar("ACGTACGT", "ACGTACGT") == (0, "8=")
+    assert O.wfa_affine_cigar("ACGT", "ACAT") == (4, "2=1X1=")
+    assert O.wfa_affine_cigar("ACGGT", "ACGT") == (8, "3=1I1=")
+    assert O.wfa_affine_cigar("ACGT", "ACGGT") == (8, "3=1D1=")
+    assert O.wfa_affine_cigar("ACGT", "ACGGGT") == (10, "3=2D1=")
+    for q, r in (("ACGT", "ACAT"), ("ACGGT", "ACGT"), ("ACGT", "ACGGT"), ("ACGT", "ACGGGT"),
+                 ("AAAA", "TTTT"), ("ACGTACGT", "ACGTACGT")):
+        c, cig = O.wfa_affine_cigar(q, r)
+        assert c == O.wfa_affine(q, r)
+        assert check_cigar(q, r, cig) == c
+
+
+def test_wfa_cigar_edge_cases():
+    assert O.wfa_affine_cigar("", "") == (0, "")
+    assert O.wfa_affine_cigar("", "ACG") == (12, "3D")
+    assert O.wfa_affine_cigar("ACGTA", "") == (16, "5I")
+    assert O.wfa_affine_cigar("ACGT", "TTTT", max_error=11) == (-1, "")
+    assert O.wfa_affine_cigar("A" * 40, "A" * 60, max_error=45) == (-1, "")
+
+
+def test_wfa_cigar_cost_and_walk_random():
+    """Random pairs (related and unrelated, several penalty sets): the cost
+    equals wfa_affine_cpu's and the textbook Gotoh optimum, and the CIGAR
+    reproduces that cost and walks both sequences exactly."""
+    rng = random.Random(23)
+    pens = [(4, 6, 2), (1, 0, 1), (3, 2, 1), (5, 1, 3), (2, 5, 1)]
+    for _ in range(1500):
+        q = rseq(rng, rng.randint(0, 60))
+        r = "".join(c if rng.random() > 0.12 else rng.choice("ACGT") for c in q
+                    if rng.random() > 0.06) if rng.random() < 0.7 else rseq(rng, rng.randint(0, 60))
+        x, o, e = rng.choice(pens)
+        c, cig = O.wfa_affine_cigar(q, r, x, o, e, max_error=400)
+        assert c == O.wfa_affine(q, r, x, o, e, max_error=400)
+        assert c == O.gotoh_global(q, r, x, o, e)
+        assert check_cigar(q, r, cig, x, o, e) == c
+
+
+def test_wfa_cigar_long_diverged_and_batch_driver():
+    rng = random.Random(31)
+    qs, rs, mes = [], [], []
+    for _ in range(40):
+        q = rseq(rng, rng.randint(200, 1200))
+        r = "".join(c if rng.random() > 0.03 else rng.choice("ACGT") for c in q
+                    if rng.random() > 0.01)
+        c = O.wfa_affine(q, r, max_error=1000)
+        assert c >= 0
+        qs.append(q); rs.append(r); mes.append(c + rng.choice([0, 2, 30]))
+    pq = [O.pack_ascii(s) for s in qs]
+    pr = [O.pack_ascii(s) for s in rs]
+    qoff = np.cumsum([0] + [w.size for w, _ in pq])[:-1].astype(np.int32)
+    roff = np.cumsum([0] + [w.size for w, _ in pr])[:-1].astype(np.int32)
+    cost, cigars = O.wfa_cigar_batch(np.concatenate([w for w, _ in pq]),
+                                     np.concatenate([w for w, _ in pr]), qoff, roff,
+                                     [len(s) for s in qs], [len(s) for s in rs],
+                                     max_error=np.array(mes, np.int32), threads=4)
+    for q, r, me, c, cig in zip(qs, rs, mes, cost, cigars):
+        assert (c, cig) == O.wfa_affine_cigar(q, r, max_error=me)
+        assert check_cigar(q, r, cig) == c

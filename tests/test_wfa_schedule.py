@@ -203,3 +203,117 @@ def test_incremental_step_state_equals_division_form():
             assert (back(ms, dO, nM) if si >= dO else -1) == \
                 (((s - o - e) // g) % nM if s >= o + e else -1)
             assert (back(is_, de, nI) if si >= de else -1) == (((s - e) // g) % nI if s >= e else -1)
+
+
+def wfa_cigar_sched(q, r, x, o, e, me):
+    """The traceback schedule of the CIGAR kernel: the cost first (any exact
+    schedule), then a history of every score step 0..cost/g, each row the
+    diagonals -K..K with K = min(me, max(qlen, rlen), R(cost)) and only
+    |k| <= R(s) computed, then the reference's backtrace (align_cpu.mojo:
+    510-600) with M before extension recomputed from the stored rows.
+    Returns (cost, CIGAR string)."""
+    ql, rl = len(q), len(r)
+    cost = wfa_sched(q, r, x, o, e, me)
+    if cost < 0 or (ql == 0 and rl == 0):
+        return cost, ""
+    g = math.gcd(math.gcd(x, o), e)
+    R = lambda s: (s - o) // e if s >= o + e else 0
+    K = min(me, max(ql, rl), R(cost))
+    W = 2 * K + 1
+    rows = cost // g + 1
+    M = [[NEG] * W for _ in range(rows)]
+    I = [[NEG] * W for _ in range(rows)]
+    D = [[NEG] * W for _ in range(rows)]
+    def ext(row, k):
+        col = row - k
+        while row < ql and 0 <= col < rl and q[row] == r[col]:
+            row += 1; col += 1
+        return row
+    M[0][K] = ext(0, 0)
+    dx, dO, de = x // g, (o + e) // g, e // g
+    for si in range(1, rows):
+        s = si * g
+        lo, hi = max(-rl, -me, -R(s)), min(ql, me, R(s))
+        for k in range(max(lo, -K), min(hi, K) + 1):
+            di = k + K
+            a = M[si - dO][di - 1] if si >= dO and di > 0 else NEG
+            b = M[si - dO][di + 1] if si >= dO and di < W - 1 else NEG
+            c = I[si - de][di - 1] if si >= de and di > 0 else NEG
+            d = D[si - de][di + 1] if si >= de and di < W - 1 else NEG
+            m = M[si - dx][di] if si >= dx else NEG
+            iv = max(a + 1 if a >= 0 else NEG, c + 1 if c >= 0 else NEG)
+            dv = max(b, d)
+            mv = max(m + 1 if m >= 0 else NEG, iv, dv)
+            I[si][di], D[si][di] = iv, dv
+            M[si][di] = ext(mv, k) if mv >= 0 else NEG
+    get = lambda A, si, k: A[si][k + K] if -K <= k <= K else NEG
+    def m_pre(si, k):
+        if si == 0:
+            return 0 if k == 0 else NEG
+        v = NEG
+        if si >= dx and get(M, si - dx, k) >= 0:
+            v = max(v, get(M, si - dx, k) + 1)
+        for A in (I, D):
+            if get(A, si, k) >= 0:
+                v = max(v, get(A, si, k))
+        return v
+    ops = []
+    si, d, mat = cost // g, ql - rl, 0
+    row = get(M, si, d)
+    while True:
+        if mat == 0:
+            pre = m_pre(si, d)
+            ops += ["="] * (row - pre)
+            row = pre
+            if si == 0 and d == 0:
+                break
+            pm = get(M, si - dx, d) if si >= dx else NEG
+            if pm >= 0 and pm + 1 == row:
+                ops.append("X"); si -= dx; row = pm; continue
+            if get(I, si, d) >= 0 and get(I, si, d) == row:
+                mat = 1; continue
+            if get(D, si, d) >= 0 and get(D, si, d) == row:
+                mat = 2; continue
+            break
+        elif mat == 1:
+            ops.append("I")
+            mv = get(M, si - dO, d - 1) if si >= dO else NEG
+            if mv >= 0 and mv + 1 == row:
+                si -= dO; d -= 1; row = mv; mat = 0; continue
+            iv = get(I, si - de, d - 1) if si >= de else NEG
+            if iv >= 0 and iv + 1 == row:
+                si -= de; d -= 1; row = iv; continue
+            break
+        else:
+            ops.append("D")
+            mv = get(M, si - dO, d + 1) if si >= dO else NEG
+            if mv >= 0 and mv == row:
+                si -= dO; d += 1; row = mv; mat = 0; continue
+            dv = get(D, si - de, d + 1) if si >= de else NEG
+            if dv >= 0 and dv == row:
+                si -= de; d += 1; row = dv; continue
+            break
+    out, i = "", len(ops) - 1
+    while i >= 0:
+        n = 1
+        while i - n >= 0 and ops[i - n] == ops[i]:
+            n += 1
+        out += f"{n}{ops[i]}"
+        i -= n
+    return cost, out
+
+
+def test_cigar_schedule_equals_oracle():
+    rng = random.Random(8)
+    pens = [(4, 6, 2), (1, 0, 1), (3, 2, 1), (5, 1, 3), (2, 7, 1), (6, 4, 2), (9, 3, 3)]
+    for _ in range(2500):
+        a = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 30)))
+        if rng.random() < 0.4:
+            b = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 30)))
+        else:
+            b = "".join(c if rng.random() > 0.15 else rng.choice("ACGT")
+                        for c in a if rng.random() > 0.1)
+        x, o, e = rng.choice(pens)
+        me = rng.choice([0, 3, 8, 20, 60, 150, 400])
+        assert wfa_cigar_sched(a, b, x, o, e, me) == O.wfa_affine_cigar(a, b, x, o, e, me), \
+            (a, b, x, o, e, me)
