@@ -189,6 +189,67 @@ def serving_extra(m, duration=3.0):
     return res
 
 
+def nw_extra(m, dev, w, total_cells, steps=5):
+    """Banded NW on the same 200,000-pair batch (band 16-64, 2/-4/-4/-2): GCUPS
+    over the same valid band cells, automatic (register lanes) and plain
+    (one workgroup per pair) layouts, equal on every pair."""
+    up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
+                                                  "q_off", "r_off", "band")}
+    n = w.n_pairs
+    out = [m.DeviceArray(n, np.int32) for _ in range(3)]
+    res = {}
+    got = {}
+    for name, layout, st in (("auto", 0, steps), ("plain", 1, 1)):
+        def step():
+            m._lib.call("cgm_nw_banded_launch", up["q_packed"].dptr, up["r_packed"].dptr,
+                        up["q_len"].dptr, up["r_len"].dptr, up["q_off"].dptr, up["r_off"].dptr, n,
+                        w.q_packed.size, w.r_packed.size, up["band"].dptr, 0, SW["match"],
+                        SW["mismatch"], SW["gap_open"], SW["gap_extend"], out[0].dptr,
+                        out[1].dptr, out[2].dptr, layout, dev.stream)
+        wall, kms = time_steps(m, dev, step, st, 1)
+        got[name] = [dev.download(a, n) for a in out]
+        res[name] = {"gcups": total_cells * st / wall / 1e9, "ms_per_step": wall / st * 1e3}
+    res["identical"] = all(bool(np.array_equal(a, b)) for a, b in zip(got["auto"], got["plain"]))
+    log(f"[bench] nw: {res}")
+    return res
+
+
+def cigar_extra(m, dev, gen, n_pairs=20_000, steps=2):
+    """WFA with traceback on 20,000 pairs of 1-10 kb (x=4, o=6, e=2): pairs/s
+    of cgm_wfa_cigar_launch (costs, history sizing, history + backtrace),
+    beside the cost-only automatic launch on the same batch."""
+    w = gen.generate(3, n_pairs)
+    n = w.n_pairs
+    up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
+                                                  "q_off", "r_off", "max_error")}
+    cfg = m.WFAConfig(4, 6, 2, int(w.max_error.max()))
+    cap = m.cigar_capacity(w.max_error, cfg)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(cap)
+    d_off = dev.upload(off)
+    d_runs = m.DeviceArray(int(off[-1]), np.uint32)
+    d_cost, d_len = m.DeviceArray(n, np.int32), m.DeviceArray(n, np.int32)
+
+    def step():
+        m._lib.call("cgm_wfa_cigar_launch", up["q_packed"].dptr, up["r_packed"].dptr,
+                    up["q_len"].dptr, up["r_len"].dptr, up["q_off"].dptr, up["r_off"].dptr, n,
+                    w.q_packed.size, w.r_packed.size, up["max_error"].dptr, 0, 4, 6, 2,
+                    d_cost.dptr, d_runs.dptr, d_off.dptr, d_len.dptr, dev.stream)
+    wall, kms = time_steps(m, dev, step, steps, 1)
+    cost = dev.download(d_cost)
+    lens = dev.download(d_len)
+    # history traffic of the traceback kernel: three int16/int32 planes of
+    # (cost / 2 + 1) rows x (2 R(cost) + 1) diagonals per pair, written once
+    K = np.minimum(np.minimum(w.max_error, np.maximum(w.q_len, w.r_len)),
+                   np.where(cost >= 8, (cost - 6) // 2, 0)).astype(np.int64)
+    cells = ((cost.astype(np.int64) // 2 + 1) * (2 * K + 1)).sum()
+    res = {"pairs": n, "pairs_per_s": n * steps / wall, "ms_per_call": wall / steps * 1e3,
+           "mean_runs": float(lens.mean()), "history_cells": int(cells),
+           "all_cigars_present": bool(np.all(lens > 0))}
+    log(f"[bench] cigar: {res}")
+    return res
+
+
 def ingest_extra(m, dev, gen, n_pairs=200_000, steps=5):
     """Device ingest: the queries and references of the batched SW workload
     (200,000 + 200,000 sequences of ~100-300 bp) as ASCII in HBM, encoded
@@ -324,6 +385,8 @@ def main():
         extras["mixed_config4"] = mixed_extra(m, dev, gen)
         extras["serving_config5"] = serving_extra(m)
         extras["ingest_config2"] = ingest_extra(m, dev, gen)
+        extras["wfa_cigar_config3"] = cigar_extra(m, dev, gen)
+        extras["nw_config2"] = nw_extra(m, dev, w, total_cells)
         wall_p, kms_p = time_steps(m, dev, lambda: run.launch(1), max(2, args.steps // 5), 1)
         plain_res = run.results()
         same = all(np.array_equal(a, b) for a, b in zip(plain_res, gpu_res))

@@ -13,7 +13,8 @@ from .align import (AlignConfig, AlignResult, MixedPlan, SwResult, WFAConfig, al
                     launch_banded_sw, launch_wfa_affine, set_default_tuning,
                     smith_waterman_banded, sw_banded_batch, wfa_affine, wfa_affine_batch,
                     wfa_affine_cpu, WFAResult, cigar_capacity, runs_to_cigar,
-                    wfa_affine_cigar_batch, wfa_affine_cigar_cpu)
+                    wfa_affine_cigar_batch, wfa_affine_cigar_cpu, needleman_wunsch_banded,
+                    nw_banded_batch)
 from .device import DeviceArray, GenomicsDevice, Timer
 from .serve import GeneratorHandle, ServingEngine
 from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_to_2bit,
@@ -28,5 +29,6 @@ __all__ = [
     "GenomicsDevice", "Timer", "GeneratorHandle", "ServingEngine", "BASES_PER_WORD", "Base",
     "SequenceBatch", "SequenceView", "ascii_to_2bit", "get_base_from_word", "get_view",
     "pack_codes", "concat_ascii", "decode_ascii", "WFAResult", "cigar_capacity",
-    "runs_to_cigar", "wfa_affine_cigar_batch", "wfa_affine_cigar_cpu", "encode_ascii_launch", "encode_batch_gpu",
+    "runs_to_cigar", "wfa_affine_cigar_batch", "wfa_affine_cigar_cpu",
+    "needleman_wunsch_banded", "nw_banded_batch", "encode_ascii_launch", "encode_batch_gpu",
 ]

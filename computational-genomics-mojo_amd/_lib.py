@@ -95,6 +95,10 @@ SIGNATURES = {
                            _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32],
     "cgm_wfa_affine_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                             _i32, _i32, _i32, _vp, _i32],
+    "cgm_nw_banded_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
+                             _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp],
+    "cgm_nw_banded_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
+                           _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32],
     "cgm_wfa_cigar_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _i32,
                              _i32, _i32, _vp, _vp, _vp, _vp, _vp],
     "cgm_wfa_cigar_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _i32, _i32,

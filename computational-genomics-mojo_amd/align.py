@@ -93,6 +93,24 @@ def sw_banded_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignCo
     return score, qe, re_
 
 
+def nw_banded_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignConfig,
+                    band=None, layout=_lib.LAYOUT_AUTO):
+    """Banded NW (needleman_wunsch_banded, align_cpu.mojo:312-377) of every
+    pair on the GPU; host arrays in, (score, query_end, ref_end) out."""
+    q_packed = np.ascontiguousarray(q_packed, np.uint64)
+    r_packed = np.ascontiguousarray(r_packed, np.uint64)
+    ql, rl, qo, ro, bd = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(band)
+    n = int(ql.size)
+    score, qe, re_ = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    if n:
+        _lib.call("cgm_nw_banded_host", _lib.ptr(_nonempty(q_packed)), _lib.ptr(_nonempty(r_packed)),
+                  _lib.ptr(ql), _lib.ptr(rl), _lib.ptr(qo), _lib.ptr(ro), n, q_packed.size,
+                  r_packed.size, _lib.ptr(bd), cfg.band_width, cfg.match_score, cfg.mismatch,
+                  cfg.gap_open, cfg.gap_extend, _lib.ptr(score), _lib.ptr(qe), _lib.ptr(re_),
+                  layout)
+    return score, qe, re_
+
+
 def wfa_affine_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: WFAConfig,
                      max_error=None, layout=_lib.LAYOUT_AUTO):
     """Gap-affine WFA cost (or -1) of every pair; host arrays in and out."""
@@ -177,6 +195,15 @@ def smith_waterman_banded(query: SequenceView, reference: SequenceView,
     qp, qo, ql = _view_arrays(query)
     rp, ro, rl = _view_arrays(reference)
     s, qe, re_ = sw_banded_batch(qp, qo, ql, rp, ro, rl, cfg)
+    return AlignResult(int(s[0]), 0, 0, int(qe[0]), int(re_[0]), [])
+
+
+def needleman_wunsch_banded(query: SequenceView, reference: SequenceView,
+                            cfg: AlignConfig) -> AlignResult:
+    """align_cpu.mojo:312-377 (same name and signature), computed on the GPU."""
+    qp, qo, ql = _view_arrays(query)
+    rp, ro, rl = _view_arrays(reference)
+    s, qe, re_ = nw_banded_batch(qp, qo, ql, rp, ro, rl, cfg)
     return AlignResult(int(s[0]), 0, 0, int(qe[0]), int(re_[0]), [])
 
 

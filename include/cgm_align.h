@@ -110,6 +110,37 @@ int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                          int32_t *out_score, int32_t *out_q_end, int32_t *out_r_end,
                          int32_t layout, void *stream);
 
+/* ------------------------------------------------------- banded NW batch
+ * The batch form of needleman_wunsch_banded (genomics/cpu/align_cpu.mojo:
+ * 312-377), which the reference has on the CPU only: global score within
+ * |ri - qi| <= band (every gap base costs gap_open + gap_extend inside the
+ * matrix; the row before the first charges gap_open + c * gap_extend for a
+ * leading reference gap of c bases; no leading query gap), out_score =
+ * that score or -1000000 when |rlen - qlen| > band, out_q_end = qlen - 1,
+ * out_r_end = rlen - 1.  The reference allocates its two rows with
+ * List(capacity=width) and indexes them without appending (align_cpu.mojo:
+ * 324-337); the rows are taken to hold 2*band+1 entries.  Arguments as
+ * cgm_sw_banded_launch; layout CGM_LAYOUT_AUTO (one pair per lane for
+ * effective bands <= 64, one workgroup per pair above) or CGM_LAYOUT_PLAIN
+ * (one workgroup per pair).  Pairs whose values could leave int32 range,
+ * (qlen + rlen + 2 band + 4) * (|match| + |mismatch| + |open| + |extend|)
+ * + 10^6 > 2^30, or with an effective band above 1600, are reported with
+ * CGM_BAD_PAIR. */
+int cgm_nw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed,
+                         const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                         const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                         const int32_t *band, int32_t band_scalar, int32_t match,
+                         int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                         int32_t *out_score, int32_t *out_q_end, int32_t *out_r_end,
+                         int32_t layout, void *stream);
+/* Host-pointer form (copies in, runs on an internal stream, copies out). */
+int cgm_nw_banded_host(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_len,
+                       const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                       int64_t n_pairs, int64_t q_words, int64_t r_words, const int32_t *band,
+                       int32_t band_scalar, int32_t match, int32_t mismatch, int32_t gap_open,
+                       int32_t gap_extend, int32_t *out_score, int32_t *out_q_end,
+                       int32_t *out_r_end, int32_t layout);
+
 /* ------------------------------------------------------ gap-affine WFA batch
  * Replaces launch_wfa_affine[max_error, x, o, e] (genomics/gpu/align_gpu.mojo:
  * 496-535) with the semantics of wfa_affine_cpu (align_cpu.mojo:75-226):

@@ -11,6 +11,7 @@
  *   or_sw_banded         genomics/cpu/align_cpu.mojo:229-309  (smith_waterman_banded)
  *   or_wfa_affine        genomics/cpu/align_cpu.mojo:75-226   (wfa_affine_cpu)
  *   or_wfa_affine_cigar  genomics/cpu/align_cpu.mojo:380-623  (wfa_affine_cigar_cpu)
+ *   or_nw_banded         genomics/cpu/align_cpu.mojo:312-377  (needleman_wunsch_banded)
  * Independent cross-checks (not restatements; used to pin the above):
  *   or_sw_antidiag       the same SW recurrence swept by anti-diagonals with
  *                        an explicit (H desc, qi asc, ri asc) reduction
@@ -30,6 +31,13 @@
  *     CIGARs it returns (mismatch before insertion before deletion in M,
  *     gap opening before extension in I and D) follows align_cpu.mojo:
  *     520-600 line by line and is pinned by the known answers only.
+ *   or_nw_banded: reference known answers test_alignment.mojo:61-80, closed
+ *     forms (identical pair, one substitution, band too narrow for the
+ *     length difference, empty inputs) and a brute force over every banded
+ *     path on tiny inputs.  The reference allocates its two rows with
+ *     List(capacity=width) and indexes them without appending
+ *     (align_cpu.mojo:324-337); this restatement takes the rows to hold
+ *     width entries, which is what the indexing intends.
  *   or_sw_banded: reference known answers test_alignment.mojo:24-58, closed
  *     forms (identical pair, no common base, empty input, band 0, one
  *     substitution), invariants (band monotonicity, single-diagonal Kadane
@@ -484,6 +492,45 @@ int or_wfa_cigar_batch(const uint64_t *q_packed, const uint64_t *r_packed, const
     return 0;
 }
 
+/* ------------------------------------------------------- banded NW (exact) */
+
+/* align_cpu.mojo:312-377.  Global score within the band |ri - qi| <= band;
+ * out[0] = score, out[1] = query_end = qlen - 1, out[2] = ref_end = rlen - 1. */
+void or_nw_banded(const uint64_t *q, int64_t qlen, const uint64_t *r, int64_t rlen,
+                  int64_t match, int64_t mismatch, int64_t gap_open, int64_t gap_extend,
+                  int64_t band, int64_t *out) {
+    const int64_t width = 2 * band + 1;
+    int64_t *H_prev = (int64_t *)malloc((size_t)width * sizeof(int64_t));
+    int64_t *H_curr = (int64_t *)malloc((size_t)width * sizeof(int64_t));
+    for (int64_t j = 0; j < width; ++j) { /* :327-334 */
+        const int64_t ri = -band + j;
+        if (ri < 0) H_prev[j] = OR_NEG_INF;
+        else if (ri == 0) H_prev[j] = 0;
+        else H_prev[j] = gap_open + ri * gap_extend;
+    }
+    for (int64_t qi = 0; qi < qlen; ++qi) { /* :336-361 */
+        for (int64_t j = 0; j < width; ++j) H_curr[j] = OR_NEG_INF;
+        for (int64_t ro = 0; ro < width; ++ro) {
+            const int64_t ri = qi - band + ro;
+            if (ri < 0 || ri >= rlen) continue;
+            const int64_t sub = or_get_base(q, qi) == or_get_base(r, ri) ? match : mismatch;
+            const int64_t h_diag = H_prev[ro];
+            const int64_t h_up = ro + 1 < width ? H_prev[ro + 1] : OR_NEG_INF;
+            const int64_t h_left = ro > 0 ? H_curr[ro - 1] : OR_NEG_INF;
+            H_curr[ro] = or_max(h_diag + sub, or_max(h_up + gap_open + gap_extend,
+                                                     h_left + gap_open + gap_extend));
+        }
+        for (int64_t j = 0; j < width; ++j) H_prev[j] = H_curr[j];
+    }
+    const int64_t final_offset = (rlen - 1) - (qlen - 1) + band; /* :363-367 */
+    int64_t final_score = OR_NEG_INF;
+    if (final_offset >= 0 && final_offset < width) final_score = H_prev[final_offset];
+    out[0] = final_score;
+    out[1] = qlen - 1;
+    out[2] = rlen - 1;
+    free(H_prev); free(H_curr);
+}
+
 /* ---------------------------------------------------- textbook Gotoh (check) */
 
 /* Global alignment, minimum cost: match 0, mismatch x, a gap of length l
@@ -546,6 +593,16 @@ typedef struct {
 static void or_do_pair(const or_job *j, int64_t i) {
     const uint64_t *q = j->qp + j->qo[i];
     const uint64_t *r = j->rp + j->ro[i];
+    if (j->mode == 2) {
+        int64_t band = j->band ? j->band[i] : j->band_scalar;
+        int64_t out[3];
+        or_nw_banded(q, j->ql[i], r, j->rl[i], j->match, j->mismatch, j->gap_open, j->gap_extend,
+                     band, out);
+        j->score[i] = (int32_t)out[0];
+        if (j->q_end) j->q_end[i] = (int32_t)out[1];
+        if (j->r_end) j->r_end[i] = (int32_t)out[2];
+        return;
+    }
     int is_wfa = j->kind ? j->kind[i] != 0 : j->mode == 1;
     if (is_wfa) {
         int64_t me = j->max_err ? j->max_err[i] : j->max_err_scalar;
@@ -578,7 +635,8 @@ static void *or_worker(void *arg) {
 
 /* Runs every pair of a batch with n_threads POSIX threads pulling pair
  * indices from a shared counter.  kind[i] (0 = SW, 1 = WFA) may be NULL, in
- * which case mode selects: 0 = all SW, 1 = all WFA. */
+ * which case mode selects: 0 = all SW, 1 = all WFA; mode 2 = all banded NW
+ * (kind ignored). */
 int or_batch(int mode,
              const uint64_t *q_packed, const uint64_t *r_packed,
              const int32_t *q_off, const int32_t *r_off,

@@ -45,7 +45,7 @@ def lib():
         L.or_pack_ascii.argtypes = [ctypes.c_char_p, _i64, _vp, _vp]
         L.or_get_base_public.restype = ctypes.c_int
         L.or_get_base_public.argtypes = [_vp, _i64]
-        for name in ("or_sw_banded", "or_sw_antidiag"):
+        for name in ("or_sw_banded", "or_sw_antidiag", "or_nw_banded"):
             f = getattr(L, name)
             f.restype = None
             f.argtypes = [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp]
@@ -110,6 +110,17 @@ def sw_antidiag(q, r, match=2, mismatch=-3, gap_open=-5, gap_extend=-2, band=32)
     out = np.zeros(3, dtype=np.int64)
     lib().or_sw_antidiag(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl,
                          match, mismatch, gap_open, gap_extend, band, _ptr(out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def nw_banded(q, r, match=2, mismatch=-3, gap_open=-5, gap_extend=-2, band=32):
+    """(score, query_end, ref_end) of needleman_wunsch_banded,
+    align_cpu.mojo:312-377 (defaults = default_config())."""
+    qw, ql = _words(q)
+    rw, rl = _words(r)
+    out = np.zeros(3, dtype=np.int64)
+    lib().or_nw_banded(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl, match, mismatch,
+                       gap_open, gap_extend, band, _ptr(out))
     return int(out[0]), int(out[1]), int(out[2])
 
 

@@ -443,3 +443,89 @@ def test_wfa_cigar_long_diverged_and_batch_driver():
     for q, r, me, c, cig in zip(qs, rs, mes, cost, cigars):
         assert (c, cig) == O.wfa_affine_cigar(q, r, max_error=me)
         assert check_cigar(q, r, cig) == c
+
+
+# -------------------------------------------------------------- banded NW
+
+NW_NEG = -1000000
+
+
+def test_nw_reference_known_answers():
+    """test_alignment.mojo:61-80 (default_config, band 4)."""
+    assert O.nw_banded("ACGT", "ACGT", band=4)[0] == 8
+    assert O.nw_banded("AAAA", "TTTT", band=4)[0] == -12
+
+
+@pytest.mark.parametrize("L,band", [(1, 0), (7, 0), (30, 3), (100, 32), (65, 64)])
+def test_nw_closed_forms(L, band):
+    rng = random.Random(L * 131 + band)
+    q = rseq(rng, L)
+    assert O.nw_banded(q, q, band=band) == (2 * L, L - 1, L - 1)
+    p = rng.randrange(L)
+    r = q[:p] + "ACGT"[("ACGT".index(q[p]) + 1) % 4] + q[p + 1:]
+    # one substitution: match everywhere else unless two gaps (2 x -7) beat
+    # the mismatch (-3 - 2 = -5 versus the match), which they never do
+    assert O.nw_banded(q, r, band=band)[0] == 2 * (L - 1) - 3
+
+
+def test_nw_band_too_narrow_and_empty():
+    assert O.nw_banded("ACGTACGT", "ACG", band=4)[0] == NW_NEG   # |qlen - rlen| = 5 > 4
+    assert O.nw_banded("", "", band=3) == (0, -1, -1)
+    assert O.nw_banded("", "ACG", band=3) == (-5 + 3 * -2, -1, 2)   # leading reference gap
+    assert O.nw_banded("", "ACGT", band=3)[0] == NW_NEG             # past the first row's band
+    assert O.nw_banded("ACG", "", band=5)[0] == NW_NEG              # no leading query gap
+
+
+def _nw_paths_best(q, r, band, match, mismatch, go, ge):
+    """Every banded path from the origin, the reference's cost model: a
+    start on row -1 at column c (value 0 for c = -1, go + (c + 1) ge for
+    0 <= c < band, i.e. a leading reference gap), then diagonal steps
+    (match / mismatch), down and right steps (go + ge each) through cells
+    0 <= ri < rlen, |ri - qi| <= band; no cell of column -1 below row -1."""
+    ql, rl = len(q), len(r)
+    if ql == 0:
+        if rl == 0:
+            return 0
+        return go + rl * ge if rl <= band else NW_NEG
+    best = None
+
+    @functools.lru_cache(maxsize=None)
+    def walk(qi, ri):
+        # best path value from cell (qi, ri) (already entered) to the end
+        if (qi, ri) == (ql - 1, rl - 1):
+            return 0
+        out = []
+        for dq, dr in ((1, 1), (1, 0), (0, 1)):
+            a, b = qi + dq, ri + dr
+            if a >= ql or b >= rl or b < 0 or abs(b - a) > band:
+                continue
+            step = (match if q[a] == r[b] else mismatch) if (dq, dr) == (1, 1) else go + ge
+            rest = walk(a, b)
+            if rest is not None:
+                out.append(step + rest)
+        return max(out) if out else None
+
+    for c in range(-1, band):
+        start = 0 if c == -1 else go + (c + 1) * ge
+        for dq, dr in ((1, 1), (1, 0)):
+            a, b = 0, c + dr
+            if b < 0 or b >= rl or abs(b - a) > band:
+                continue
+            step = (match if q[0] == r[b] else mismatch) if dr == 1 else go + ge
+            rest = walk(a, b)
+            if rest is not None:
+                v = start + step + rest
+                best = v if best is None else max(best, v)
+    return NW_NEG if best is None else best
+
+
+def test_nw_equals_path_brute_force():
+    rng = random.Random(77)
+    scorings = [(2, -3, -5, -2), (1, -1, -1, -1), (3, 1, -2, 0), (2, -4, 0, -3)]
+    for _ in range(2500):
+        q = rseq(rng, rng.randint(0, 6))
+        r = rseq(rng, rng.randint(0, 6))
+        band = rng.randint(0, 6)
+        sc = rng.choice(scorings)
+        exp = _nw_paths_best(q, r, band, *sc)
+        assert O.nw_banded(q, r, *sc, band=band)[0] == exp, (q, r, band, sc)
