@@ -50,6 +50,21 @@ constexpr int kPackedBand = 128;
 constexpr int kMixThreads = 256;
 constexpr int kMixGroup = kMixThreads;
 
+// Layout of the plan's small device buffer (int32 words).  Register-lane SW
+// tasks never straddle two effective bands: a wavefront that holds two bands
+// runs once per band, and a task key taken from its first pair would then
+// understate its work, which puts long tasks at the end of the queue.
+constexpr int kRegBands = 65;                // effective bands 0..64
+constexpr int kSmallCounts = 0;              // pairs per class [kMixClasses]
+constexpr int kSmallRanges = 8;              // [start, end) per class in order[]
+constexpr int kSmallNTasks = 24;             // tasks in the queue
+constexpr int kSmallQueue = 25;              // next task (atomic counter)
+constexpr int kSmallRegTasks = 26;           // register-lane SW tasks
+constexpr int kSmallBandCount = 32;          // register-lane pairs per band [kRegBands]
+constexpr int kSmallBandTask = 32 + 72;      // first task of each band, widest first [kRegBands+1]
+constexpr int kSmallBandPair = 32 + 2 * 72;  // first pair (in the class) of each band [kRegBands]
+constexpr int kSmallWords = 32 + 3 * 72;
+
 // Diagonals per thread and pass in WFA tasks: 4 would spill at the 256 VGPRs
 // the register SW layout leaves.
 constexpr int kMixWfaUnroll = 2;
@@ -79,7 +94,10 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
                                  const int32_t *band, int32_t band_scalar, const int32_t *me_arr,
                                  int32_t me_scalar, int32_t me_cap, SwScoring sc, WfaPen p,
                                  wfadev::WfaGeom gm, MixTuning tune, SwOut out, uint32_t *keys,
-                                 int32_t *idx, int32_t *counts) {
+                                 int32_t *idx, int32_t *counts, int32_t *band_counts) {
+    __shared__ int32_t hist[kRegBands];
+    for (int j = threadIdx.x; j < kRegBands; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool in = i < b.n_pairs;
     const uint64_t *q, *r;
@@ -110,6 +128,7 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
                     cls = kMixSwPlain;
                 }
                 payload = ((uint32_t)min(be, 2047) << 17) | (uint32_t)min(max(ql, rl), (1 << 17) - 1);
+                if (cls == kMixSwReg) atomicAdd(&hist[be], 1);
             }
         } else if (k == 1) {
             const int me = me_arr ? me_arr[i] : me_scalar;
@@ -129,6 +148,9 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
         const uint64_t m = __ballot(cls == c);
         if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&counts[c], __popcll(m));
     }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kRegBands; j += blockDim.x)
+        if (hist[j]) atomicAdd(&band_counts[j], hist[j]);
     if (!in) return;
     if (cls == kMixBad) {
         out.score[i] = CGM_BAD_PAIR;
@@ -139,39 +161,66 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
     idx[i] = (int32_t)i;
 }
 
-__global__ void mix_ranges_kernel(const int32_t *counts, int32_t *ranges, int32_t *n_tasks) {
+// Class ranges in the sorted order and the task counts.  Register-lane pairs
+// are sorted widest band first, so band b's pairs follow those of every
+// wider band; each band gets ceil(count / kMixGroup) tasks of its own.
+__global__ void mix_ranges_kernel(int32_t *small) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int32_t *counts = small + kSmallCounts;
+    int32_t *ranges = small + kSmallRanges;
     int32_t s = 0;
     for (int c = 0; c < kMixClasses; ++c) {
         ranges[2 * c] = s;
         s += counts[c];
         ranges[2 * c + 1] = s;
     }
-    *n_tasks = (counts[kMixSwReg] + kMixGroup - 1) / kMixGroup +
-               (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup + counts[kMixWfa] +
-               (counts[kMixWfaWave] + kMixWaves - 1) / kMixWaves;
+    int32_t tasks = 0, pairs = 0;
+    for (int bd = kRegBands - 1; bd >= 0; --bd) {
+        const int32_t c = small[kSmallBandCount + bd];
+        small[kSmallBandTask + bd] = tasks;
+        small[kSmallBandPair + bd] = pairs;
+        tasks += (c + kMixGroup - 1) / kMixGroup;
+        pairs += c;
+    }
+    small[kSmallBandTask + kRegBands] = tasks;
+    small[kSmallRegTasks] = tasks;
+    small[kSmallNTasks] = tasks + (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup +
+                          counts[kMixWfa] + (counts[kMixWfaWave] + kMixWaves - 1) / kMixWaves;
 }
 
-__global__ void mix_tasks_kernel(const int32_t *ranges, const int32_t *n_tasks,
-                                 const uint32_t *sorted_keys, int64_t cap, Task *tasks,
-                                 uint32_t *tkeys, int32_t *tidx) {
+__global__ void mix_tasks_kernel(const int32_t *small, const uint32_t *sorted_keys, int64_t cap,
+                                 Task *tasks, uint32_t *tkeys, int32_t *tidx) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= cap) return;
-    const int32_t c0 = ranges[2 * kMixSwReg + 1] - ranges[2 * kMixSwReg];
+    const int32_t *ranges = small + kSmallRanges;
     const int32_t c1 = ranges[2 * kMixSwPacked + 1] - ranges[2 * kMixSwPacked];
-    const int64_t n0 = (c0 + kMixGroup - 1) / kMixGroup, n1 = (c1 + kMixGroup - 1) / kMixGroup;
+    const int64_t n0 = small[kSmallRegTasks], n1 = (c1 + kMixGroup - 1) / kMixGroup;
     Task task = {0, 0, 0, 0};
     uint32_t key = 0;
-    if (t < *n_tasks) {
+    if (t < small[kSmallNTasks]) {
         uint32_t payload;
         if (t < n0 + n1) {
-            const bool packed = t >= n0;
-            const int64_t g = packed ? t - n0 : t;
-            const int32_t base = packed ? ranges[2 * kMixSwPacked] : ranges[2 * kMixSwReg];
-            const int32_t cnt = packed ? c1 : c0;
-            task.type = packed ? kTaskSwPacked : kTaskSwReg;
-            task.start = base + (int32_t)(kMixGroup * g);
-            task.count = (int32_t)min((int64_t)kMixGroup, cnt - kMixGroup * g);
+            if (t < n0) {
+                // band of task t: the last band (widest first) whose first task is <= t
+                int lo = 0, hi = kRegBands - 1;  // band task starts decrease with the band
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (small[kSmallBandTask + mid] <= t) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int bd = lo;
+                const int64_t g = t - small[kSmallBandTask + bd];
+                task.type = kTaskSwReg;
+                task.start = ranges[2 * kMixSwReg] + small[kSmallBandPair + bd] +
+                             (int32_t)(kMixGroup * g);
+                task.count = (int32_t)min((int64_t)kMixGroup,
+                                          small[kSmallBandCount + bd] - kMixGroup * g);
+            } else {
+                const int64_t g = t - n0;
+                task.type = kTaskSwPacked;
+                task.start = ranges[2 * kMixSwPacked] + (int32_t)(kMixGroup * g);
+                task.count = (int32_t)min((int64_t)kMixGroup, c1 - kMixGroup * g);
+            }
             payload = sorted_keys[task.start] & 0x0fffffffu;
             const uint32_t band = payload >> 17, len = payload & 0x1ffffu;
             // ~10 VALU ops x 4 cycles per wave-step of 64 cells, units of 16 cycles
@@ -297,7 +346,7 @@ struct MixPlan {
     // workspace
     uint32_t *keys = nullptr, *keys_out = nullptr, *tkeys = nullptr, *tkeys_out = nullptr;
     int32_t *idx = nullptr, *order = nullptr, *tidx = nullptr, *task_order = nullptr;
-    int32_t *small = nullptr;  // counts[8] | ranges[16] | n_tasks | queue
+    int32_t *small = nullptr;  // kSmallWords words, layout above
     Task *tasks = nullptr;
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -414,7 +463,7 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
     CGM_ALLOC(P->tidx, nt * 4);
     CGM_ALLOC(P->task_order, nt * 4);
     CGM_ALLOC(P->tasks, nt * sizeof(Task));
-    CGM_ALLOC(P->small, 64 * 4);
+    CGM_ALLOC(P->small, kSmallWords * 4);
     size_t b1 = 0, b2 = 0;
     if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b1, P->keys, P->keys_out, P->idx,
                                                      P->order, (int)n, 0, 32) != hipSuccess ||
@@ -516,22 +565,22 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     int dev = 0;
     CGM_HIP(hipGetDevice(&dev));
     if (dev != P->device) return set_error(CGM_ERR_INVALID, "plan belongs to another device");
-    int32_t *counts = P->small, *ranges = P->small + 8, *n_tasks = P->small + 24,
-            *queue = P->small + 25;
+    int32_t *counts = P->small + kSmallCounts, *ranges = P->small + kSmallRanges,
+            *n_tasks = P->small + kSmallNTasks, *queue = P->small + kSmallQueue;
     const int64_t n = b.n_pairs, nt = P->task_cap;
-    hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(64), 0, stream, P->small, 64);
+    hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(256), 0, stream, P->small, kSmallWords);
     const MixTuning tune = P->tune;
     hipLaunchKernelGGL(mix_class_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, b,
                        kind, kind_scalar, band, band_scalar, me, me_scalar, P->me_cap, sc, pen,
-                       P->gm, tune, out, P->keys, P->idx, counts);
-    hipLaunchKernelGGL(mix_ranges_kernel, dim3(1), dim3(64), 0, stream, counts, ranges, n_tasks);
+                       P->gm, tune, out, P->keys, P->idx, counts, P->small + kSmallBandCount);
+    hipLaunchKernelGGL(mix_ranges_kernel, dim3(1), dim3(64), 0, stream, P->small);
     size_t tb = P->sort_tmp_bytes;
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->keys, P->keys_out,
                                                          P->idx, P->order, (int)n, 0, 32, stream));
     const int64_t tasks_needed = n + (n + kMixGroup - 1) / kMixGroup + 1;
     const int64_t tn = tasks_needed < nt ? tasks_needed : nt;
     hipLaunchKernelGGL(mix_tasks_kernel, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, stream,
-                       ranges, n_tasks, P->keys_out, tn, P->tasks, P->tkeys, P->tidx);
+                       P->small, P->keys_out, tn, P->tasks, P->tkeys, P->tidx);
     tb = P->sort_tmp_bytes;
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->tkeys, P->tkeys_out,
                                                          P->tidx, P->task_order, (int)tn, 0, 32,
