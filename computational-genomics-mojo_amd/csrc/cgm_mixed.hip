@@ -273,7 +273,7 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
                       int32_t me_scalar, SwScoring sc, WfaPen p, wfadev::WfaGeom gm, SwOut out,
                       const int32_t *order, const Task *tasks, const int32_t *task_order,
                       const int32_t *n_tasks, int32_t *queue, int32_t lds_bytes,
-                      int32_t tile_steps, int32_t *slab, int64_t slab_words) {
+                      int32_t tile_steps, int32_t sw_dual, int32_t *slab, int64_t slab_words) {
     extern __shared__ uint8_t lds_raw[];
     __shared__ int next_task;
     int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
@@ -291,8 +291,8 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
             const int count = min(64, task.count - 64 * wave);
             if (count > 0) {
                 if (task.type == kTaskSwReg)
-                    swdev::sw_reg_group<0, 64, false>(b, band_arr, band_scalar, sc, out, order,
-                                                      start, count);
+                    swdev::sw_reg_group<0, 64, false, true>(b, band_arr, band_scalar, sc, out,
+                                                            order, start, count, sw_dual != 0);
                 else
                     swdev::sw_reg_group<kPackedBand, kPackedBand, true>(
                         b, band_arr, band_scalar, sc, out, order, start, count);
@@ -380,6 +380,7 @@ void tuning_to_array(const MixTuning &t, int32_t *a) {
     a[3] = t.wave_max_width;
     a[4] = t.reg_max_band;
     a[5] = t.lane_max_len;
+    a[6] = t.sw_dual;
 }
 
 int tuning_from_array(const int32_t *a, MixTuning *t) {
@@ -390,6 +391,7 @@ int tuning_from_array(const int32_t *a, MixTuning *t) {
     v.wave_max_width = a[3];
     v.reg_max_band = a[4];
     v.lane_max_len = a[5];
+    v.sw_dual = a[6];
     if (v.blocks_per_cu < 1 || v.blocks_per_cu > kMixMaxBlocksPerCu)
         return set_error(CGM_ERR_INVALID, "tuning: blocks_per_cu must be 1 or 2");
     // static LDS of the kernel (a few ints) needs room beside the dynamic part
@@ -401,6 +403,8 @@ int tuning_from_array(const int32_t *a, MixTuning *t) {
     if (v.reg_max_band < 0 || v.reg_max_band > 64)
         return set_error(CGM_ERR_INVALID, "tuning: reg_max_band out of range");
     if (v.lane_max_len < 1) return set_error(CGM_ERR_INVALID, "tuning: lane_max_len < 1");
+    if (v.sw_dual != 0 && v.sw_dual != 1)
+        return set_error(CGM_ERR_INVALID, "tuning: sw_dual must be 0 or 1");
     *t = v;
     return CGM_OK;
 }
@@ -592,7 +596,7 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                        dim3(grid), dim3(kMixThreads),
                        (size_t)tune.lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
                        P->gm, out, P->order, P->tasks, P->task_order, n_tasks, queue,
-                       tune.lds_bytes, tune.tile_steps, P->slab, P->slab_words);
+                       tune.lds_bytes, tune.tile_steps, tune.sw_dual, P->slab, P->slab_words);
     CGM_HIP(hipGetLastError());
     CGM_HIP(hipEventRecord(P->ev_stop, stream));
     // uncommon SW classes: LDS lanes, then one workgroup per pair
@@ -751,6 +755,11 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
         for (int x : v)
             if (x != cur.lane_max_len) (c[nc] = cur, c[nc++].lane_max_len = x);
         try_all(c, nc);
+    }
+    {   // register SW lanes: dual-row int16 or int32
+        MixTuning c = cur;
+        c.sw_dual = 1 - cur.sw_dual;
+        try_all(&c, 1);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -282,6 +282,182 @@ __device__ __forceinline__ LaneResult sw_reg_lane_packed(const uint64_t *q, int 
     return {best, bq, br};
 }
 
+// ------------------------------------------------ dual-row int16 lanes
+// The band frame of W = 2B + 1 cells is cut into a left half (j in [0, S))
+// and a right half (j in [S, 2S)), S = B + 1, and both halves are swept in
+// one pass over the row registers: register u holds cell u of the left half
+// in its low 16 bits and cell S + u of the right half in its high 16 bits,
+// the right half one row behind the left one.  Step t computes row t of the
+// left half and row t - 1 of the right half with packed int16 instructions
+// (v_pk_add_u16 / v_pk_max_i16: two cells per instruction).  The right
+// half's first cell reads its left neighbour (row t - 1, cell B) from the
+// left half's previous step; the left half's last cell reads its upper
+// neighbour (row t - 1, cell B + 1) from the right half's first cell of this
+// step.  Cell 2S - 1 = W lies past the frame: it is forced to H = 0 so that
+// cell W - 1 reads the zero upper neighbour the frame edge gives.
+//
+// Exactness: the per-cell recurrence, the forced mismatches outside the
+// matrix and the first-cell boundary values are those of sw_reg_lane, in
+// 16-bit lanes.  Every value stays in [-16384 + gap_extend, H_max + match]
+// with H_max = match * min(qlen, rlen), so dual_ok() admits a pair only when
+// that range fits int16; the E boundary value -16384 then lies below every
+// value it is compared with (gap_open >= -16384), as -1000000 does in the
+// reference.  Tie-break: per half, the first row whose maximum beats the
+// half's best (strict) is kept whole; its first cell holding the best gives
+// the column.  The two halves' candidates are then ordered by score, row,
+// and left half first, which is the reference's first-in-row-major-order.
+__device__ __forceinline__ bool dual_scoring_ok(const SwScoring &sc) {
+    return sc.match <= 16383 && sc.mismatch >= -8191 && sc.gap_open >= -8191 &&
+           sc.gap_extend >= -8191;
+}
+__device__ __forceinline__ bool dual_ok(const SwScoring &sc, int qlen, int rlen) {
+    return (int64_t)max(sc.match, 0) * min(qlen, rlen) + sc.match <= 32767;
+}
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s16x2 pk_max(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
+// v_pk_max_i16 as an opaque step (the running row maximum), for the same
+// reason as max3_chain.
+__device__ __forceinline__ s16x2 pk_max_chain(s16x2 a, s16x2 b) {
+    uint32_t d;
+    asm volatile("v_pk_max_i16 %0, %1, %2" : "=v"(d) : "v"(as_u32(a)), "v"(as_u32(b)));
+    return as_s16x2(d);
+}
+__device__ __forceinline__ uint32_t bit_mask_at(uint32_t half, int u) {
+    int v;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(v) : "v"(half), "s"(30 - 2 * (u & 15)));
+    return (uint32_t)v;
+}
+__device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t a, uint32_t b) {
+    return (a & m) | (b & ~m);
+}
+
+template <int B>
+__device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, const uint64_t *r,
+                                                   int rlen, int rows, const SwScoring &sc) {
+    constexpr int S = B + 1;
+    constexpr int NCH = (S + 31) / 32;
+    const int oe = sc.gap_open + sc.gap_extend;
+    const uint32_t OE = ((uint32_t)oe & 0xffffu) * 0x10001u;
+    const uint32_t EXT = ((uint32_t)sc.gap_extend & 0xffffu) * 0x10001u;
+    const uint32_t SUBM = ((uint32_t)(sc.match - oe) & 0xffffu) * 0x10001u;
+    const uint32_t SUBX = ((uint32_t)(sc.mismatch - oe) & 0xffffu) * 0x10001u;
+    constexpr uint32_t kNegE16 = 0xC000u;  // -16384
+    const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
+    s16x2 P[S], SV[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        P[u] = as_s16x2(OE);
+        SV[u] = as_s16x2(OE);
+    }
+    int best_lo = 0, row_lo = 0, best_hi = 0, row_hi = 0;
+    uint64_t qwin = 0;
+    int qb_prev = 0;
+    uint32_t el_carry = kNegE16;  // E of the left half's last cell, previous step
+    AlignedWords<NCH + 1> rwl, rwh;
+    rwl.load(r, (-B) >> 5, rwords);
+    rwh.load(r, 0, rwords);
+    uint32_t p0_hi = OE >> 16;  // right half's first cell, G of its latest row
+    for (int t = 0; t <= rows; ++t) {
+        const int rs_lo = t - B, rs_hi = t;  // reference base of each half's first cell
+        if (t > 0 && (rs_lo & 31) == 0) rwl.advance(r, rs_lo >> 5, rwords);
+        if (t > 0 && (rs_hi & 31) == 0) rwh.advance(r, rs_hi >> 5, rwords);
+        if ((t & 31) == 0) qwin = ((t >> 5) < qwords) ? q[t >> 5] : 0ull;
+        const int qb = (int)(qwin >> 62);
+        qwin <<= 2;
+        const uint64_t qrep_lo = (uint64_t)qb * 0x5555555555555555ull;
+        const uint64_t qrep_hi = (uint64_t)qb_prev * 0x5555555555555555ull;
+        qb_prev = qb;
+        const bool ok_lo = t < qlen, ok_hi = t >= 1 && t - 1 < qlen;
+        const int sh_lo = 2 * (rs_lo & 31), sh_hi = 2 * (rs_hi & 31);
+        // left neighbours of the two first cells: H = 0 and the E boundary
+        // value for the left half; cell B of row t - 1 for the right half
+        s16x2 Gl = as_s16x2((as_u32(P[S - 1]) << 16) | (OE & 0xffffu));
+        s16x2 El = as_s16x2((el_carry << 16) | kNegE16);
+        s16x2 rowmax = as_s16x2(0u);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const uint64_t ml =
+                chunk_mask(rwl.window(c, sh_lo), qrep_lo, rs_lo + 32 * c, rlen, ok_lo);
+            const uint64_t mh =
+                chunk_mask(rwh.window(c, sh_hi), qrep_hi, rs_hi + 32 * c, rlen, ok_hi);
+            const uint32_t ml1 = (uint32_t)(ml >> 32), ml0 = (uint32_t)ml;
+            const uint32_t mh1 = (uint32_t)(mh >> 32), mh0 = (uint32_t)mh;
+#pragma unroll
+            for (int v = 0; v < 32; ++v) {
+                const int u = 32 * c + v;
+                if (u < S) {
+                    const uint32_t bl = bit_mask_at(v < 16 ? ml1 : ml0, v);
+                    const uint32_t bh = bit_mask_at(v < 16 ? mh1 : mh0, v);
+                    const uint32_t sel = bit_select(bit_select(0xffff0000u, bh, bl), SUBM, SUBX);
+                    // upper neighbour: register u + 1 of the previous step;
+                    // for the last cell, the right half's first cell of this
+                    // step (low) and H = 0 past the frame (high)
+                    const s16x2 Gu = (u + 1 < S) ? P[u + 1]
+                                                 : as_s16x2((OE & 0xffff0000u) | p0_hi);
+                    const s16x2 dterm = P[u] + as_s16x2(sel);
+                    const s16x2 e = pk_max(El + as_s16x2(EXT), Gu);
+                    s16x2 h = pk_max(pk_max(dterm, as_s16x2(0u)), pk_max(e, Gl));
+                    if (u == S - 1) h = as_s16x2(as_u32(h) & 0xffffu);
+                    rowmax = pk_max_chain(rowmax, h);
+                    const s16x2 pn = h + as_s16x2(OE);
+                    P[u] = pn;
+                    Gl = pn;
+                    El = e;
+                    if (u == 0) p0_hi = as_u32(pn) >> 16;
+                }
+            }
+        }
+        el_carry = as_u32(El) & 0xffffu;
+        const int rm_lo = (int)(short)(as_u32(rowmax) & 0xffffu);
+        const int rm_hi = (int)(short)(as_u32(rowmax) >> 16);
+        const bool up_lo = rm_lo > best_lo, up_hi = rm_hi > best_hi;
+        if (__ballot(up_lo || up_hi)) {
+            const uint32_t m = (up_lo ? 0xffffu : 0u) | (up_hi ? 0xffff0000u : 0u);
+#pragma unroll
+            for (int u = 0; u < S; ++u) SV[u] = as_s16x2(bit_select(m, as_u32(P[u]), as_u32(SV[u])));
+            if (up_lo) {
+                best_lo = rm_lo;
+                row_lo = t;
+            }
+            if (up_hi) {
+                best_hi = rm_hi;
+                row_hi = t - 1;
+            }
+        }
+    }
+    // first cell of each half's kept row that holds the half's best
+    int ulo = 0, uhi = 0;
+#pragma unroll
+    for (int u = S - 1; u >= 0; --u) {
+        const uint32_t v = as_u32(SV[u]);
+        if ((int)(short)(v & 0xffffu) - oe == best_lo) ulo = u;
+        if ((int)(short)(v >> 16) - oe == best_hi) uhi = u;
+    }
+    LaneResult lo = {best_lo, row_lo, row_lo - B + ulo};
+    LaneResult hi = {best_hi, row_hi, row_hi + 1 + uhi};  // ri = row - B + S + u
+    if (best_lo == 0) lo = {0, 0, 0};
+    if (best_hi == 0) hi = {0, 0, 0};
+    if (hi.score > lo.score || (hi.score == lo.score && hi.score > 0 && hi.q_end < lo.q_end))
+        return hi;
+    return lo;
+}
+
+template <int LO, int HI>
+__device__ __forceinline__ LaneResult sw_dual_dispatch(int band, const uint64_t *q, int qlen,
+                                                       const uint64_t *r, int rlen, int rows,
+                                                       const SwScoring &sc) {
+    if constexpr (LO == HI) {
+        return sw_dual_lane<LO>(q, qlen, r, rlen, rows, sc);
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        if (band <= MID) return sw_dual_dispatch<LO, MID>(band, q, qlen, r, rlen, rows, sc);
+        return sw_dual_dispatch<MID + 1, HI>(band, q, qlen, r, rlen, rows, sc);
+    }
+}
+
 template <int LO, int HI>
 __device__ __forceinline__ LaneResult sw_reg_dispatch(int band, const uint64_t *q, int qlen,
                                                       const uint64_t *r, int rlen, int rows,
@@ -298,12 +474,14 @@ __device__ __forceinline__ LaneResult sw_reg_dispatch(int band, const uint64_t *
 // One wavefront aligns order[start + lane] for lane < count with the
 // register layout: lanes are served once per distinct band among them
 // (pairs arrive sorted by band, so normally once).  PACKED selects the
-// packed-int16 instantiation for band 128, otherwise bands LO..HI.
-template <int LO, int HI, bool PACKED>
+// packed-int16 instantiation for band 128, otherwise bands LO..HI.  DUAL
+// compiles the dual-row int16 lanes in as well; they serve a pass when
+// `dual` is set and every lane of the pass fits them (dual_ok).
+template <int LO, int HI, bool PACKED, bool DUAL = false>
 __device__ __forceinline__ void sw_reg_group(const Batch &b, const int32_t *band_arr,
                                              int32_t band_scalar, const SwScoring &sc,
                                              const SwOut &out, const int32_t *order, int64_t start,
-                                             int count) {
+                                             int count, bool dual = false) {
     const int lane = threadIdx.x & 63;
     const bool active = lane < count;
     const int64_t pair = active ? order[start + lane] : 0;
@@ -324,6 +502,15 @@ __device__ __forceinline__ void sw_reg_group(const Batch &b, const int32_t *band
         LaneResult res;
         if constexpr (PACKED) {
             res = sw_reg_lane_packed<HI>(q, mine ? qlen : 0, r, mine ? rlen : 0, rows, sc);
+        } else if constexpr (DUAL) {
+            const bool use_dual =
+                dual && dual_scoring_ok(sc) && __all(!mine || dual_ok(sc, qlen, rlen));
+            if (use_dual)
+                res = sw_dual_dispatch<LO, HI>(bsel, q, mine ? qlen : 0, r, mine ? rlen : 0, rows,
+                                               sc);
+            else
+                res = sw_reg_dispatch<LO, HI>(bsel, q, mine ? qlen : 0, r, mine ? rlen : 0, rows,
+                                              sc);
         } else {
             res = sw_reg_dispatch<LO, HI>(bsel, q, mine ? qlen : 0, r, mine ? rlen : 0, rows, sc);
         }

@@ -243,6 +243,11 @@ int cgm_plan_kernel_ms(void *plan, float *ms);
  *                   lanes, wider ones in LDS lanes (band 128: packed lanes)
  *   LANE_MAX_LEN    switch-over: SW pairs longer than this run one workgroup
  *                   per pair instead of one lane per pair
+ *   SW_DUAL         1: register SW lanes hold the band frame's two halves,
+ *                   one row apart, in the two 16-bit halves of each register
+ *                   (packed int16 arithmetic) wherever every value fits int16
+ *                   (match * min(qlen, rlen) + match <= 32767, penalties
+ *                   >= -8191); 0: int32 lanes only
  * cgm_plan_autotune times the whole pipeline on the given batch for
  * candidate values of each field in turn (coordinate descent, best of
  * `reps` runs per candidate), keeps a change only if it is >= 2 % faster,
@@ -256,7 +261,8 @@ int cgm_plan_kernel_ms(void *plan, float *ms);
 #define CGM_TUNE_WAVE_MAX_WIDTH 3
 #define CGM_TUNE_REG_MAX_BAND 4
 #define CGM_TUNE_LANE_MAX_LEN 5
-#define CGM_TUNE_N 6
+#define CGM_TUNE_SW_DUAL 6
+#define CGM_TUNE_N 7
 int cgm_plan_get_tuning(void *plan, int32_t *params);
 int cgm_plan_set_tuning(void *plan, const int32_t *params);
 int cgm_tuning_get_default(int32_t *params);

@@ -112,6 +112,8 @@ TUNINGS = [
     {"blocks_per_cu": 1, "lds_bytes": 159 * 1024, "tile_steps": 64},
     {"reg_max_band": 32, "wave_max_width": 512},
     {"lane_max_len": 1024, "tile_steps": 16, "lds_bytes": 24 * 1024},
+    {"sw_dual": 0},
+    {"sw_dual": 1, "blocks_per_cu": 1, "lds_bytes": 96 * 1024},
 ]
 
 

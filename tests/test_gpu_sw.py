@@ -123,3 +123,56 @@ def test_sw_is_deterministic(cgm, dev, gen_mod):
     a = sw_gpu(cgm, dev, up, LAYOUTS["auto"])
     b = sw_gpu(cgm, dev, up, LAYOUTS["auto"])
     assert_same(a, b, "run twice")
+
+
+def plan_sw(cgm, dev, up, plan, scoring):
+    cfg = cgm.AlignConfig(scoring["match"], scoring["mismatch"], scoring["gap_open"],
+                          scoring["gap_extend"], 0)
+    out = cgm.SwResult(up.n)
+    plan.align(dev, (up.q, up.r, up.ql, up.rl, up.qo, up.ro), up.n, None, up.band, None, cfg, out)
+    dev.synchronize()
+    return tuple(dev.download(a, up.n) for a in (out.scores, out.query_end, out.ref_end))
+
+
+@pytest.mark.parametrize("dual", [1, 0])
+def test_sw_register_lane_forms_equal_oracle(cgm, dev, orc, gen_mod, dual):
+    """The int32 and the dual-row int16 register lanes (tuning sw_dual) on
+    random pairs over several scorings and every band 0-64."""
+    rng = np.random.default_rng(31)
+    for k, sc in enumerate((CONFIG_SW, DEFAULT_SW, dict(match=1, mismatch=-1, gap_open=-1,
+                                                        gap_extend=-1),
+                            dict(match=3, mismatch=0, gap_open=-2, gap_extend=0),
+                            dict(match=5, mismatch=-4, gap_open=0, gap_extend=0))):
+        w = gen_mod.generate(2, 3000, seed=40 + k)
+        w.band[:] = rng.integers(0, 65, w.n_pairs)
+        up = Uploaded(dev, w)
+        plan = cgm.MixedPlan(w.n_pairs, 0)
+        plan.set_tuning(sw_dual=dual)
+        print(f"[case] register lanes dual={dual} scoring {sc}", flush=True)
+        assert_same(plan_sw(cgm, dev, up, plan, sc), sw_oracle(orc, w, sc), f"dual={dual} {sc}")
+        plan.close()
+
+
+def test_sw_dual_lanes_int16_boundary(cgm, dev, orc):
+    """match * min(qlen, rlen) + match at 32766 (dual-row lanes) and 32768
+    (int32 lanes), alone and together in one wavefront (which then runs the
+    int32 lanes); every result equals the oracle and the identical pairs
+    score 2 * length."""
+    rng = np.random.default_rng(8)
+    a = "".join(rng.choice(list("ACGT"), 16383))
+    mut = list(a)
+    for p in rng.choice(len(mut), 40, replace=False):
+        mut[p] = "ACGT"[("ACGT".index(mut[p]) + 1) % 4]
+    mut = "".join(mut)
+    pairs = [(a[:16382], a[:16382]), (a, a), (a[:16382], mut[:16382]), (mut, a),
+             (a[:300], a[:300])]
+    w = workload_from_strings(pairs, band=[8, 9, 12, 12, 64])
+    up = Uploaded(dev, w)
+    plan = cgm.MixedPlan(w.n_pairs, 0)
+    plan.set_tuning(sw_dual=1)
+    print("[case] dual int16 boundary", flush=True)
+    got = plan_sw(cgm, dev, up, plan, CONFIG_SW)
+    assert_same(got, sw_oracle(orc, w, CONFIG_SW), "int16 boundary")
+    scores, q_end = got[0], got[1]
+    assert scores[0] == 32764 and q_end[0] == 16381 and scores[1] == 32766
+    plan.close()

@@ -60,3 +60,94 @@ def test_register_schedule_equals_oracle():
         sc = rng.choice(scorings)
         assert sw_register_schedule(q, r, band, *sc) == O.sw_banded(q, r, *sc, band=band), \
             (q, r, band, sc)
+
+
+def _w16(x):
+    """int16 wrap-around (v_pk_add_u16 on two's-complement halves)."""
+    x &= 0xFFFF
+    return x - 0x10000 if x & 0x8000 else x
+
+
+def sw_dual_schedule(q, r, band, match, mismatch, gap_open, gap_extend, extra_rows=0):
+    """The dual-row int16 lane (csrc/cgm_sw_dev.h sw_dual_lane): halves
+    [0, S) and [S, 2S) of the band frame, S = band + 1, the right half one
+    row behind, every value in 16-bit lanes, cell 2S - 1 (past the frame)
+    forced to H = 0, each half's first best row kept and searched for its
+    first best cell.  `extra_rows` adds rows past the query, as a lane whose
+    wavefront runs longer queries does."""
+    ql, rl = len(q), len(r)
+    B = min(band, max(ql, rl))
+    S = B + 1
+    oe = gap_open + gap_extend
+    NEG16 = -16384
+    P = [[oe, oe] for _ in range(S)]  # [low half, high half]
+    SV = [[oe, oe] for _ in range(S)]
+    best = [0, 0]
+    row = [0, 0]
+    el_carry = NEG16
+    p0_hi = oe
+    rows = ql + extra_rows
+    for t in range(rows + 1):
+        rs = (t - B, t)
+        ok = (t < ql, 1 <= t and t - 1 < ql)
+        qb = (q[t] if t < ql else None, q[t - 1] if 1 <= t <= ql else None)
+        Gl = [oe, P[S - 1][0]]
+        El = [NEG16, el_carry]
+        rowmax = [0, 0]
+        for u in range(S):
+            newP = [0, 0]
+            for k in (0, 1):
+                ri = rs[k] + u
+                eq = ok[k] and 0 <= ri < rl and r[ri] == qb[k]
+                if u + 1 < S:
+                    Gu = P[u + 1][k]
+                else:
+                    Gu = p0_hi if k == 0 else oe
+                dterm = _w16(P[u][k] + _w16((match if eq else mismatch) - oe))
+                e = max(_w16(El[k] + gap_extend), Gu)
+                h = max(max(dterm, 0), max(e, Gl[k]))
+                if u == S - 1 and k == 1:
+                    h = 0
+                rowmax[k] = max(rowmax[k], h)
+                newP[k] = _w16(h + oe)
+                Gl[k] = newP[k]
+                El[k] = e
+            P[u] = newP
+            if u == 0:
+                p0_hi = newP[1]
+        el_carry = El[0]
+        for k in (0, 1):
+            if rowmax[k] > best[k]:
+                best[k] = rowmax[k]
+                row[k] = t - k
+                for u in range(S):
+                    SV[u][k] = P[u][k]
+    cand = []
+    for k in (0, 1):
+        if best[k] == 0:
+            cand.append((0, 0, 0))
+            continue
+        u = next(u for u in range(S) if SV[u][k] - oe == best[k])
+        cand.append((best[k], row[k], row[k] - B + k * S + u))
+    lo, hi = cand
+    if hi[0] > lo[0] or (hi[0] == lo[0] and hi[0] > 0 and hi[1] < lo[1]):
+        return hi
+    return lo
+
+
+def test_dual_schedule_equals_oracle():
+    rng = random.Random(5)
+    scorings = [(2, -3, -5, -2), (2, -4, -4, -2), (1, -1, -1, -1), (3, 0, -2, 0), (2, -1, 0, -1),
+                (5, -4, 0, 0), (1, 0, 0, 0)]
+    for _ in range(5000):
+        q = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 40)))
+        if rng.random() < 0.5:
+            r = "".join(c if rng.random() > 0.1 else rng.choice("ACGT") for c in q
+                        if rng.random() > 0.05)
+        else:
+            r = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 40)))
+        band = rng.randint(0, 70)
+        sc = rng.choice(scorings)
+        got = sw_dual_schedule(q, r, band, *sc, extra_rows=rng.choice([0, 0, 1, 7]))
+        want = O.sw_banded(q, r, *sc, band=band)
+        assert got == want, (q, r, band, sc, got, want)
