@@ -318,17 +318,23 @@ typedef short s16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
 __device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ s16x2 pk_max(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
-// v_pk_max_i16 as an opaque step (the running row maximum), for the same
-// reason as max3_chain.
-__device__ __forceinline__ s16x2 pk_max_chain(s16x2 a, s16x2 b) {
-    uint32_t d;
-    asm volatile("v_pk_max_i16 %0, %1, %2" : "=v"(d) : "v"(as_u32(a)), "v"(as_u32(b)));
+// Running row maximum, one v_pk_max_i16 per cell kept in sequence (for the
+// same reason as max3_chain) by an empty asm on its result.  Unlike an asm
+// instruction this leaves the hazard recognizer free to schedule around it:
+// on gfx950 a packed instruction that reads the result of the packed
+// instruction just before it needs a wait state, and an asm instruction is
+// always assumed to.
+__device__ __forceinline__ s16x2 pk_max_seq(s16x2 a, s16x2 b) {
+    uint32_t d = as_u32(__builtin_elementwise_max(a, b));
+    asm("" : "+v"(d));
     return as_s16x2(d);
 }
-__device__ __forceinline__ uint32_t bit_mask_at(uint32_t half, int u) {
-    int v;
-    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(v) : "v"(half), "s"(30 - 2 * (u & 15)));
-    return (uint32_t)v;
+// Sign-extended bit (30 - 2 * (u % 16)) of a match-mask half (0 / -1).  The
+// bit offset is taken from `base30` (30, unknown to the compiler), so the
+// extraction stays one v_bfe_i32 instead of being turned into a compare and
+// select.
+__device__ __forceinline__ uint32_t bit_mask_at(uint32_t half, int u, int base30) {
+    return (uint32_t)__builtin_amdgcn_sbfe((int)half, base30 - 2 * (u & 15), 1);
 }
 __device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t a, uint32_t b) {
     return (a & m) | (b & ~m);
@@ -346,6 +352,8 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
     const uint32_t SUBX = ((uint32_t)(sc.mismatch - oe) & 0xffffu) * 0x10001u;
     constexpr uint32_t kNegE16 = 0xC000u;  // -16384
     const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
+    int base30;
+    asm volatile("s_mov_b32 %0, 30" : "=s"(base30));
     s16x2 P[S], SV[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) {
@@ -374,6 +382,8 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
         const int sh_lo = 2 * (rs_lo & 31), sh_hi = 2 * (rs_hi & 31);
         // left neighbours of the two first cells: H = 0 and the E boundary
         // value for the left half; cell B of row t - 1 for the right half
+        // left neighbours of the two first cells: H = 0 and the E boundary
+        // value for the left half; cell B of row t - 1 for the right half
         s16x2 Gl = as_s16x2((as_u32(P[S - 1]) << 16) | (OE & 0xffffu));
         s16x2 El = as_s16x2((el_carry << 16) | kNegE16);
         s16x2 rowmax = as_s16x2(0u);
@@ -389,8 +399,8 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
             for (int v = 0; v < 32; ++v) {
                 const int u = 32 * c + v;
                 if (u < S) {
-                    const uint32_t bl = bit_mask_at(v < 16 ? ml1 : ml0, v);
-                    const uint32_t bh = bit_mask_at(v < 16 ? mh1 : mh0, v);
+                    const uint32_t bl = bit_mask_at(v < 16 ? ml1 : ml0, v, base30);
+                    const uint32_t bh = bit_mask_at(v < 16 ? mh1 : mh0, v, base30);
                     const uint32_t sel = bit_select(bit_select(0xffff0000u, bh, bl), SUBM, SUBX);
                     // upper neighbour: register u + 1 of the previous step;
                     // for the last cell, the right half's first cell of this
@@ -399,9 +409,9 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
                                                  : as_s16x2((OE & 0xffff0000u) | p0_hi);
                     const s16x2 dterm = P[u] + as_s16x2(sel);
                     const s16x2 e = pk_max(El + as_s16x2(EXT), Gu);
-                    s16x2 h = pk_max(pk_max(dterm, as_s16x2(0u)), pk_max(e, Gl));
+                    s16x2 h = pk_max(pk_max(pk_max(dterm, as_s16x2(0u)), e), Gl);
                     if (u == S - 1) h = as_s16x2(as_u32(h) & 0xffffu);
-                    rowmax = pk_max_chain(rowmax, h);
+                    rowmax = pk_max_seq(rowmax, h);
                     const s16x2 pn = h + as_s16x2(OE);
                     P[u] = pn;
                     Gl = pn;
