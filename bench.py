@@ -356,7 +356,7 @@ def main():
         # autotuned layout (untimed): workgroup shape, band split, switch-over
         t = time.perf_counter()
         tuning, best_ms, start_ms = plan.autotune(dev, bufs, w.n_pairs, None, run.band, None,
-                                                  run.cfg, run.out, reps=3)
+                                                  run.cfg, run.out, reps=5)
         autotune = {"tuning": tuning, "start_ms": start_ms, "best_ms": best_ms,
                     "seconds": time.perf_counter() - t}
         log(f"[bench] autotune: {autotune}")

@@ -279,6 +279,31 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
     int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
     const int total = *n_tasks;
     const int wave = threadIdx.x >> 6;
+    if constexpr (!WITH_WFA) {
+        // SW only: every wavefront takes the next 64-pair unit of the
+        // longest-first task list by itself (unit u = part u % 4 of task
+        // u / 4), so no wavefront waits for the others of its workgroup.
+        const int lane = threadIdx.x & 63;
+        const int64_t units = (int64_t)total * kMixWaves;
+        for (;;) {
+            int u = 0;
+            if (lane == 0) u = atomicAdd(queue, 1);
+            u = __shfl(u, 0, 64);
+            if ((int64_t)u >= units) break;
+            const Task task = tasks[task_order[u / kMixWaves]];
+            const int part = u % kMixWaves;
+            const int64_t start = task.start + 64 * part;
+            const int count = min(64, task.count - 64 * part);
+            if (count <= 0) continue;
+            if (task.type == kTaskSwReg)
+                swdev::sw_reg_group<0, 64, false, true>(b, band_arr, band_scalar, sc, out, order,
+                                                        start, count, sw_dual != 0);
+            else
+                swdev::sw_reg_group<kPackedBand, kPackedBand, true>(b, band_arr, band_scalar, sc,
+                                                                    out, order, start, count);
+        }
+        return;
+    }
     for (;;) {
         if (threadIdx.x == 0) next_task = atomicAdd(queue, 1);
         __syncthreads();
@@ -676,7 +701,10 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
     // best of `reps` timed enqueues of the whole pipeline under tuning t
     auto measure = [&](const MixTuning &t, float *ms) -> int {
         P->tune = t;
-        float lo = 3.4e38f;
+        // median of the runs: two workgroups per CU vary from run to run
+        // (which of two wavefronts sharing a SIMD issues first), and the best
+        // run alone would favour the shape with the widest spread
+        float runs[100];
         for (int i = 0; i < reps; ++i) {
             CGM_HIP(hipEventRecord(e0, stream));
             if (int rc = mix_enqueue(P, b, kind, kind_scalar, band, band_scalar, me, me_scalar, sc,
@@ -686,9 +714,10 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
             CGM_HIP(hipEventSynchronize(e1));
             float x = 0;
             CGM_HIP(hipEventElapsedTime(&x, e0, e1));
-            lo = std::min(lo, x);
+            runs[i] = x;
         }
-        *ms = lo;
+        std::sort(runs, runs + reps);
+        *ms = runs[reps / 2];
         return CGM_OK;
     };
     int rc = CGM_OK;

@@ -42,13 +42,13 @@ __device__ __forceinline__ uint64_t window_signed(const uint64_t *s, int pos, in
     return (lo << sh) | (hi >> (64 - sh));
 }
 
-// Even-bit mask (base u at bit 62 - 2u) of window bases u in [lo, hi).
+// Even-bit mask (base u at bit 62 - 2u) of window bases u in [lo, hi),
+// without branches: bases >= lo AND bases < hi (empty when lo >= hi).
 __device__ __forceinline__ uint64_t base_range_mask(int lo, int hi) {
     lo = min(max(lo, 0), 32);
     hi = min(max(hi, 0), 32);
-    if (lo >= hi) return 0ull;
-    const uint64_t m1 = ~0ull >> (2 * lo);
-    const uint64_t m2 = hi == 32 ? ~0ull : (~0ull << (64 - 2 * hi));
+    const uint64_t m1 = lo == 32 ? 0ull : (~0ull >> (2 * lo));
+    const uint64_t m2 = hi == 0 ? 0ull : (~0ull << (64 - 2 * hi));
     return m1 & m2 & 0x5555555555555555ull;
 }
 

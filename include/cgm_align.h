@@ -249,7 +249,7 @@ int cgm_plan_kernel_ms(void *plan, float *ms);
  *                   (match * min(qlen, rlen) + match <= 32767, penalties
  *                   >= -8191); 0: int32 lanes only
  * cgm_plan_autotune times the whole pipeline on the given batch for
- * candidate values of each field in turn (coordinate descent, best of
+ * candidate values of each field in turn (coordinate descent, median of
  * `reps` runs per candidate), keeps a change only if it is >= 2 % faster,
  * leaves the best tuning in the plan and returns it with its time and the
  * starting tuning's time; the outputs hold the batch's results.  New plans
