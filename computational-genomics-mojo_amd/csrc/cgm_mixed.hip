@@ -63,7 +63,10 @@ constexpr int kSmallRegTasks = 26;           // register-lane SW tasks
 constexpr int kSmallBandCount = 32;          // register-lane pairs per band [kRegBands]
 constexpr int kSmallBandTask = 32 + 72;      // first task of each band, widest first [kRegBands+1]
 constexpr int kSmallBandPair = 32 + 2 * 72;  // first pair (in the class) of each band [kRegBands]
-constexpr int kSmallWords = 32 + 3 * 72;
+constexpr int kSmallUnits = 28;              // SW-only kernel: two-ended unit counter (64 bit)
+constexpr int kSmallSimdSlots = 256;         // wavefronts started per SIMD [kSimdKeys]
+constexpr int kSimdKeys = 8 << 10;           // XCC (3 bits), CU bits of HW_ID (8), SIMD (2)
+constexpr int kSmallWords = kSmallSimdSlots + kSimdKeys;
 
 // Diagonals per thread and pass in WFA tasks: 4 would spill at the 256 VGPRs
 // the register SW layout leaves.
@@ -280,16 +283,37 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
     const int total = *n_tasks;
     const int wave = threadIdx.x >> 6;
     if constexpr (!WITH_WFA) {
-        // SW only: every wavefront takes the next 64-pair unit of the
-        // longest-first task list by itself (unit u = part u % 4 of task
-        // u / 4), so no wavefront waits for the others of its workgroup.
+        // SW only: every wavefront takes 64-pair units of the longest-first
+        // task list by itself (unit u = part u % 4 of task u / 4), so no
+        // wavefront waits for the others of its workgroup.  Of two
+        // wavefronts on one SIMD the older one gets most issue slots, so the
+        // first wavefront started on a SIMD takes units from the long end of
+        // the list and the second from the short end: the longest units are
+        // never left to a starved wavefront at the end of the launch.  The
+        // two ends meet once every unit is taken (one 64-bit counter:
+        // long-end claims in the low half, short-end claims in the high
+        // half).
         const int lane = threadIdx.x & 63;
-        const int64_t units = (int64_t)total * kMixWaves;
+        const uint32_t units = (uint32_t)total * kMixWaves;
+        int32_t *small = queue - kSmallQueue;
+        unsigned long long *ends = (unsigned long long *)(small + kSmallUnits);
+        int slot = 0;
+        if (lane == 0) {
+            const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_ID
+            const uint32_t xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20);  // XCC_ID
+            const uint32_t key = ((xcc & 7u) << 10) | (((hw >> 8) & 0xffu) << 2) | ((hw >> 4) & 3u);
+            slot = atomicAdd(small + kSmallSimdSlots + key, 1);
+        }
+        const bool short_end = (__shfl(slot, 0, 64) & 1) != 0;
         for (;;) {
-            int u = 0;
-            if (lane == 0) u = atomicAdd(queue, 1);
+            int u = -1;
+            if (lane == 0) {
+                const unsigned long long old = atomicAdd(ends, short_end ? (1ull << 32) : 1ull);
+                const uint32_t h = (uint32_t)old, tl = (uint32_t)(old >> 32);
+                if ((uint64_t)h + tl < units) u = short_end ? (int)(units - 1 - tl) : (int)h;
+            }
             u = __shfl(u, 0, 64);
-            if ((int64_t)u >= units) break;
+            if (u < 0) break;
             const Task task = tasks[task_order[u / kMixWaves]];
             const int part = u % kMixWaves;
             const int64_t start = task.start + 64 * part;
