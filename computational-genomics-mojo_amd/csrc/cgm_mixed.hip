@@ -93,75 +93,90 @@ __device__ __forceinline__ uint32_t wfa_work_estimate(int me, int qlen, int rlen
     return (uint32_t)min(est, (int64_t)((1 << 28) - 1));
 }
 
+// Classification, one pair per thread in a grid-stride loop over a grid of
+// at most one workgroup per CU: class counts and register-band counts are
+// gathered in LDS and added to the global counters once per workgroup (a
+// global atomic per wavefront and class serialised on a handful of
+// addresses and dominated this kernel).
 __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scalar,
                                  const int32_t *band, int32_t band_scalar, const int32_t *me_arr,
                                  int32_t me_scalar, int32_t me_cap, SwScoring sc, WfaPen p,
                                  wfadev::WfaGeom gm, MixTuning tune, SwOut out, uint32_t *keys,
                                  int32_t *idx, int32_t *counts, int32_t *band_counts) {
     __shared__ int32_t hist[kRegBands];
+    __shared__ int32_t cls_count[kMixClasses];
     for (int j = threadIdx.x; j < kRegBands; j += blockDim.x) hist[j] = 0;
+    if (threadIdx.x < kMixClasses) cls_count[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool in = i < b.n_pairs;
-    const uint64_t *q, *r;
-    int ql = 0, rl = 0;
-    int cls = in ? kMixBad : -1;
-    uint32_t payload = 0;
-    if (in && load_pair(b, i, q, r, ql, rl)) {
-        const int k = kind ? kind[i] : kind_scalar;
-        if (k == 0) {
-            const int bd = band ? band[i] : band_scalar;
-            if (bd >= 0) {
-                const int be = effective_band(bd, ql, rl);
-                const int64_t hmax = (int64_t)max(sc.match, 0) * min(ql, rl);
-                const int oe = sc.gap_open + sc.gap_extend;
-                const bool reg_scoring = sc.mismatch <= 0 && sc.gap_extend <= 0 && oe <= 0;
-                const bool long_pair = max(ql, rl) > tune.lane_max_len;
-                if (hmax > kMaxScoreProduct) {
-                    cls = kMixBad;
-                } else if (long_pair && be <= kPlainMaxBand) {
-                    cls = kMixSwPlain;
-                } else if (reg_scoring && be <= tune.reg_max_band && hmax < (1 << 23)) {
-                    cls = kMixSwReg;
-                } else if (reg_scoring && be == kPackedBand && hmax + oe <= 32767 && oe >= -32768) {
-                    cls = kMixSwPacked;
-                } else if (be <= kLdsLaneMaxBand) {
-                    cls = kMixSwLds;
-                } else if (be <= kPlainMaxBand) {
-                    cls = kMixSwPlain;
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n_round = (b.n_pairs + stride - 1) / stride * stride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_round; i += stride) {
+        const bool in = i < b.n_pairs;
+        const uint64_t *q, *r;
+        int ql = 0, rl = 0;
+        int cls = in ? kMixBad : -1;
+        uint32_t payload = 0;
+        if (in && load_pair(b, i, q, r, ql, rl)) {
+            const int k = kind ? kind[i] : kind_scalar;
+            if (k == 0) {
+                const int bd = band ? band[i] : band_scalar;
+                if (bd >= 0) {
+                    const int be = effective_band(bd, ql, rl);
+                    const int64_t hmax = (int64_t)max(sc.match, 0) * min(ql, rl);
+                    const int oe = sc.gap_open + sc.gap_extend;
+                    const bool reg_scoring = sc.mismatch <= 0 && sc.gap_extend <= 0 && oe <= 0;
+                    const bool long_pair = max(ql, rl) > tune.lane_max_len;
+                    if (hmax > kMaxScoreProduct) {
+                        cls = kMixBad;
+                    } else if (long_pair && be <= kPlainMaxBand) {
+                        cls = kMixSwPlain;
+                    } else if (reg_scoring && be <= tune.reg_max_band && hmax < (1 << 23)) {
+                        cls = kMixSwReg;
+                    } else if (reg_scoring && be == kPackedBand && hmax + oe <= 32767 &&
+                               oe >= -32768) {
+                        cls = kMixSwPacked;
+                    } else if (be <= kLdsLaneMaxBand) {
+                        cls = kMixSwLds;
+                    } else if (be <= kPlainMaxBand) {
+                        cls = kMixSwPlain;
+                    }
+                    payload = ((uint32_t)min(be, 2047) << 17) |
+                              (uint32_t)min(max(ql, rl), (1 << 17) - 1);
+                    if (cls == kMixSwReg) atomicAdd(&hist[be], 1);
                 }
-                payload = ((uint32_t)min(be, 2047) << 17) | (uint32_t)min(max(ql, rl), (1 << 17) - 1);
-                if (cls == kMixSwReg) atomicAdd(&hist[be], 1);
-            }
-        } else if (k == 1) {
-            const int me = me_arr ? me_arr[i] : me_scalar;
-            if (me >= 0 && me <= me_cap) {
-                const int K = min(min(me, max(ql, rl)), wfadev::reach(me, p));
-                const bool wave = 2 * K + 1 <= tune.wave_max_width &&
-                                  wfadev::rows_fit_int16(ql, me, p) &&
-                                  wfadev::wave_pair_bytes(ql, rl, me, p, gm) <=
-                                      wave_part_bytes(tune.lds_bytes);
-                cls = wave ? kMixWfaWave : kMixWfa;
-                payload = wfa_work_estimate(me, ql, rl, p, gm.g);
+            } else if (k == 1) {
+                const int me = me_arr ? me_arr[i] : me_scalar;
+                if (me >= 0 && me <= me_cap) {
+                    const int K = min(min(me, max(ql, rl)), wfadev::reach(me, p));
+                    const bool wave = 2 * K + 1 <= tune.wave_max_width &&
+                                      wfadev::rows_fit_int16(ql, me, p) &&
+                                      wfadev::wave_pair_bytes(ql, rl, me, p, gm) <=
+                                          wave_part_bytes(tune.lds_bytes);
+                    cls = wave ? kMixWfaWave : kMixWfa;
+                    payload = wfa_work_estimate(me, ql, rl, p, gm.g);
+                }
             }
         }
-    }
-    const int lane = threadIdx.x & 63;
-    for (int c = 0; c < kMixClasses; ++c) {
-        const uint64_t m = __ballot(cls == c);
-        if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&counts[c], __popcll(m));
+        for (int c = 0; c < kMixClasses; ++c) {
+            const uint64_t m = __ballot(cls == c);
+            if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&cls_count[c], __popcll(m));
+        }
+        if (in) {
+            if (cls == kMixBad) {
+                out.score[i] = CGM_BAD_PAIR;
+                if (out.q_end) out.q_end[i] = -1;
+                if (out.r_end) out.r_end[i] = -1;
+            }
+            keys[i] = ((uint32_t)(7 - cls) << 28) | (payload & 0x0fffffffu);
+            idx[i] = (int32_t)i;
+        }
     }
     __syncthreads();
+    if (threadIdx.x < kMixClasses && cls_count[threadIdx.x])
+        atomicAdd(&counts[threadIdx.x], cls_count[threadIdx.x]);
     for (int j = threadIdx.x; j < kRegBands; j += blockDim.x)
         if (hist[j]) atomicAdd(&band_counts[j], hist[j]);
-    if (!in) return;
-    if (cls == kMixBad) {
-        out.score[i] = CGM_BAD_PAIR;
-        if (out.q_end) out.q_end[i] = -1;
-        if (out.r_end) out.r_end[i] = -1;
-    }
-    keys[i] = ((uint32_t)(7 - cls) << 28) | (payload & 0x0fffffffu);
-    idx[i] = (int32_t)i;
 }
 
 // Class ranges in the sorted order and the task counts.  Register-lane pairs
@@ -623,14 +638,19 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     const int64_t n = b.n_pairs, nt = P->task_cap;
     hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(256), 0, stream, P->small, kSmallWords);
     const MixTuning tune = P->tune;
-    hipLaunchKernelGGL(mix_class_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, b,
+    const unsigned class_grid = (unsigned)std::min<int64_t>((n + 255) / 256, P->cus);
+    hipLaunchKernelGGL(mix_class_kernel, dim3(class_grid), dim3(256), 0, stream, b,
                        kind, kind_scalar, band, band_scalar, me, me_scalar, P->me_cap, sc, pen,
                        P->gm, tune, out, P->keys, P->idx, counts, P->small + kSmallBandCount);
     hipLaunchKernelGGL(mix_ranges_kernel, dim3(1), dim3(64), 0, stream, P->small);
     size_t tb = P->sort_tmp_bytes;
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->keys, P->keys_out,
                                                          P->idx, P->order, (int)n, 0, 32, stream));
-    const int64_t tasks_needed = n + (n + kMixGroup - 1) / kMixGroup + 1;
+    // SW-only batches hold at most ceil(n / kMixGroup) + kRegBands + 1 tasks
+    // (band-aligned register tasks, packed tasks); mixed ones at most n + ...
+    const bool sw_only = kind == nullptr && kind_scalar == 0;
+    const int64_t tasks_needed = sw_only ? (n + kMixGroup - 1) / kMixGroup + kRegBands + 1
+                                         : n + (n + kMixGroup - 1) / kMixGroup + 1;
     const int64_t tn = tasks_needed < nt ? tasks_needed : nt;
     hipLaunchKernelGGL(mix_tasks_kernel, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, stream,
                        P->small, P->keys_out, tn, P->tasks, P->tkeys, P->tidx);
@@ -639,7 +659,6 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                                                          P->tidx, P->task_order, (int)tn, 0, 32,
                                                          stream));
     CGM_HIP(hipEventRecord(P->ev_start, stream));
-    const bool sw_only = kind == nullptr && kind_scalar == 0;
     const int grid = std::min(P->cus * tune.blocks_per_cu, P->max_grid);
     hipLaunchKernelGGL(sw_only ? mix_persistent_kernel<false> : mix_persistent_kernel<true>,
                        dim3(grid), dim3(kMixThreads),
