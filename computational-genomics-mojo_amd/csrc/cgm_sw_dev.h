@@ -97,10 +97,23 @@ struct AlignedWords {
 
 // Match mask of one 32-base chunk: bit 62-2u set when base u of the chunk
 // equals the row's query base and lies inside [0, rlen) on a valid row.
+// rmin (wave-uniform) is the shortest reference among the wavefront's lanes
+// that have query bases: a chunk inside [0, rmin) needs no range mask, and
+// that test is one scalar branch (the other lanes never have a valid row).
 __device__ __forceinline__ uint64_t chunk_mask(uint64_t win, uint64_t qrep, int rs, int rlen,
-                                               bool row_ok) {
+                                               bool row_ok, int rmin) {
     const uint64_t x = win ^ qrep;
-    return ~(x | (x >> 1)) & (row_ok ? base_range_mask(-rs, rlen - rs) : 0ull);
+    const uint64_t eq = ~(x | (x >> 1));
+    uint64_t range = 0x5555555555555555ull;
+    if (rs < 0 || rs + 32 > rmin) range = base_range_mask(-rs, rlen - rs);
+    return eq & (row_ok ? range : 0ull);
+}
+
+// Shortest reference among the lanes that have query bases (wave-uniform).
+__device__ __forceinline__ int wave_min_ref(int qlen, int rlen) {
+    int v = qlen > 0 ? rlen : INT32_MAX;
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 template <int B>
@@ -114,6 +127,7 @@ __device__ __forceinline__ LaneResult sw_reg_lane(const uint64_t *q, int qlen, c
     const int cmis = sc.mismatch - oe;
     const int diff = sc.match - sc.mismatch;
     const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
+    const int rmin = wave_min_ref(qlen, rlen);
     int G[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) G[j] = oe;
@@ -139,7 +153,7 @@ __device__ __forceinline__ LaneResult sw_reg_lane(const uint64_t *q, int qlen, c
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const int rs = rs0 + 32 * c;
-            const uint64_t m = chunk_mask(rw.window(c, sh), qrep, rs, rlen, row_ok);
+            const uint64_t m = chunk_mask(rw.window(c, sh), qrep, rs, rlen, row_ok, rmin);
             const uint32_t mh = (uint32_t)(m >> 32), ml = (uint32_t)m;
 #pragma unroll
             for (int u = 0; u < 32; ++u) {
@@ -202,6 +216,7 @@ __device__ __forceinline__ LaneResult sw_reg_lane_packed(const uint64_t *q, int 
     const int cmis = sc.mismatch - oe;
     const int diff = sc.match - sc.mismatch;
     const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
+    const int rmin = wave_min_ref(qlen, rlen);
     const uint32_t oe_pair = ((uint32_t)oe & 0xffffu) | ((uint32_t)oe << 16);
     uint32_t P[NP];
 #pragma unroll
@@ -228,7 +243,7 @@ __device__ __forceinline__ LaneResult sw_reg_lane_packed(const uint64_t *q, int 
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const int rs = rs0 + 32 * c;
-            const uint64_t m = chunk_mask(rw.window(c, sh), qrep, rs, rlen, row_ok);
+            const uint64_t m = chunk_mask(rw.window(c, sh), qrep, rs, rlen, row_ok, rmin);
             const uint32_t mh = (uint32_t)(m >> 32), ml = (uint32_t)m;
 #pragma unroll
             for (int u = 0; u < 32; ++u) {
@@ -352,6 +367,7 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
     const uint32_t SUBX = ((uint32_t)(sc.mismatch - oe) & 0xffffu) * 0x10001u;
     constexpr uint32_t kNegE16 = 0xC000u;  // -16384
     const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
+    const int rmin = wave_min_ref(qlen, rlen);
     int base30;
     asm volatile("s_mov_b32 %0, 30" : "=s"(base30));
     s16x2 P[S], SV[S];
@@ -390,9 +406,9 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const uint64_t ml =
-                chunk_mask(rwl.window(c, sh_lo), qrep_lo, rs_lo + 32 * c, rlen, ok_lo);
+                chunk_mask(rwl.window(c, sh_lo), qrep_lo, rs_lo + 32 * c, rlen, ok_lo, rmin);
             const uint64_t mh =
-                chunk_mask(rwh.window(c, sh_hi), qrep_hi, rs_hi + 32 * c, rlen, ok_hi);
+                chunk_mask(rwh.window(c, sh_hi), qrep_hi, rs_hi + 32 * c, rlen, ok_hi, rmin);
             const uint32_t ml1 = (uint32_t)(ml >> 32), ml0 = (uint32_t)ml;
             const uint32_t mh1 = (uint32_t)(mh >> 32), mh0 = (uint32_t)mh;
 #pragma unroll
