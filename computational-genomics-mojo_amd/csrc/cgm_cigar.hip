@@ -10,15 +10,23 @@
 //  2. per pair, the size of its full wavefront history: one row of
 //     2K + 1 diagonals (K = the reach of its cost) per score step, for M
 //     (after extension), I and D, int16 when every row offset fits;
-//  3. pairs in chunks whose histories fit the workspace budget; per chunk,
+//  3. pairs ordered by history size, largest first (the longest pairs
+//     start first, so the launch does not end on one of them), in chunks
+//     whose histories fit the workspace budget (half the free device
+//     memory, at least 8 GiB and at most 96 GiB); per chunk,
 //     one 256-thread workgroup per pair recomputes the forward pass (the
 //     same exact reductions as the cost kernels: only multiples of
-//     gcd(x, o, e), only diagonals |k| <= (s - o) / e) writing every row to
-//     its HBM history, then thread 0 walks the reference's backtrace.
-// A diagonal or score outside the stored rows holds no reachable state, so
-// reading it as invalid gives the reference's NEG_INF there.  M before
-// extension (the reference's M_pre) is recomputed from the stored rows with
-// the forward formula.
+//     gcd(x, o, e), only diagonals in the score's range
+//     [max(-rlen, -me, -R(s)), min(qlen, me, R(s))], R(s) = (s - o) / e),
+//     keeping the score rows it still reads in an LDS ring (as the cost
+//     kernels do) and writing each row's range to its HBM history, then
+//     thread 0 walks the reference's backtrace.  Pairs whose ring does not
+//     fit LDS read the earlier rows back from the history instead.
+// A diagonal outside a row's range holds no reachable state, so the
+// backtrace reads it as invalid (the reference's NEG_INF) without touching
+// memory; rows are written only inside their range.  M before extension
+// (the reference's M_pre) is recomputed from the stored rows with the
+// forward formula.
 
 #include <algorithm>
 #include <vector>
@@ -32,8 +40,9 @@ namespace cgm {
 namespace {
 
 constexpr int kCigarThreads = 256;
-constexpr int kCigarSeqLds = 48 * 1024;  // sequences staged to LDS up to this
-constexpr int64_t kCigarBudget = int64_t(8) << 30;
+constexpr int kCigarLds = 52 * 1024;  // ring + sequences per workgroup (3 per CU)
+constexpr int64_t kCigarBudgetMin = int64_t(8) << 30;
+constexpr int64_t kCigarBudgetMax = int64_t(96) << 30;
 
 struct CigarGeom {
     int64_t K, W, rows, esize;  // K = 0 and rows = 0: nothing to store
@@ -76,14 +85,37 @@ __device__ __forceinline__ int hist_get(const T *row, int64_t K, int64_t k) {
     return (int)row[k + K];
 }
 
+// Diagonal range of score step si (score si * g): the only entries of the
+// row that the forward pass writes.
+struct RowRange {
+    int qlen, rlen, me, g;
+    WfaPen p;
+    __device__ __forceinline__ bool has(int si, int64_t k) const {
+        const int64_t R = reach64((int64_t)si * g, p);
+        const int64_t lo = max(max((int64_t)-rlen, (int64_t)-me), -R);
+        const int64_t hi = min(min((int64_t)qlen, (int64_t)me), R);
+        return k >= lo && k <= hi;
+    }
+};
+
 // Backtrace of align_cpu.mojo:510-600 over the stored rows; writes the runs
 // (reversed, then flipped) and returns their number, or -2 past `cap`.
 template <typename T>
 __device__ int cigar_backtrace(const T *M, const T *I, const T *D, int64_t K, int64_t W,
-                               int qlen, int rlen, int cost, const WfaPen &p, int g,
+                               int qlen, int rlen, int me, int cost, const WfaPen &p, int g,
                                uint32_t *out, int64_t cap) {
     const int dx = p.x / g, dO = (p.o + p.e) / g, de = p.e / g;
-    auto row_of = [&](const T *A, int si) { return A + (int64_t)si * W; };
+    const RowRange rr{qlen, rlen, me, g, p};
+    // entry k of step si of A, or -1 outside the step's range
+    struct Row {
+        const T *base;
+        int si;
+    };
+    auto row_of = [&](const T *A, int si) { return Row{A + (int64_t)si * W, si}; };
+    auto hist_get = [&](Row row, int64_t KK, int64_t k) -> int {
+        if (!rr.has(row.si, k)) return -1;
+        return cgm::hist_get(row.base, KK, k);
+    };
     // M before extension at (si, k): the forward formula over stored rows
     auto m_pre = [&](int si, int64_t k) -> int {
         if (si == 0) return k == 0 ? 0 : -1;
@@ -205,6 +237,72 @@ __device__ int cigar_backtrace(const T *M, const T *I, const T *D, int64_t K, in
     return (int)nr;
 }
 
+// Forward pass with the score rows it still reads in an LDS ring of
+// nM M-rows and nI I- and D-rows (slot of step si: si mod n), as the cost
+// kernels keep them; every row's range is also written to the HBM history
+// (row si of M, I and D).  Ranges grow with the score, so a ring slot's
+// entries outside the current range were never written and still hold the
+// initial invalid value.
+template <typename T>
+__device__ void cigar_forward_ring(const uint64_t *q, int qlen, const uint64_t *r, int rlen, int me,
+                                   int cost, const WfaPen &p, const wfadev::WfaGeom &gm, T *M,
+                                   T *I, T *D, int64_t K, int64_t W, T *ring) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const T neg = wfadev::RowNeg<T>::v;
+    const int w = (int)W, kk = (int)K;
+    T *RM = ring, *RI = ring + gm.nM * w, *RD = RI + gm.nI * w;
+    for (int i = tid; i < (gm.nM + 2 * gm.nI) * w; i += nt) ring[i] = neg;
+    __syncthreads();
+    if (tid == 0) {
+        const T v = (T)wfadev::extend_diag(q, qlen, qw, r, rlen, rw, 0, 0);
+        RM[kk] = v;
+        M[kk] = v;
+        I[kk] = neg;
+        D[kk] = neg;
+    }
+    __syncthreads();
+    wfadev::WfaSteps st;
+    st.init(p, gm);
+    const int nrows = cost / gm.g + 1;
+    for (int si = 1; si < nrows; ++si) {
+        st.next(p, gm);
+        const int lo = max(max(-rlen, -me), -st.R), hi = min(min(qlen, me), st.R);
+        const int ms = st.ms * w, is = st.is * w;
+        const int mx = st.mx(gm) >= 0 ? st.mx(gm) * w : -1;
+        const int mo = st.mo(gm) >= 0 ? st.mo(gm) * w : -1;
+        const int ie = st.ie(gm) >= 0 ? st.ie(gm) * w : -1;
+        T *Mr = M + (int64_t)si * W, *Ir = I + (int64_t)si * W, *Dr = D + (int64_t)si * W;
+        for (int k = lo + tid; k <= hi; k += nt) {
+            const int di = k + kk;
+            int a = wfadev::kNegRow, c = wfadev::kNegRow, bb = wfadev::kNegRow,
+                d = wfadev::kNegRow, m = wfadev::kNegRow;
+            if (mo >= 0) {
+                if (di > 0) a = (int)RM[mo + di - 1];
+                if (di < w - 1) bb = (int)RM[mo + di + 1];
+            }
+            if (ie >= 0) {
+                if (di > 0) c = (int)RI[ie + di - 1];
+                if (di < w - 1) d = (int)RD[ie + di + 1];
+            }
+            if (mx >= 0) m = (int)RM[mx + di];
+            const int iv = max(a, c) + 1;
+            const int dv = max(bb, d);
+            int mv = max(max(m + 1, iv), dv);
+            const T ti = iv < 0 ? neg : (T)iv, td = dv < 0 ? neg : (T)dv;
+            RI[is + di] = ti;
+            RD[is + di] = td;
+            Ir[di] = ti;
+            Dr[di] = td;
+            if (mv >= 0) mv = wfadev::extend_diag(q, qlen, qw, r, rlen, rw, mv, k);
+            const T tm = mv < 0 ? neg : (T)mv;
+            RM[ms + di] = tm;
+            Mr[di] = tm;
+        }
+        __syncthreads();
+    }
+}
+
 // Forward pass with full history, one workgroup per pair; rows of score
 // step si hold diagonals -K..K (entries outside the step's range invalid).
 template <typename T>
@@ -265,11 +363,12 @@ __device__ void cigar_forward(const uint64_t *q, int qlen, const uint64_t *r, in
 }
 
 __global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
-    Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p, int g, const int32_t *cost_arr,
-    const int64_t *hist_off, uint8_t *ws, int64_t p0, uint32_t *cigar, const int64_t *cigar_off,
-    int32_t *cigar_len) {
-    __shared__ uint64_t seq[kCigarSeqLds / 8];
-    const int64_t pair = p0 + blockIdx.x;
+    Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p, wfadev::WfaGeom gm,
+    const int32_t *cost_arr, const int64_t *hist_off, uint8_t *ws, const int32_t *order,
+    int64_t p0, uint32_t *cigar, const int64_t *cigar_off, int32_t *cigar_len) {
+    extern __shared__ uint64_t lds[];  // kCigarLds bytes: ring, then sequences
+    const int g = gm.g;
+    const int64_t pair = order[p0 + blockIdx.x];
     const int cost = cost_arr[pair];
     const uint64_t *qg, *rg;
     int qlen = 0, rlen = 0;
@@ -281,7 +380,13 @@ __global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
     const CigarGeom cg = cigar_geom(qlen, rlen, me, cost, p, g);
     const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
     const uint64_t *q = qg, *r = rg;
-    if (8 * (qw + rw) <= kCigarSeqLds) {
+    // LDS: the ring when it fits, then the sequences when they fit after it
+    const int64_t ring_bytes =
+        (((int64_t)gm.nM + 2 * gm.nI) * cg.W * cg.esize + 15) & ~(int64_t)15;
+    const bool ring = ring_bytes <= kCigarLds;
+    const int64_t seq_at = ring ? ring_bytes : 0;
+    if (seq_at + 8 * (qw + rw) <= kCigarLds) {
+        uint64_t *seq = lds + seq_at / 8;
         for (int i = threadIdx.x; i < qw + rw; i += blockDim.x) seq[i] = i < qw ? qg[i] : rg[i - qw];
         q = seq;
         r = seq + qw;
@@ -293,18 +398,26 @@ __global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
     const int64_t cap = cigar_off[pair + 1] - cigar_off[pair];
     if (cg.esize == 2) {
         int16_t *M = (int16_t *)h;
-        cigar_forward<int16_t>(q, qlen, r, rlen, me, cost, p, g, M, M + plane, M + 2 * plane, cg.K,
-                               cg.W);
+        if (ring)
+            cigar_forward_ring<int16_t>(q, qlen, r, rlen, me, cost, p, gm, M, M + plane,
+                                        M + 2 * plane, cg.K, cg.W, (int16_t *)lds);
+        else
+            cigar_forward<int16_t>(q, qlen, r, rlen, me, cost, p, g, M, M + plane, M + 2 * plane,
+                                   cg.K, cg.W);
         if (threadIdx.x == 0)
             cigar_len[pair] = cigar_backtrace<int16_t>(M, M + plane, M + 2 * plane, cg.K, cg.W,
-                                                       qlen, rlen, cost, p, g, out, cap);
+                                                       qlen, rlen, me, cost, p, g, out, cap);
     } else {
         int32_t *M = (int32_t *)h;
-        cigar_forward<int32_t>(q, qlen, r, rlen, me, cost, p, g, M, M + plane, M + 2 * plane, cg.K,
-                               cg.W);
+        if (ring)
+            cigar_forward_ring<int32_t>(q, qlen, r, rlen, me, cost, p, gm, M, M + plane,
+                                        M + 2 * plane, cg.K, cg.W, (int32_t *)lds);
+        else
+            cigar_forward<int32_t>(q, qlen, r, rlen, me, cost, p, g, M, M + plane, M + 2 * plane,
+                                   cg.K, cg.W);
         if (threadIdx.x == 0)
             cigar_len[pair] = cigar_backtrace<int32_t>(M, M + plane, M + 2 * plane, cg.K, cg.W,
-                                                       qlen, rlen, cost, p, g, out, cap);
+                                                       qlen, rlen, me, cost, p, g, out, cap);
     }
 }
 
@@ -347,46 +460,62 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
     if (err == hipSuccess) err = hipStreamSynchronize(stream);
     (void)hipFreeAsync(d_bytes, stream);
     if (err != hipSuccess) return hip_error(err, "cigar: history sizes");
-    // chunks [p0, p1) of consecutive pairs; a pair larger than the budget
-    // forms a chunk of its own
+    // pairs by history size, largest first; chunks [p0, p1) of that order,
+    // a pair larger than the budget forming a chunk of its own
+    std::vector<int32_t> order((size_t)n_pairs);
+    for (int64_t i = 0; i < n_pairs; ++i) order[(size_t)i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t c) { return bytes[(size_t)a] > bytes[(size_t)c]; });
+    size_t free_b = 0, total_b = 0;
+    int64_t budget = kCigarBudgetMin;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        budget = std::min(kCigarBudgetMax, std::max(kCigarBudgetMin, (int64_t)(free_b / 2)));
     std::vector<std::pair<int64_t, int64_t>> chunks;
     int64_t ws_need = 0;
     for (int64_t i = 0; i < n_pairs;) {
         int64_t j = i, acc = 0;
-        while (j < n_pairs && (j == i || acc + bytes[j] <= kCigarBudget)) {
-            off[j] = acc;
-            acc += bytes[j];
+        while (j < n_pairs && (j == i || acc + bytes[(size_t)order[(size_t)j]] <= budget)) {
+            off[(size_t)order[(size_t)j]] = acc;
+            acc += bytes[(size_t)order[(size_t)j]];
             ++j;
         }
         chunks.push_back({i, j});
         ws_need = std::max(ws_need, acc);
         i = j;
     }
-    CGM_HIP(hipMallocAsync((void **)&d_off, (size_t)n_pairs * 8, stream));
+    // workspaces first, uploads after: every exit below waits for the stream,
+    // so the host vectors outlive their asynchronous copies
+    int32_t *d_order = nullptr;
     uint8_t *ws = nullptr;
-    err = hipMallocAsync((void **)&ws, (size_t)std::max<int64_t>(ws_need, 256), stream);
-    if (err != hipSuccess) {
-        (void)hipFreeAsync(d_off, stream);
-        return hip_error(err, "cigar: history workspace");
-    }
     int rc = CGM_OK;
-    err = hipMemcpyAsync(d_off, off.data(), (size_t)n_pairs * 8, hipMemcpyHostToDevice, stream);
-    if (err != hipSuccess) rc = hip_error(err, "cigar: offsets");
+    if ((err = hipMallocAsync((void **)&d_order, (size_t)n_pairs * 4, stream)) != hipSuccess ||
+        (err = hipMallocAsync((void **)&d_off, (size_t)n_pairs * 8, stream)) != hipSuccess ||
+        (err = hipMallocAsync((void **)&ws, (size_t)std::max<int64_t>(ws_need, 256), stream)) !=
+            hipSuccess)
+        rc = hip_error(err, "cigar: workspace");
+    if (!rc && (err = hipMemcpyAsync(d_order, order.data(), (size_t)n_pairs * 4,
+                                     hipMemcpyHostToDevice, stream)) != hipSuccess)
+        rc = hip_error(err, "cigar: order");
+    if (!rc && (err = hipMemcpyAsync(d_off, off.data(), (size_t)n_pairs * 8,
+                                     hipMemcpyHostToDevice, stream)) != hipSuccess)
+        rc = hip_error(err, "cigar: offsets");
     // 3. forward with history + backtrace, chunk by chunk (stream order
     // keeps a chunk's workspace intact until its kernel is done)
     for (size_t c = 0; !rc && c < chunks.size(); ++c) {
         const int64_t p0 = chunks[c].first, np = chunks[c].second - p0;
-        hipLaunchKernelGGL(wfa_cigar_kernel, dim3((unsigned)np), dim3(kCigarThreads), 0, stream, b,
-                           max_error, max_error_scalar, p, g, out_cost, d_off, ws, p0, cigar,
+        hipLaunchKernelGGL(wfa_cigar_kernel, dim3((unsigned)np), dim3(kCigarThreads),
+                           (size_t)kCigarLds, stream, b, max_error, max_error_scalar, p,
+                           wfadev::WfaGeom{g, nM, nI}, out_cost, d_off, ws, d_order, p0, cigar,
                            cigar_off, cigar_len);
         err = hipGetLastError();
         if (err != hipSuccess) rc = hip_error(err, "cigar: kernel launch");
     }
-    // the host copy of `off` must outlive the asynchronous upload
+    // the host copies of `off` and `order` must outlive the asynchronous uploads
     err = hipStreamSynchronize(stream);
     if (!rc && err != hipSuccess) rc = hip_error(err, "cigar: synchronize");
-    (void)hipFreeAsync(ws, stream);
-    (void)hipFreeAsync(d_off, stream);
+    if (ws) (void)hipFreeAsync(ws, stream);
+    if (d_off) (void)hipFreeAsync(d_off, stream);
+    if (d_order) (void)hipFreeAsync(d_order, stream);
     return rc;
 }
 

@@ -170,7 +170,8 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
  * pair or two empty sequences; -2 when the pair's capacity was too small.
  * A CIGAR of cost c has at most 2 c / min(x, o + e) + 1 runs.  Computes
  * the costs first, then re-runs each pair with its whole wavefront history
- * in HBM (chunks of at most 8 GiB) and walks it back; allocates its
+ * in HBM (longest pairs first, in chunks of at most half the free device
+ * memory, 8-96 GiB) and walks it back; allocates its
  * workspace on `stream` and synchronises with the host before returning. */
 int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                          const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
