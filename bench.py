@@ -314,7 +314,8 @@ def main():
                          "one register kernel per band bucket")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run 3 device-to-device copies of this many bytes (counter scale)")
-    ap.add_argument("--only-extra", choices=["wfa_config3", "ingest_config2", "wfa_cigar_config3"],
+    ap.add_argument("--only-extra", choices=["wfa_config3", "ingest_config2", "wfa_cigar_config3",
+                                             "nw_config2"],
                     help="run only this extra workload and print its result (for profiling "
                          "its kernels alone)")
     args = ap.parse_args()
@@ -322,9 +323,14 @@ def main():
         m = load_package()
         from cgm_amd import gen
         dev = m.GenomicsDevice(0)
-        fn = {"wfa_config3": wfa_extra, "ingest_config2": ingest_extra,
-              "wfa_cigar_config3": cigar_extra}[args.only_extra]
-        res = fn(m, dev, gen)
+        if args.only_extra == "nw_config2":
+            w = gen.generate(2, args.pairs)
+            cells = int(gen.valid_band_cells(w.q_len, w.r_len, w.band).sum())
+            res = nw_extra(m, dev, w, cells)
+        else:
+            fn = {"wfa_config3": wfa_extra, "ingest_config2": ingest_extra,
+                  "wfa_cigar_config3": cigar_extra}[args.only_extra]
+            res = fn(m, dev, gen)
         print(json.dumps({"extra": args.only_extra, **res}), flush=True)
         return
     if args.copy_probe:

@@ -137,26 +137,30 @@ __device__ __forceinline__ int nw_reg_dispatch(int band, const uint64_t *q, int 
     }
 }
 
-// Register lane kernel for effective bands LO..HI over order[range[0] ..
-// range[1]): one wavefront per 64 entries, once per distinct band among
-// them (entries arrive sorted by band and length).
-template <int LO, int HI, int WPE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void nw_reg_kernel(
+// Register lanes over order[range[0] .. range[1]) (every pair there has an
+// effective band <= 64, sorted by band then length, largest first): each
+// wavefront takes the next 64 entries from an atomic counter until the range
+// is exhausted, so the widest and longest groups start first and no
+// wavefront idles while another has work; a group holding two bands runs
+// once per band.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void nw_queue_kernel(
     Batch b, const int32_t *band_arr, int32_t band_scalar, SwScoring sc, SwOut out,
-    const int32_t *order, const int32_t *range) {
+    const int32_t *order, const int32_t *range, int32_t *next) {
     const int64_t start = range[0], end = range[1];
     const int lane = threadIdx.x & 63;
-    for (int64_t base = start + (int64_t)blockIdx.x * 64; base < end;
-         base += (int64_t)gridDim.x * 64) {
+    for (;;) {
+        int u = 0;
+        if (lane == 0) u = atomicAdd(next, 1);
+        u = __shfl(u, 0, 64);
+        const int64_t base = start + 64 * (int64_t)u;
+        if (base >= end) break;
         const int count = (int)min((int64_t)64, end - base);
         const bool active = lane < count;
         const int64_t pair = active ? order[base + lane] : 0;
         const uint64_t *q = b.q_packed, *r = b.r_packed;
         int qlen = 0, rlen = 0, band = -1;
-        if (active && load_pair(b, pair, q, r, qlen, rlen)) {
+        if (active && load_pair(b, pair, q, r, qlen, rlen))
             band = effective_band(band_arr ? band_arr[pair] : band_scalar, qlen, rlen);
-            if (band < LO || band > HI) band = -1;
-        }
         uint64_t pending = __ballot(band >= 0);
         while (pending) {
             const int lead = __ffsll((unsigned long long)pending) - 1;
@@ -165,8 +169,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             pending &= ~__ballot(mine);
             int rows = mine ? qlen : 0;
             for (int o = 32; o > 0; o >>= 1) rows = max(rows, __shfl_xor(rows, o, 64));
-            const int s = nw_reg_dispatch<LO, HI>(bsel, q, mine ? qlen : 0, r, mine ? rlen : 0,
-                                                  rows, sc);
+            const int s = nw_reg_dispatch<0, 64>(bsel, q, mine ? qlen : 0, r, mine ? rlen : 0,
+                                                 rows, sc);
             if (mine) {
                 out.score[pair] = s;
                 out.q_end[pair] = qlen - 1;
@@ -266,18 +270,33 @@ __global__ void nw_class_kernel(Batch b, const int32_t *band, int32_t band_scala
     if (!in) return;
     if (cls == kNwBad) nw_write_bad(out, i);
     const uint32_t len = (uint32_t)min(max(ql, rl), (1 << 17) - 1);
-    keys[i] = ((uint32_t)(15 - cls) << 28) | ((uint32_t)min(be, 2047) << 17) | len;
+    // the register classes share the top key, so they sort by band then
+    // length, widest first (classes 3, 2, 1, 0), then plain, then bad
+    const uint32_t top = cls <= 3 ? 15u : (uint32_t)(18 - cls);
+    keys[i] = (top << 28) | ((uint32_t)min(be, 2047) << 17) | len;
     idx[i] = (int32_t)i;
 }
 
+// Class ranges in the sorted order: register classes 3, 2, 1, 0, then
+// plain, then bad.
 __global__ void nw_ranges_kernel(int32_t *counts, int32_t *ranges) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int seq[kNwClasses] = {3, 2, 1, 0, kNwPlain, kNwBad};
     int32_t s = 0;
-    for (int c = 0; c < kNwClasses; ++c) {
+    for (int i = 0; i < kNwClasses; ++i) {
+        const int c = seq[i];
         ranges[2 * c] = s;
         s += counts[c];
         ranges[2 * c + 1] = s;
     }
+}
+
+// [start, end) of the register classes 3..0, which follow each other in the
+// sorted order.
+__global__ void nw_range_join_kernel(const int32_t *ranges, int32_t *reg_range) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    reg_range[0] = ranges[2 * 3];
+    reg_range[1] = ranges[2 * 0 + 1];
 }
 
 __global__ void nw_zero_kernel(int32_t *p, int n) {
@@ -313,23 +332,16 @@ int launch_nw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
                        band, band_scalar, sc, out, keys, idx, counts);
     hipLaunchKernelGGL(nw_ranges_kernel, dim3(1), dim3(64), 0, stream, counts, ranges);
     int rc = sort_desc_by_key(keys, idx, order, n, stream);
+    // register classes 3, 2, 1, 0 (widest band first) are one contiguous
+    // range of the sorted order: one queue over all of them
     const int64_t waves = (n + 63) / 64;
-    const int grid = (int)(waves < (int64_t)cus * 16 ? waves : (int64_t)cus * 16);
-    hipStream_t aux[4];
-    if (!rc) rc = fork_streams(stream, 4, aux);
+    const int grid = (int)(waves < (int64_t)cus * 8 ? waves : (int64_t)cus * 8);
     if (!rc) {
-#define CGM_NW_REG(K, LO, HI, WPE)                                                            \
-    hipLaunchKernelGGL((nw_reg_kernel<LO, HI, WPE>), dim3(grid), dim3(64), 0, aux[3 - (K)], b, \
-                       band, band_scalar, sc, out, order, ranges + 2 * (K));
-        CGM_NW_REG(3, 49, 64, 2)
-        CGM_NW_REG(2, 33, 48, 3)
-        CGM_NW_REG(1, 17, 32, 4)
-        CGM_NW_REG(0, 0, 16, 5)
-#undef CGM_NW_REG
+        hipLaunchKernelGGL(nw_range_join_kernel, dim3(1), dim3(64), 0, stream, ranges, small + 40);
+        hipLaunchKernelGGL(nw_queue_kernel, dim3(grid), dim3(64), 0, stream, b, band, band_scalar,
+                           sc, out, order, small + 40, small + 42);
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) rc = hip_error(e, "nw register kernels");
-        const int jrc = join_streams(stream, 4, aux);
-        if (!rc) rc = jrc;
+        if (e != hipSuccess) rc = hip_error(e, "nw register kernel");
     }
     if (!rc)
         rc = launch_nw_plain(b, band, band_scalar, sc, out, order, ranges + 2 * kNwPlain, cus * 2,
