@@ -269,7 +269,7 @@ __device__ __noinline__ int mix_wfa_block(const uint64_t *q, int qlen, const uin
                                           int me, WfaPen p, wfadev::WfaGeom gm, uint8_t *lds,
                                           int lds_bytes, int32_t *slab, int64_t slab_words,
                                           int tile_steps) {
-    return wfadev::wfa_pair_block<kMixWfaUnroll>(q, qlen, r, rlen, me, p, gm, lds, lds_bytes,
+    return wfadev::wfa_pair_block<kMixWfaUnroll, kMixThreads>(q, qlen, r, rlen, me, p, gm, lds, lds_bytes,
                                                  slab, slab_words, tile_steps);
 }
 

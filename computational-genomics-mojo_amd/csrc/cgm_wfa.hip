@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kTeamThreads) void wfa_wave_kernel(
         const int me = me_arr ? me_arr[pair] : me_scalar;
         const int res = (!ok || me < 0 || me > me_cap)
                             ? CGM_BAD_PAIR
-                            : wfa_pair_block<kTeamUnroll>(q, qlen, r, rlen, me, p, gm, lds_raw,
+                            : wfa_pair_block<kTeamUnroll, kTeamThreads>(q, qlen, r, rlen, me, p, gm, lds_raw,
                                                           lds_bytes, my_slab, slab_words);
         if (threadIdx.x == 0) out[pair] = res;
         __syncthreads();

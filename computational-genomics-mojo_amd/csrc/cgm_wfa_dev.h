@@ -211,6 +211,192 @@ __device__ int wfa_pair_team(const uint64_t *q, int qlen, const uint64_t *r, int
     return -1;
 }
 
+// ---------------------------------------------------------------------------
+// Branch-free team layout for pairs whose sequences and int16 wavefronts fit
+// LDS (the common case).  Same recurrence and schedule as wfa_pair_team; what
+// changes is the data layout, so that a diagonal costs no bounds or slot
+// tests and its first extension step no branch:
+//  * sequences are staged as 32-bit words with base i at bits 2i..2i+1 of
+//    word i / 16 (each half of a packed word bit-reversed, which also swaps
+//    the two bits of a base: harmless, bases are only compared), followed by
+//    zero words.  The 32 bases from any position are then two alignbit
+//    funnel shifts of three consecutive words, and the matching prefix of
+//    two windows is the trailing-zero count of their XOR over two;
+//  * every wavefront row has a guard entry on each side that stays invalid,
+//    so diagonals k - 1 and k + 1 need no range test, and I and D of one
+//    diagonal sit side by side (one address for I[k - 1] and D[k + 1]);
+//  * the slot of s - x, s - o - e or s - e before that score exists is a
+//    ring slot never written so far (slots 0..s/g only), so it reads as
+//    invalid without the existence test;
+//  * diagonals are taken in wavefront-wide groups: a group past the score's
+//    range is skipped as a whole (a scalar test), and lanes of a partial
+//    group past `hi` compute on whatever their rows hold but write nothing.
+//    `kFastSlack` bytes after the rows keep those reads inside the layout.
+// A valid M/I/D row on diagonal k is always >= max(k, 0) (each step adds to
+// the row and column or keeps both), so the column row - k of a valid cell
+// is never negative; clamping positions only matters for invalid lanes,
+// whose results are discarded.
+// ---------------------------------------------------------------------------
+constexpr int kFastSlack = 512;
+
+__device__ __forceinline__ int64_t team_fast_bytes(int qlen, int rlen, int W, const WfaGeom &gm) {
+    const int64_t nq = 2 * (int64_t)((qlen + 31) >> 5) + 4, nr = 2 * (int64_t)((rlen + 31) >> 5) + 4;
+    const int64_t WP = W + 2;
+    const int64_t m_ents = (gm.nM * WP + 1) & ~(int64_t)1;
+    return 4 * (nq + nr) + 2 * m_ents + 4 * gm.nI * WP + kFastSlack;
+}
+
+// Bases pos .. pos + 31 of an LSB-first sequence as two words (base pos + i
+// at bits 2i of w0 for i < 16, of w1 for i >= 16).
+__device__ __forceinline__ void window_lsb(const uint32_t *s, int pos, uint32_t &w0, uint32_t &w1) {
+    const uint32_t *p = s + (pos >> 4);
+    const uint32_t sh = 2u * (uint32_t)(pos & 15);
+    const uint32_t a = p[0], b = p[1], c = p[2];
+    w0 = __builtin_amdgcn_alignbit(b, a, sh);
+    w1 = __builtin_amdgcn_alignbit(c, b, sh);
+}
+
+// Leading bases (0..32) on which two 32-base windows agree.
+__device__ __forceinline__ int match32(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
+    const uint64_t x = ((uint64_t)(a1 ^ b1) << 32) | (uint64_t)(a0 ^ b0);
+    return x ? (int)(__builtin_ctzll(x) >> 1) : 32;
+}
+
+// Furthest row reached from (row, col), col >= 0, following matching bases.
+__device__ __forceinline__ int extend_lsb(const uint32_t *q, int qlen, const uint32_t *r, int rlen,
+                                          int row, int col) {
+    while (row < qlen && col < rlen) {
+        uint32_t a0, a1, b0, b1;
+        window_lsb(q, row, a0, a1);
+        window_lsb(r, col, b0, b1);
+        const int n = match32(a0, a1, b0, b1);
+        const int lim = min(qlen - row, rlen - col);
+        if (n >= lim) return row + lim;
+        row += n;
+        col += n;
+        if (n < 32) break;
+    }
+    return row;
+}
+
+// One pair aligned by the whole workgroup of NT threads in the layout above
+// (`lds` holds at least team_fast_bytes; the caller checks rows_fit_int16).
+template <int U, int NT>
+__device__ int wfa_pair_team_fast(const uint64_t *qg, int qlen, const uint64_t *rg, int rlen,
+                                  int me, const WfaPen &p, const WfaGeom &gm, uint8_t *lds, int W,
+                                  int K) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int nq = 2 * qw + 4, nr = 2 * rw + 4;
+    uint32_t *qs = (uint32_t *)lds;
+    uint32_t *rs = qs + nq;
+    for (int i = tid; i < nq + nr; i += NT) {
+        const bool isq = i < nq;
+        const int j = isq ? i : i - nq;
+        uint32_t v = 0;
+        if ((j >> 1) < (isq ? qw : rw)) {
+            const uint64_t w = (isq ? qg : rg)[j >> 1];
+            v = __builtin_bitreverse32((j & 1) ? (uint32_t)w : (uint32_t)(w >> 32));
+        }
+        qs[i] = v;
+    }
+    const int WP = W + 2;
+    const int m_ents = (gm.nM * WP + 1) & ~1;
+    int16_t *Mb = (int16_t *)(lds + 4 * (nq + nr));
+    int16_t *IDb = Mb + m_ents;
+    const int16_t neg = RowNeg<int16_t>::v;
+    {
+        // rows and the slack after them, two entries per store
+        const int words = (m_ents + 2 * gm.nI * WP + kFastSlack / 2) >> 1;
+        uint32_t *w32 = (uint32_t *)Mb;
+        for (int i = tid; i < words; i += NT) w32[i] = 0x80008000u;
+    }
+    __shared__ int done_at;  // score at which the target was reached
+    if (tid == 0) done_at = -1;
+    __syncthreads();
+    if (tid == 0) Mb[K + 1] = (int16_t)extend_lsb(qs, qlen, rs, rlen, 0, 0);
+    __syncthreads();
+    const int target = qlen - rlen;
+    if (target == 0 && (int)Mb[K + 1] >= qlen) return 0;
+    // first thread of this wavefront, as a scalar so that group tests branch
+    const int wave0 = __builtin_amdgcn_readfirstlane(tid & ~63);
+    WfaSteps st;
+    st.init(p, gm);
+    for (int s = gm.g; s <= me; s += gm.g) {
+        st.next(p, gm);
+        const int lo = max(max(-rlen, -me), -st.R), hi = min(min(qlen, me), st.R);
+        const int16_t *Mo = Mb + WfaSteps::back(st.ms, st.dO, gm.nM) * WP;
+        const int16_t *Mx = Mb + WfaSteps::back(st.ms, st.dx, gm.nM) * WP;
+        const int16_t *IDe = IDb + WfaSteps::back(st.is, st.de, gm.nI) * 2 * WP;
+        int16_t *Ms = Mb + st.ms * WP;
+        int16_t *IDs = IDb + st.is * 2 * WP;
+        int found = 0;
+        for (int kb = lo + wave0; kb <= hi; kb += U * NT) {
+            // di = k + K: row entry di + 1 holds diagonal k
+            const int di0 = kb + lane + K;
+            const int ng = min(U, (hi - kb) / NT + 1);  // groups of this pass in range
+            int mv[U], mr[U];
+            // NEXT: M, I, D of every group from the older scores
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (u >= ng) break;
+                const int di = di0 + u * NT;
+                const int a = Mo[di], b = Mo[di + 2];
+                const int c = IDe[2 * di], d = IDe[2 * di + 5];
+                const int m = Mx[di + 1];
+                const int iv = max(a, c) + 1;
+                const int dv = max(b, d);  // negative only when both are `neg`
+                mv[u] = max(max(m + 1, iv), dv);
+                if (di - K <= hi) {
+                    IDs[2 * di + 2] = iv < 0 ? neg : (int16_t)iv;
+                    IDs[2 * di + 3] = (int16_t)dv;
+                }
+            }
+            // EXTEND, first 32 bases of every group: no branch
+            uint64_t more = 0;  // bit u: some lane of group u matched all 32
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (u >= ng) break;
+                const int k = di0 + u * NT - K;
+                const int row = mv[u], col = row - k;
+                uint32_t a0, a1, b0, b1;
+                window_lsb(qs, min(max(row, 0), qlen), a0, a1);
+                window_lsb(rs, min(max(col, 0), rlen), b0, b1);
+                const int lim = min(qlen - row, rlen - col);
+                const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
+                mr[u] = row + n;
+                if (__ballot(row >= 0 && n == 32 && lim > 32 && k <= hi)) more |= 1ull << u;
+            }
+            // the rest of long matches (few lanes)
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!(more >> u & 1)) continue;
+                    const int k = di0 + u * NT - K;
+                    const int row = mv[u], col = row - k;
+                    if (row >= 0 && mr[u] == row + 32 && min(qlen - row, rlen - col) > 32 && k <= hi)
+                        mr[u] = extend_lsb(qs, qlen, rs, rlen, row + 32, col + 32);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (u >= ng) break;
+                const int di = di0 + u * NT, k = di - K;
+                if (k <= hi) {
+                    const int row = mv[u];
+                    found |= (k == target && row >= 0 && mr[u] >= qlen);
+                    Ms[di + 1] = row < 0 ? neg : (int16_t)mr[u];
+                }
+            }
+        }
+        // One barrier ends the score (see wfa_pair_team).
+        if (found) done_at = s;
+        __syncthreads();
+        if (done_at == s) return s;
+    }
+    return -1;
+}
+
 // Trapezoid tiling of the team layout for pairs whose wavefronts exceed
 // LDS.  Scores advance in blocks of up to TB steps.  Each block's widest
 // range is cut into tiles of TW = LW - 2 TB diagonals; a tile copies every
@@ -446,11 +632,13 @@ __device__ int wfa_pair_wave(const uint64_t *qg, int qlen, const uint64_t *rg, i
     return -1;
 }
 
-// Everything one workgroup needs to align a WFA pair: sequences staged to
-// LDS when they fit `lds_bytes`; int16 wavefronts in LDS when they fit the
-// rest, otherwise in the workgroup's HBM slab (int16 rows when they fit
-// int16, int32 otherwise).  Returns the cost, -1, or CGM_BAD_PAIR.
-template <int U>
+// Everything one workgroup of NT threads needs to align a WFA pair: the
+// branch-free LDS layout when sequences and int16 wavefronts fit it;
+// otherwise sequences staged to LDS when they fit `lds_bytes`, int16
+// wavefronts in LDS when they fit the rest, else in the workgroup's HBM slab
+// (int16 rows when they fit int16, int32 otherwise).  Returns the cost, -1,
+// or CGM_BAD_PAIR.
+template <int U, int NT>
 __device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, const uint64_t *rg,
                                               int rlen, int me, const WfaPen &p,
                                               const WfaGeom &gm, uint8_t *lds, int lds_bytes,
@@ -466,6 +654,8 @@ __device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, cons
     const int64_t seq_bytes = 8 * (int64_t)(qw + rw);
     const int64_t wf16_bytes = 2 * (int64_t)slots * W;
     const bool fit16 = rows_fit_int16(qlen, me, p);
+    if (fit16 && team_fast_bytes(qlen, rlen, W, gm) <= lds_bytes)
+        return wfa_pair_team_fast<U, NT>(qg, qlen, rg, rlen, me, p, gm, lds, W, K);
     const uint64_t *q = qg, *r = rg;
     if (seq_bytes <= lds_bytes) {
         // Sequences go to LDS whenever they fit, also when the wavefronts

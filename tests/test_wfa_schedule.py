@@ -205,6 +205,171 @@ def test_incremental_step_state_equals_division_form():
             assert (back(is_, de, nI) if si >= de else -1) == (((s - e) // g) % nI if s >= e else -1)
 
 
+NEG16 = -32768
+FAST_SLACK = 512  # bytes of slack after the rows (kFastSlack)
+
+
+def _bitrev32(v):
+    return int(format(v, "032b")[::-1], 2)
+
+
+def lsb_words(seq):
+    """LDS staging of the branch-free team layout (wfa_pair_team_fast): each
+    32-bit half of the oracle's packed MSB-first words bit-reversed, high
+    half first, then four zero words."""
+    w, _ = O.pack_ascii(seq)
+    out = []
+    for v in w.tolist():
+        out += [_bitrev32(v >> 32), _bitrev32(v & 0xFFFFFFFF)]
+    return out + [0, 0, 0, 0]
+
+
+def _alignbit(a, b, sh):
+    return (((a << 32) | b) >> (sh & 31)) & 0xFFFFFFFF
+
+
+def window_lsb(s, pos):
+    p, sh = pos >> 4, 2 * (pos & 15)
+    assert 0 <= p and p + 2 < len(s), (pos, len(s))
+    return _alignbit(s[p + 1], s[p], sh), _alignbit(s[p + 2], s[p + 1], sh)
+
+
+def match32(a, b):
+    x = ((a[1] ^ b[1]) << 32) | (a[0] ^ b[0])
+    return ((x & -x).bit_length() - 1) >> 1 if x else 32
+
+
+def extend_lsb(qs, ql, rs, rl, row, col):
+    while row < ql and col < rl:
+        n = match32(window_lsb(qs, row), window_lsb(rs, col))
+        lim = min(ql - row, rl - col)
+        if n >= lim:
+            return row + lim
+        row += n
+        col += n
+        if n < 32:
+            break
+    return row
+
+
+def wfa_team_fast(q, r, x, o, e, me, U=4, NT=256):
+    """wfa_pair_team_fast restated on flat arrays with its own index
+    arithmetic: guard entries around every row, I and D of a diagonal side by
+    side, ring slots read before their score exists (no existence test),
+    wavefront-wide groups of which lanes past `hi` read whatever their
+    addresses hold and write nothing, LSB-first 32-base windows with clamped
+    positions.  Every read must stay inside the layout, every stored D must
+    be valid or exactly the invalid value."""
+    ql, rl = len(q), len(r)
+    if ql == 0 and rl == 0: return 0
+    target = ql - rl
+    g = math.gcd(math.gcd(x, o), e)
+    R = lambda s: (s - o) // e if s >= o + e else 0
+    K = min(me, max(ql, rl), R(me))
+    W = 2 * K + 1
+    if target != 0 and abs(target) > K: return -1
+    nM, nI = max(x, o + e) // g + 1, e // g + 1
+    dx, dO, de = x // g, (o + e) // g, e // g
+    qs, rs = lsb_words(q), lsb_words(r)
+    WP = W + 2
+    m_ents = (nM * WP + 1) & ~1
+    size = m_ents + 2 * nI * WP + FAST_SLACK // 2
+    L = [NEG16] * size
+    IDb = m_ents
+    L[K + 1] = extend_lsb(qs, ql, rs, rl, 0, 0)
+    if target == 0 and L[K + 1] >= ql: return 0
+    back = lambda slot, d, n: slot - d if slot >= d else slot - d + n  # noqa: E731
+    si = ms = is_ = Rr = 0
+    for s in range(g, me + 1, g):
+        si += 1
+        ms = 0 if ms + 1 == nM else ms + 1
+        is_ = 0 if is_ + 1 == nI else is_ + 1
+        while (Rr + 1) * e + o <= s:
+            Rr += 1
+        lo, hi = max(-rl, -me, -Rr), min(ql, me, Rr)
+        Mo, Mx = back(ms, dO, nM) * WP, back(ms, dx, nM) * WP
+        IDe = IDb + back(is_, de, nI) * 2 * WP
+        Ms, IDs = ms * WP, IDb + is_ * 2 * WP
+        found = False
+        for wave0 in range(0, NT, 64):
+            kb = lo + wave0
+            while kb <= hi:
+                ng = min(U, (hi - kb) // NT + 1)
+                mv, mr = {}, {}
+                for u in range(ng):
+                    for lane in range(64):
+                        di = kb + lane + K + u * NT
+                        idx = [Mo + di, Mo + di + 2, IDe + 2 * di, IDe + 2 * di + 5, Mx + di + 1]
+                        assert all(0 <= i < size for i in idx), (di, size)
+                        a, b, c, d, m = (L[i] for i in idx)
+                        iv, dv = max(a, c) + 1, max(b, d)
+                        mv[u, lane] = max(m + 1, iv, dv)
+                        if di - K <= hi:
+                            assert dv >= 0 or dv == NEG16
+                            L[IDs + 2 * di + 2] = NEG16 if iv < 0 else iv
+                            L[IDs + 2 * di + 3] = dv
+                for u in range(ng):
+                    for lane in range(64):
+                        k = kb + lane + u * NT
+                        row = mv[u, lane]
+                        col = row - k
+                        wq = window_lsb(qs, min(max(row, 0), ql))
+                        wr = window_lsb(rs, min(max(col, 0), rl))
+                        lim = min(ql - row, rl - col)
+                        n = min(match32(wq, wr), max(lim, 0))
+                        mr[u, lane] = row + n
+                        if row >= 0 and n == 32 and lim > 32 and k <= hi:
+                            mr[u, lane] = extend_lsb(qs, ql, rs, rl, row + 32, col + 32)
+                for u in range(ng):
+                    for lane in range(64):
+                        k = kb + lane + u * NT
+                        if k <= hi:
+                            row = mv[u, lane]
+                            found |= k == target and row >= 0 and mr[u, lane] >= ql
+                            v = NEG16 if row < 0 else mr[u, lane]
+                            assert -32768 <= v <= 32767
+                            L[Ms + k + K + 1] = v
+                kb += U * NT
+        if found: return s
+    return -1
+
+
+def test_lsb_windows_equal_base_compare():
+    rng = random.Random(5)
+    for _ in range(300):
+        a = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 90)))
+        b = "".join(c if rng.random() > 0.05 else rng.choice("ACGT") for c in a)
+        qs, rs = lsb_words(a), lsb_words(b)
+        for i in range(len(a)):
+            j = rng.randint(0, len(b) - 1)
+            n = match32(window_lsb(qs, i), window_lsb(rs, j))
+            t = 0
+            while t < 32 and i + t < len(a) and j + t < len(b) and a[i + t] == b[j + t]:
+                t += 1
+            lim = min(len(a) - i, len(b) - j, 32)
+            # bases past an end are zero bits (code of A): only the capped
+            # count has to agree
+            assert min(n, lim) == t, (a, b, i, j)
+
+
+def test_fast_team_layout_equals_oracle():
+    rng = random.Random(6)
+    pens = [(4, 6, 2), (1, 1, 1), (3, 2, 1), (5, 0, 3), (2, 7, 1), (6, 4, 2), (9, 3, 3)]
+    for it in range(500):
+        a = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 120)))
+        if rng.random() < 0.3:
+            b = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 60)))
+        else:
+            div = rng.choice([0.0, 0.01, 0.05, 0.15])
+            b = "".join(c if rng.random() > div else rng.choice("ACGT")
+                        for c in a if rng.random() > div / 2)
+        x, o, e = rng.choice(pens)
+        me = rng.choice([0, 2, 5, 12, 30, 80, 200])
+        U, NT = (4, 256) if it % 2 else (2, 64)
+        assert wfa_team_fast(a, b, x, o, e, me, U, NT) == O.wfa_affine(a, b, x, o, e, me), \
+            (a, b, x, o, e, me, U, NT)
+
+
 def wfa_cigar_sched(q, r, x, o, e, me):
     """The traceback schedule of the CIGAR kernel: the cost first (any exact
     schedule), then a history of every score step 0..cost/g, each row the
