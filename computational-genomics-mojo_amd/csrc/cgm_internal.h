@@ -146,9 +146,15 @@ int cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen,
 int64_t mix_plan_capacity(const MixPlan *P, int32_t *me_cap, WfaPen *pen);
 
 
-// WFA layout: LDS budget per 256-thread workgroup and workgroups per CU.
+// WFA workgroup-per-pair layout: pairs whose branch-free LDS layout fits
+// kWfaSmallLdsBytes run on 256-thread workgroups with that budget,
+// kWfaSmallBlocksPerCu per CU; the rest on 512-thread workgroups with
+// kWfaLdsBytes, kWfaBlocksPerCu per CU (twice the wavefronts per pair keeps
+// the CU as full as the small layout does).
 constexpr int kWfaLdsBytes = 52 * 1024;
 constexpr int kWfaBlocksPerCu = 3;
+constexpr int kWfaSmallLdsBytes = 20 * 1024;
+constexpr int kWfaSmallBlocksPerCu = 8;
 // Largest diagonal |k| reachable at score s: (s - o) / e from o + e on.
 inline int reach_host(int s, const WfaPen &p) { return s >= p.o + p.e ? (s - p.o) / p.e : 0; }
 

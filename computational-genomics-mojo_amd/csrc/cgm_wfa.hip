@@ -150,22 +150,29 @@ __global__ __launch_bounds__(kPlainThreads) void wfa_plain_kernel(Batch b, const
     }
 }
 
-constexpr int kTeamThreads = 256;
-constexpr int kTeamUnroll = 4;  // diagonals per thread and pass of a score
+constexpr int kTeamThreads = 256;     // workgroups of the small-LDS pairs
+constexpr int kTeamThreadsBig = 512;  // workgroups of the other pairs
+constexpr int kTeamUnroll = 4;        // diagonal groups per wavefront and pass of a score
 
-__global__ __launch_bounds__(kTeamThreads) void wfa_wave_kernel(
+// Persistent workgroups take items of `order` from the atomic counter
+// `queue`: all items (part 0), the items before *split (part 1) or those
+// from *split on (part 2).
+template <int NT>
+__global__ __launch_bounds__(NT) void wfa_wave_kernel(
     Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap, WfaPen p, WfaGeom gm,
-    int32_t *out, const int32_t *order, int32_t *queue, int32_t lds_bytes, int32_t *slab,
-    int64_t slab_words) {
+    int32_t *out, const int32_t *order, const int32_t *split, int32_t part, int32_t *queue,
+    int32_t lds_bytes, int32_t *slab, int64_t slab_words) {
     extern __shared__ uint8_t lds_raw[];
     __shared__ int next_item;
-    int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
+    const int64_t first = part == 2 ? *split : 0;
+    const int64_t last = part == 1 ? *split : b.n_pairs;
+    int32_t *my_slab = slab ? slab + (int64_t)blockIdx.x * slab_words : nullptr;
     for (;;) {
         if (threadIdx.x == 0) next_item = atomicAdd(queue, 1);
         __syncthreads();
-        const int64_t item = next_item;
+        const int64_t item = first + next_item;
         __syncthreads();
-        if (item >= b.n_pairs) break;
+        if (item >= last) break;
         const int64_t pair = order ? order[item] : item;
         const uint64_t *q, *r;
         int qlen, rlen;
@@ -173,24 +180,36 @@ __global__ __launch_bounds__(kTeamThreads) void wfa_wave_kernel(
         const int me = me_arr ? me_arr[pair] : me_scalar;
         const int res = (!ok || me < 0 || me > me_cap)
                             ? CGM_BAD_PAIR
-                            : wfa_pair_block<kTeamUnroll, kTeamThreads>(q, qlen, r, rlen, me, p, gm, lds_raw,
-                                                          lds_bytes, my_slab, slab_words);
+                            : wfa_pair_block<kTeamUnroll, NT>(q, qlen, r, rlen, me, p, gm, lds_raw,
+                                                              lds_bytes, my_slab, slab_words);
         if (threadIdx.x == 0) out[pair] = res;
         __syncthreads();
     }
 }
 
 // WFA work key for the longest-first order: sum over scores of the
-// diagonal count, from max_error (an upper bound of the cost).
-__global__ void wfa_key_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p, int g,
-                               uint32_t *keys, int32_t *idx) {
+// diagonal count, from max_error (an upper bound of the cost), with bit 31
+// set for pairs that need the large layout (they come first).  *nbig counts
+// those pairs.
+__global__ void wfa_key_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap,
+                               WfaPen p, WfaGeom gm, int32_t small_lds, uint32_t *keys,
+                               int32_t *idx, int32_t *nbig) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= b.n_pairs) return;
     const int me = me_arr ? me_arr[i] : me_scalar;
     const int64_t R = me >= p.o + p.e ? (me - p.o) / p.e : 0;
-    const int64_t steps = (me > 0 ? me / g : 0) * (2 * R + 1);
-    keys[i] = (uint32_t)min(steps, (int64_t)0xffffffff);
+    const int64_t steps = (me > 0 ? me / gm.g : 0) * (2 * R + 1);
+    const int ql = b.q_len[i], rl = b.r_len[i];
+    bool small = false;
+    if (me >= 0 && me <= me_cap && ql >= 0 && rl >= 0) {
+        const int K = min(min(me, max(ql, rl)), reach(me, p));
+        small = rows_fit_int16(ql, me, p) && team_fast_bytes(ql, rl, 2 * K + 1, gm) <= small_lds;
+    }
+    keys[i] = (small ? 0u : 0x80000000u) | (uint32_t)min(steps, (int64_t)0x7fffffff);
     idx[i] = (int32_t)i;
+    const uint64_t m = __ballot(!small);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
+        atomicAdd(nbig, __popcll(m));
 }
 
 __global__ void zero_i32_kernel(int32_t *p, int n) {
@@ -275,10 +294,14 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
     if (grid > b.n_pairs) grid = b.n_pairs;
     // keep the HBM slabs within 4 GiB
     while (grid > 1 && grid * slab_words * 4 > (int64_t(4) << 30)) grid /= 2;
+    int64_t grid_small = (int64_t)cus * kWfaSmallBlocksPerCu;
+    if (grid_small > b.n_pairs) grid_small = b.n_pairs;
     static bool attr_set = false;
     if (!attr_set) {
-        CGM_HIP(hipFuncSetAttribute((const void *)wfa_wave_kernel,
+        CGM_HIP(hipFuncSetAttribute((const void *)wfa_wave_kernel<kTeamThreadsBig>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kWfaLdsBytes));
+        CGM_HIP(hipFuncSetAttribute((const void *)wfa_wave_kernel<kTeamThreads>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kWfaSmallLdsBytes));
         attr_set = true;
     }
     int32_t *slab = nullptr, *queue = nullptr, *own_order = nullptr;
@@ -286,22 +309,36 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
     int32_t *idx = nullptr;
     CGM_HIP(hipMallocAsync((void **)&queue, 64, stream));
     CGM_HIP(hipMallocAsync((void **)&slab, (size_t)(grid * slab_words * 4), stream));
+    // queue[0] / queue[1]: counters of the two launches; queue[2]: pairs
+    // of the large layout
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, queue, 16);
     int rc = CGM_OK;
     if (!order) {
-        // longest first: a late long pair would otherwise leave the GPU idle
+        // Longest first (a late long pair would otherwise leave the GPU
+        // idle), the large-layout pairs ahead of the others: they run first,
+        // on 512-thread workgroups, then the rest on small ones.
         CGM_HIP(hipMallocAsync((void **)&own_order, (size_t)b.n_pairs * 4, stream));
         CGM_HIP(hipMallocAsync((void **)&keys, (size_t)b.n_pairs * 4, stream));
         CGM_HIP(hipMallocAsync((void **)&idx, (size_t)b.n_pairs * 4, stream));
         hipLaunchKernelGGL(wfa_key_kernel, dim3((unsigned)((b.n_pairs + 255) / 256)), dim3(256), 0,
-                           stream, b, max_error, max_error_scalar, p, gm.g, keys, idx);
+                           stream, b, max_error, max_error_scalar, me_max, p, gm,
+                           (int32_t)kWfaSmallLdsBytes, keys, idx, queue + 2);
         rc = sort_desc_by_key(keys, idx, own_order, b.n_pairs, stream);
-        order = own_order;
-    }
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, queue, 16);
-    if (rc == CGM_OK) {
-        hipLaunchKernelGGL(wfa_wave_kernel, dim3((unsigned)grid), dim3(kTeamThreads),
-                           (size_t)kWfaLdsBytes, stream, b, max_error, max_error_scalar, me_max, p,
-                           gm, out, order, queue, kWfaLdsBytes, slab, slab_words);
+        if (rc == CGM_OK) {
+            hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreadsBig>, dim3((unsigned)grid),
+                               dim3(kTeamThreadsBig), (size_t)kWfaLdsBytes, stream, b, max_error,
+                               max_error_scalar, me_max, p, gm, out, own_order, queue + 2, 1,
+                               queue, kWfaLdsBytes, slab, slab_words);
+            hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreads>, dim3((unsigned)grid_small),
+                               dim3(kTeamThreads), (size_t)kWfaSmallLdsBytes, stream, b,
+                               max_error, max_error_scalar, me_max, p, gm, out, own_order,
+                               queue + 2, 2, queue + 1, kWfaSmallLdsBytes, nullptr, (int64_t)0);
+        }
+    } else {
+        hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreadsBig>, dim3((unsigned)grid),
+                           dim3(kTeamThreadsBig), (size_t)kWfaLdsBytes, stream, b, max_error,
+                           max_error_scalar, me_max, p, gm, out, order, nullptr, 0, queue,
+                           kWfaLdsBytes, slab, slab_words);
     }
     hipError_t le = hipGetLastError();
     CGM_HIP(hipFreeAsync(slab, stream));

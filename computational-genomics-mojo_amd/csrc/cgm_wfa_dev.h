@@ -311,8 +311,14 @@ __device__ int wfa_pair_team_fast(const uint64_t *qg, int qlen, const uint64_t *
         uint32_t *w32 = (uint32_t *)Mb;
         for (int i = tid; i < words; i += NT) w32[i] = 0x80008000u;
     }
-    __shared__ int done_at;  // score at which the target was reached
-    if (tid == 0) done_at = -1;
+    // done_at[si & 1]: score step si that reached the target, or -1.  The
+    // flag of step si is read after that step's barrier and acted on after
+    // the next one, so the read's latency hides under the next step; one
+    // extra step is computed, harmlessly.  Two slots keep the read of step
+    // si apart from writes of step si + 1 (a wavefront may run ahead into
+    // that step before another has read the flag).
+    __shared__ int done_at[2];
+    if (tid == 0) done_at[0] = done_at[1] = -1;
     __syncthreads();
     if (tid == 0) Mb[K + 1] = (int16_t)extend_lsb(qs, qlen, rs, rlen, 0, 0);
     __syncthreads();
@@ -322,6 +328,7 @@ __device__ int wfa_pair_team_fast(const uint64_t *qg, int qlen, const uint64_t *
     const int wave0 = __builtin_amdgcn_readfirstlane(tid & ~63);
     WfaSteps st;
     st.init(p, gm);
+    int seen = -1;  // done_at of the previous step
     for (int s = gm.g; s <= me; s += gm.g) {
         st.next(p, gm);
         const int lo = max(max(-rlen, -me), -st.R), hi = min(min(qlen, me), st.R);
@@ -341,16 +348,18 @@ __device__ int wfa_pair_team_fast(const uint64_t *qg, int qlen, const uint64_t *
             for (int u = 0; u < U; ++u) {
                 if (u >= ng) break;
                 const int di = di0 + u * NT;
+                // M first: every read has returned before the store below,
+                // so nothing after it waits on the store
+                const int m = Mx[di + 1];
+                __builtin_amdgcn_sched_barrier(0);  // keep the M read first
                 const int a = Mo[di], b = Mo[di + 2];
                 const int c = IDe[2 * di], d = IDe[2 * di + 5];
-                const int m = Mx[di + 1];
                 const int iv = max(a, c) + 1;
                 const int dv = max(b, d);  // negative only when both are `neg`
                 mv[u] = max(max(m + 1, iv), dv);
-                if (di - K <= hi) {
-                    IDs[2 * di + 2] = iv < 0 ? neg : (int16_t)iv;
-                    IDs[2 * di + 3] = (int16_t)dv;
-                }
+                if (di - K <= hi)  // I and D of diagonal k in one 32-bit store
+                    *(uint32_t *)(IDs + 2 * di + 2) =
+                        (uint32_t)(uint16_t)(iv < 0 ? neg : (int16_t)iv) | ((uint32_t)dv << 16);
             }
             // EXTEND, first 32 bases of every group: no branch
             uint64_t more = 0;  // bit u: some lane of group u matched all 32
@@ -389,12 +398,12 @@ __device__ int wfa_pair_team_fast(const uint64_t *qg, int qlen, const uint64_t *
                 }
             }
         }
-        // One barrier ends the score (see wfa_pair_team).
-        if (found) done_at = s;
+        if (found) done_at[st.si & 1] = s;
         __syncthreads();
-        if (done_at == s) return s;
+        if (seen >= 0) return seen;
+        seen = done_at[st.si & 1];
     }
-    return -1;
+    return seen;
 }
 
 // Trapezoid tiling of the team layout for pairs whose wavefronts exceed
