@@ -29,6 +29,7 @@
 // forward formula.
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -39,7 +40,17 @@ namespace cgm {
 
 namespace {
 
-constexpr int kCigarThreads = 256;
+constexpr int kCigarThreads = 512;
+constexpr int kCigarUnroll = 4;  // diagonal groups per wavefront and pass of a score
+
+// Per-device history workspace kept between calls (see cgm_wfa_cigar_launch).
+constexpr int kCigarMaxDevices = 64;
+struct CigarWorkspace {
+    std::mutex mu;
+    uint8_t *ptr = nullptr;
+    size_t bytes = 0;
+};
+CigarWorkspace g_cigar_ws[kCigarMaxDevices];
 constexpr int kCigarLds = 52 * 1024;  // ring + sequences per workgroup (3 per CU)
 constexpr int64_t kCigarBudgetMin = int64_t(8) << 30;
 constexpr int64_t kCigarBudgetMax = int64_t(96) << 30;
@@ -303,6 +314,123 @@ __device__ void cigar_forward_ring(const uint64_t *q, int qlen, const uint64_t *
     }
 }
 
+// Forward pass in the branch-free LDS layout of the cost kernels
+// (wfadev::wfa_pair_team_fast: LSB-first sequence words, guard-padded int16
+// ring rows, I and D side by side), writing every step's range to the int16
+// HBM history rows as cigar_forward_ring does.  `lds` holds at least
+// wfadev::team_fast_bytes(qlen, rlen, W, gm).
+template <int U, int NT>
+__device__ void cigar_forward_fast(const uint64_t *qg, int qlen, const uint64_t *rg, int rlen,
+                                   int me, int cost, const WfaPen &p, const wfadev::WfaGeom &gm,
+                                   int16_t *M, int16_t *I, int16_t *D, int K, int W, uint8_t *lds) {
+    using namespace wfadev;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int nq = 2 * qw + 4, nr = 2 * rw + 4;
+    uint32_t *qs = (uint32_t *)lds;
+    uint32_t *rs = qs + nq;
+    for (int i = tid; i < nq + nr; i += NT) {
+        const bool isq = i < nq;
+        const int j = isq ? i : i - nq;
+        uint32_t v = 0;
+        if ((j >> 1) < (isq ? qw : rw)) {
+            const uint64_t w = (isq ? qg : rg)[j >> 1];
+            v = __builtin_bitreverse32((j & 1) ? (uint32_t)w : (uint32_t)(w >> 32));
+        }
+        qs[i] = v;
+    }
+    const int WP = W + 2;
+    const int m_ents = (gm.nM * WP + 1) & ~1;
+    int16_t *Mb = (int16_t *)(lds + 4 * (nq + nr));
+    int16_t *IDb = Mb + m_ents;
+    const int16_t neg = RowNeg<int16_t>::v;
+    {
+        const int words = (m_ents + 2 * gm.nI * WP + kFastSlack / 2) >> 1;
+        uint32_t *w32 = (uint32_t *)Mb;
+        for (int i = tid; i < words; i += NT) w32[i] = 0x80008000u;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int16_t v = (int16_t)extend_lsb(qs, qlen, rs, rlen, 0, 0);
+        Mb[K + 1] = v;
+        M[K] = v;
+        I[K] = neg;
+        D[K] = neg;
+    }
+    __syncthreads();
+    const int wave0 = __builtin_amdgcn_readfirstlane(tid & ~63);
+    WfaSteps st;
+    st.init(p, gm);
+    const int nrows = cost / gm.g + 1;
+    for (int si = 1; si < nrows; ++si) {
+        st.next(p, gm);
+        const int lo = max(max(-rlen, -me), -st.R), hi = min(min(qlen, me), st.R);
+        const int16_t *Mo = Mb + WfaSteps::back(st.ms, st.dO, gm.nM) * WP;
+        const int16_t *Mx = Mb + WfaSteps::back(st.ms, st.dx, gm.nM) * WP;
+        const int16_t *IDe = IDb + WfaSteps::back(st.is, st.de, gm.nI) * 2 * WP;
+        int16_t *Ms = Mb + st.ms * WP;
+        int16_t *IDs = IDb + st.is * 2 * WP;
+        int16_t *Mr = M + (int64_t)si * W, *Ir = I + (int64_t)si * W, *Dr = D + (int64_t)si * W;
+        for (int kb = lo + wave0; kb <= hi; kb += U * NT) {
+            const int di0 = kb + lane + K;  // row entry di + 1 holds diagonal di - K
+            int mv[U], mr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (kb + u * NT > hi) break;
+                const int di = di0 + u * NT;
+                const int m = Mx[di + 1];
+                __builtin_amdgcn_sched_barrier(0);  // keep the M read first
+                const int a = Mo[di], b = Mo[di + 2];
+                const int c = IDe[2 * di], d = IDe[2 * di + 5];
+                const int iv = max(a, c) + 1;
+                const int dv = max(b, d);  // negative only when both are `neg`
+                mv[u] = max(max(m + 1, iv), dv);
+                if (di - K <= hi) {
+                    const int16_t ti = iv < 0 ? neg : (int16_t)iv;
+                    *(uint32_t *)(IDs + 2 * di + 2) = (uint32_t)(uint16_t)ti | ((uint32_t)dv << 16);
+                    Ir[di] = ti;
+                    Dr[di] = (int16_t)dv;
+                }
+            }
+            uint64_t more = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (kb + u * NT > hi) break;
+                const int k = di0 + u * NT - K;
+                const int row = mv[u], col = row - k;
+                uint32_t a0, a1, b0, b1;
+                window_lsb(qs, min(max(row, 0), qlen), a0, a1);
+                window_lsb(rs, min(max(col, 0), rlen), b0, b1);
+                const int lim = min(qlen - row, rlen - col);
+                const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
+                mr[u] = row + n;
+                if (__ballot(row >= 0 && n == 32 && lim > 32 && k <= hi)) more |= 1ull << u;
+            }
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!(more >> u & 1)) continue;
+                    const int k = di0 + u * NT - K;
+                    const int row = mv[u], col = row - k;
+                    if (row >= 0 && mr[u] == row + 32 && min(qlen - row, rlen - col) > 32 && k <= hi)
+                        mr[u] = extend_lsb(qs, qlen, rs, rlen, row + 32, col + 32);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (kb + u * NT > hi) break;
+                const int di = di0 + u * NT;
+                if (di - K <= hi) {
+                    const int16_t tm = mv[u] < 0 ? neg : (int16_t)mr[u];
+                    Ms[di + 1] = tm;
+                    Mr[di] = tm;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Forward pass with full history, one workgroup per pair; rows of score
 // step si hold diagonals -K..K (entries outside the step's range invalid).
 template <typename T>
@@ -378,6 +506,20 @@ __global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
     }
     const int me = me_arr ? me_arr[pair] : me_scalar;
     const CigarGeom cg = cigar_geom(qlen, rlen, me, cost, p, g);
+    uint8_t *h = ws + hist_off[pair];
+    const int64_t plane = cg.rows * cg.W;
+    uint32_t *out = cigar + cigar_off[pair];
+    const int64_t cap = cigar_off[pair + 1] - cigar_off[pair];
+    if (cg.esize == 2 && wfadev::team_fast_bytes(qlen, rlen, (int)cg.W, gm) <= kCigarLds) {
+        int16_t *M = (int16_t *)h;
+        cigar_forward_fast<kCigarUnroll, kCigarThreads>(qg, qlen, rg, rlen, me, cost, p, gm, M,
+                                                        M + plane, M + 2 * plane, (int)cg.K,
+                                                        (int)cg.W, (uint8_t *)lds);
+        if (threadIdx.x == 0)
+            cigar_len[pair] = cigar_backtrace<int16_t>(M, M + plane, M + 2 * plane, cg.K, cg.W,
+                                                       qlen, rlen, me, cost, p, g, out, cap);
+        return;
+    }
     const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
     const uint64_t *q = qg, *r = rg;
     // LDS: the ring when it fits, then the sequences when they fit after it
@@ -392,10 +534,6 @@ __global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
         r = seq + qw;
         __syncthreads();
     }
-    uint8_t *h = ws + hist_off[pair];
-    const int64_t plane = cg.rows * cg.W;
-    uint32_t *out = cigar + cigar_off[pair];
-    const int64_t cap = cigar_off[pair + 1] - cigar_off[pair];
     if (cg.esize == 2) {
         int16_t *M = (int16_t *)h;
         if (ring)
@@ -466,10 +604,21 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
     for (int64_t i = 0; i < n_pairs; ++i) order[(size_t)i] = (int32_t)i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t a, int32_t c) { return bytes[(size_t)a] > bytes[(size_t)c]; });
+    // The history workspace stays allocated per device between calls: it
+    // is tens of GB for thousands of kb-long pairs, and a fresh allocation
+    // of that size on every call costs more than the alignment once the
+    // driver has to hand out recycled memory.  It grows when a call needs
+    // more; calls on one device take turns on it.
+    int dev = 0;
+    if ((err = hipGetDevice(&dev)) != hipSuccess) return hip_error(err, "cigar: device");
+    if (dev < 0 || dev >= kCigarMaxDevices) return set_error(CGM_ERR_INVALID, "cigar: device index");
+    CigarWorkspace &cw = g_cigar_ws[dev];
+    std::lock_guard<std::mutex> ws_lock(cw.mu);
     size_t free_b = 0, total_b = 0;
     int64_t budget = kCigarBudgetMin;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-        budget = std::min(kCigarBudgetMax, std::max(kCigarBudgetMin, (int64_t)(free_b / 2)));
+        budget = std::min(kCigarBudgetMax,
+                          std::max(kCigarBudgetMin, (int64_t)((free_b + cw.bytes) / 2)));
     std::vector<std::pair<int64_t, int64_t>> chunks;
     int64_t ws_need = 0;
     for (int64_t i = 0; i < n_pairs;) {
@@ -486,13 +635,23 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
     // workspaces first, uploads after: every exit below waits for the stream,
     // so the host vectors outlive their asynchronous copies
     int32_t *d_order = nullptr;
-    uint8_t *ws = nullptr;
     int rc = CGM_OK;
+    if ((size_t)ws_need > cw.bytes) {
+        // the previous user of the workspace synchronised before returning
+        if (cw.ptr) (void)hipFree(cw.ptr);
+        cw.ptr = nullptr;
+        cw.bytes = 0;
+        if ((err = hipMalloc((void **)&cw.ptr, (size_t)std::max<int64_t>(ws_need, 256))) !=
+            hipSuccess) {
+            cw.ptr = nullptr;
+            return hip_error(err, "cigar: workspace");
+        }
+        cw.bytes = (size_t)std::max<int64_t>(ws_need, 256);
+    }
+    uint8_t *ws = cw.ptr;
     if ((err = hipMallocAsync((void **)&d_order, (size_t)n_pairs * 4, stream)) != hipSuccess ||
-        (err = hipMallocAsync((void **)&d_off, (size_t)n_pairs * 8, stream)) != hipSuccess ||
-        (err = hipMallocAsync((void **)&ws, (size_t)std::max<int64_t>(ws_need, 256), stream)) !=
-            hipSuccess)
-        rc = hip_error(err, "cigar: workspace");
+        (err = hipMallocAsync((void **)&d_off, (size_t)n_pairs * 8, stream)) != hipSuccess)
+        rc = hip_error(err, "cigar: order buffers");
     if (!rc && (err = hipMemcpyAsync(d_order, order.data(), (size_t)n_pairs * 4,
                                      hipMemcpyHostToDevice, stream)) != hipSuccess)
         rc = hip_error(err, "cigar: order");
@@ -513,7 +672,6 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
     // the host copies of `off` and `order` must outlive the asynchronous uploads
     err = hipStreamSynchronize(stream);
     if (!rc && err != hipSuccess) rc = hip_error(err, "cigar: synchronize");
-    if (ws) (void)hipFreeAsync(ws, stream);
     if (d_off) (void)hipFreeAsync(d_off, stream);
     if (d_order) (void)hipFreeAsync(d_order, stream);
     return rc;

@@ -171,8 +171,11 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
  * A CIGAR of cost c has at most 2 c / min(x, o + e) + 1 runs.  Computes
  * the costs first, then re-runs each pair with its whole wavefront history
  * in HBM (longest pairs first, in chunks of at most half the free device
- * memory, 8-96 GiB) and walks it back; allocates its
- * workspace on `stream` and synchronises with the host before returning. */
+ * memory, 8-96 GiB) and walks it back; synchronises with the host before
+ * returning.  The history workspace (the largest one a call has needed so
+ * far) stays allocated per device between calls, so repeated calls do not
+ * pay for a tens-of-GB allocation each; calls on one device take turns on
+ * it. */
 int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                          const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
                          const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
