@@ -536,6 +536,179 @@ __device__ int wfa_pair_tiled(const uint64_t *q, int qlen, const uint64_t *r, in
     return -1;
 }
 
+// Tile width (diagonals, even) of the branch-free tiled layout in
+// `lds_bytes`: LSB-first sequence words, then nM M rows and nI I/D rows
+// (I and D of a diagonal side by side) of LW entries, then kFastSlack bytes.
+__device__ __forceinline__ int tiled_fast_width(int qlen, int rlen, const WfaGeom &gm,
+                                                int lds_bytes) {
+    const int64_t seq = 4 * (2 * (int64_t)((qlen + 31) >> 5) + 4 + 2 * (int64_t)((rlen + 31) >> 5) + 4);
+    const int64_t left = lds_bytes - seq - kFastSlack;
+    return left > 0 ? (int)((left / (2 * (gm.nM + 2 * gm.nI))) & ~(int64_t)1) : 0;
+}
+
+// The trapezoid tiling of wfa_pair_tiled (same blocks of TB score steps,
+// same tiles, same slab sets and copies) with the tile in the branch-free
+// layout of wfa_pair_team_fast.  A step reads diagonals k - 1 .. k + 1 of its
+// range only, which stays inside [base, base + LW), so tile rows need no
+// guards; lanes of a group past the step's range read into the next row or
+// the slack and write nothing.
+template <int U, int NT>
+__device__ int wfa_pair_tiled_fast(const uint64_t *qg, int qlen, const uint64_t *rg, int rlen,
+                                   int me, const WfaPen &p, const WfaGeom &gm, int16_t *slab,
+                                   int W, int K, uint8_t *lds, int LW, int TB) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int nq = 2 * qw + 4, nr = 2 * rw + 4;
+    uint32_t *qs = (uint32_t *)lds;
+    uint32_t *rs = qs + nq;
+    for (int i = tid; i < nq + nr; i += NT) {
+        const bool isq = i < nq;
+        const int j = isq ? i : i - nq;
+        uint32_t v = 0;
+        if ((j >> 1) < (isq ? qw : rw)) {
+            const uint64_t w = (isq ? qg : rg)[j >> 1];
+            v = __builtin_bitreverse32((j & 1) ? (uint32_t)w : (uint32_t)(w >> 32));
+        }
+        qs[i] = v;
+    }
+    int16_t *Mt = (int16_t *)(lds + 4 * (nq + nr));  // nM rows of LW
+    int16_t *IDt = Mt + gm.nM * LW;                     // nI rows of 2 LW
+    const int16_t neg = RowNeg<int16_t>::v;
+    const int slots = gm.nM + 2 * gm.nI;
+    const int64_t set_len = (int64_t)slots * W;
+    for (int64_t i = tid; i < 2 * set_len; i += NT) slab[i] = neg;
+    for (int i = tid; i < kFastSlack / 2; i += NT) IDt[2 * gm.nI * LW + i] = neg;
+    __syncthreads();
+    const int target = qlen - rlen;
+    __shared__ int done_at;  // score at which the target was reached
+    if (tid == 0) {
+        slab[K] = (int16_t)extend_lsb(qs, qlen, rs, rlen, 0, 0);
+        done_at = -1;
+    }
+    __syncthreads();
+    if (target == 0 && (int)slab[K] >= qlen) return 0;
+    const int wave0 = __builtin_amdgcn_readfirstlane(tid & ~63);
+    const int TW = LW - 2 * TB;
+    WfaSteps blk;  // state at the start of the current block of scores
+    blk.init(p, gm);
+    int cur = 0;
+    for (int s0 = 0; s0 + gm.g <= me;) {
+        const int nsteps = min(TB, (me - s0) / gm.g);
+        const int s_end = s0 + nsteps * gm.g;
+        const int Re = reach(s_end, p);
+        const int lo_e = max(max(-rlen, -me), -Re), hi_e = min(min(qlen, me), Re);
+        const int16_t *src = slab + cur * set_len;
+        int16_t *dst = slab + (cur ^ 1) * set_len;
+        for (int a = lo_e; a <= hi_e; a += TW) {
+            const int b = min(a + TW - 1, hi_e);
+            const int base = a - TB;  // diagonal of local index 0
+            for (int sl = 0; sl < gm.nM; ++sl)
+                for (int j = tid; j < LW; j += NT) {
+                    const int k = base + j;
+                    Mt[sl * LW + j] = (k >= -K && k <= K) ? src[(int64_t)sl * W + k + K] : neg;
+                }
+            for (int sl = 0; sl < gm.nI; ++sl)
+                for (int j = tid; j < LW; j += NT) {
+                    const int k = base + j;
+                    const bool in = k >= -K && k <= K;
+                    const int16_t iv = in ? src[(int64_t)(gm.nM + sl) * W + k + K] : neg;
+                    const int16_t dv = in ? src[(int64_t)(gm.nM + gm.nI + sl) * W + k + K] : neg;
+                    *(uint32_t *)(IDt + sl * 2 * LW + 2 * j) =
+                        (uint32_t)(uint16_t)iv | ((uint32_t)(uint16_t)dv << 16);
+                }
+            __syncthreads();
+            WfaSteps st = blk;
+            for (int t = 1; t <= nsteps; ++t) {
+                const int s = s0 + t * gm.g;
+                st.next(p, gm);
+                const int shrink = nsteps - t;
+                const int klo = max(max(max(-rlen, -me), -st.R), a - shrink);
+                const int khi = min(min(min(qlen, me), st.R), b + shrink);
+                const int16_t *Mo = Mt + WfaSteps::back(st.ms, st.dO, gm.nM) * LW;
+                const int16_t *Mx = Mt + WfaSteps::back(st.ms, st.dx, gm.nM) * LW;
+                const int16_t *IDe = IDt + WfaSteps::back(st.is, st.de, gm.nI) * 2 * LW;
+                int16_t *Ms = Mt + st.ms * LW;
+                int16_t *IDs = IDt + st.is * 2 * LW;
+                int found = 0;
+                for (int kb = klo + wave0; kb <= khi; kb += U * NT) {
+                    const int j0 = kb + lane - base;  // local index of the lane's diagonal
+                    int mv[U], mr[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (kb + u * NT > khi) break;
+                        const int j = j0 + u * NT;
+                        const int m = Mx[j];
+                        __builtin_amdgcn_sched_barrier(0);  // keep the M read first
+                        const int a_ = Mo[j - 1], b_ = Mo[j + 1];
+                        const int c = IDe[2 * j - 2], d = IDe[2 * j + 3];
+                        const int iv = max(a_, c) + 1;
+                        const int dv = max(b_, d);  // negative only when both are `neg`
+                        mv[u] = max(max(m + 1, iv), dv);
+                        if (j + base <= khi)
+                            *(uint32_t *)(IDs + 2 * j) =
+                                (uint32_t)(uint16_t)(iv < 0 ? neg : (int16_t)iv) |
+                                ((uint32_t)dv << 16);
+                    }
+                    uint64_t more = 0;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (kb + u * NT > khi) break;
+                        const int k = j0 + u * NT + base;
+                        const int row = mv[u], col = row - k;
+                        uint32_t a0, a1, b0, b1;
+                        window_lsb(qs, min(max(row, 0), qlen), a0, a1);
+                        window_lsb(rs, min(max(col, 0), rlen), b0, b1);
+                        const int lim = min(qlen - row, rlen - col);
+                        const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
+                        mr[u] = row + n;
+                        if (__ballot(row >= 0 && n == 32 && lim > 32 && k <= khi)) more |= 1ull << u;
+                    }
+                    if (more) {
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            if (!(more >> u & 1)) continue;
+                            const int k = j0 + u * NT + base;
+                            const int row = mv[u], col = row - k;
+                            if (row >= 0 && mr[u] == row + 32 && min(qlen - row, rlen - col) > 32 &&
+                                k <= khi)
+                                mr[u] = extend_lsb(qs, qlen, rs, rlen, row + 32, col + 32);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (kb + u * NT > khi) break;
+                        const int j = j0 + u * NT, k = j + base;
+                        if (k <= khi) {
+                            const int row = mv[u];
+                            found |= (k == target && row >= 0 && mr[u] >= qlen);
+                            Ms[j] = row < 0 ? neg : (int16_t)mr[u];
+                        }
+                    }
+                }
+                // the first score that reaches the target ends the pair
+                if (found) done_at = s;
+                __syncthreads();
+                if (done_at == s) return s;
+            }
+            const int wd = b - a + 1;
+            for (int sl = 0; sl < gm.nM; ++sl)
+                for (int j = tid; j < wd; j += NT)
+                    dst[(int64_t)sl * W + a + j + K] = Mt[sl * LW + (a + j - base)];
+            for (int sl = 0; sl < gm.nI; ++sl)
+                for (int j = tid; j < wd; j += NT) {
+                    const uint32_t v = *(const uint32_t *)(IDt + sl * 2 * LW + 2 * (a + j - base));
+                    dst[(int64_t)(gm.nM + sl) * W + a + j + K] = (int16_t)(v & 0xffffu);
+                    dst[(int64_t)(gm.nM + gm.nI + sl) * W + a + j + K] = (int16_t)(v >> 16);
+                }
+            __syncthreads();
+        }
+        for (int t = 0; t < nsteps; ++t) blk.next(p, gm);
+        cur ^= 1;
+        s0 = s_end;
+    }
+    return -1;
+}
+
 // Orders the LDS accesses of one wavefront's lanes: a wavefront's LDS
 // operations complete in issue order, so only the compiler must be kept
 // from moving accesses across this point.
@@ -665,6 +838,10 @@ __device__ __forceinline__ int wfa_pair_block(const uint64_t *qg, int qlen, cons
     const bool fit16 = rows_fit_int16(qlen, me, p);
     if (fit16 && team_fast_bytes(qlen, rlen, W, gm) <= lds_bytes)
         return wfa_pair_team_fast<U, NT>(qg, qlen, rg, rlen, me, p, gm, lds, W, K);
+    const int LWf = tiled_fast_width(qlen, rlen, gm, lds_bytes);
+    if (fit16 && LWf - 2 * tile_steps >= kTileMinWidth && (int64_t)slots * W <= slab_words)
+        return wfa_pair_tiled_fast<U, NT>(qg, qlen, rg, rlen, me, p, gm, (int16_t *)slab, W, K,
+                                          lds, LWf, tile_steps);
     const uint64_t *q = qg, *r = rg;
     if (seq_bytes <= lds_bytes) {
         // Sequences go to LDS whenever they fit, also when the wavefronts
