@@ -294,6 +294,13 @@ int launch_sw_lds_lane(const Batch &b, const int32_t *band, int32_t band_scalar,
     return CGM_OK;
 }
 
+// Bytes of freed stream-ordered allocations the device's default pool keeps
+// for reuse: the launchers allocate their temporaries (sort space, slabs,
+// orders) per call with hipMallocAsync, and a pool that hands everything back
+// at each synchronisation makes every call fetch (recycled, to-be-cleared)
+// memory from the driver again.
+constexpr uint64_t kAsyncPoolKeepBytes = uint64_t(8) << 30;
+
 int device_cu_count(int *cus) {
     static int cached[64] = {0};
     int dev = 0;
@@ -304,6 +311,12 @@ int device_cu_count(int *cus) {
     }
     int n = 0;
     CGM_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    // first use on this device: keep freed async allocations for reuse
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = kAsyncPoolKeepBytes;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     if (dev < 64) cached[dev] = n;
     *cus = n;
     return CGM_OK;
