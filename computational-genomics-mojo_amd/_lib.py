@@ -22,7 +22,7 @@ CGM_BAD_PAIR = -(2 ** 31)
 
 # cgm_tuning fields (CGM_TUNE_* order)
 TUNE_FIELDS = ("blocks_per_cu", "lds_bytes", "tile_steps", "wave_max_width", "reg_max_band",
-               "lane_max_len", "sw_dual")
+               "lane_max_len", "sw_dual", "sw_split")
 
 LAYOUT_AUTO = 0
 LAYOUT_PLAIN = 1

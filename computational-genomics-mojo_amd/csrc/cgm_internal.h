@@ -124,8 +124,9 @@ struct MixTuning {
     int32_t reg_max_band = 64;         // SW bands up to this: register lanes, wider: LDS lanes
     int32_t lane_max_len = 1 << 30;    // SW pairs longer than this: one workgroup per pair
     int32_t sw_dual = 1;               // register SW lanes: dual-row int16 where exact (1) or int32 (0)
+    int32_t sw_split = 55;             // SW-only: percent of the 64-pair units in the long queue
 };
-constexpr int kMixTuneN = 7;
+constexpr int kMixTuneN = 8;
 void tuning_to_array(const MixTuning &t, int32_t *a);
 int tuning_from_array(const int32_t *a, MixTuning *t);  // validates
 MixTuning mix_default_tuning();

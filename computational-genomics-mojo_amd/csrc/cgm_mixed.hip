@@ -291,7 +291,8 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
                       int32_t me_scalar, SwScoring sc, WfaPen p, wfadev::WfaGeom gm, SwOut out,
                       const int32_t *order, const Task *tasks, const int32_t *task_order,
                       const int32_t *n_tasks, int32_t *queue, int32_t lds_bytes,
-                      int32_t tile_steps, int32_t sw_dual, int32_t *slab, int64_t slab_words) {
+                      int32_t tile_steps, int32_t sw_dual, int32_t sw_split, int32_t *slab,
+                      int64_t slab_words) {
     extern __shared__ uint8_t lds_raw[];
     __shared__ int next_task;
     int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
@@ -300,14 +301,16 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
     if constexpr (!WITH_WFA) {
         // SW only: every wavefront takes 64-pair units of the longest-first
         // task list by itself (unit u = part u % 4 of task u / 4), so no
-        // wavefront waits for the others of its workgroup.  Of two
-        // wavefronts on one SIMD the older one gets most issue slots, so the
-        // first wavefront started on a SIMD takes units from the long end of
-        // the list and the second from the short end: the longest units are
-        // never left to a starved wavefront at the end of the launch.  The
-        // two ends meet once every unit is taken (one 64-bit counter:
-        // long-end claims in the low half, short-end claims in the high
-        // half).
+        // wavefront waits for the others of its workgroup.  The list is cut
+        // in two queues, both taken longest first: the first sw_split
+        // percent of the units (the long queue) and the rest (the short
+        // queue).  Of two wavefronts on one SIMD the older one gets most
+        // issue slots, so the first wavefront started on a SIMD takes the
+        // long queue and, once it is empty, the short one; the second takes
+        // only the short queue.  The longest units never go to a starved
+        // wavefront, and the last units of the launch are the shortest (one
+        // 64-bit counter: long-queue claims in the low half, short-queue
+        // claims in the high half).
         const int lane = threadIdx.x & 63;
         const uint32_t units = (uint32_t)total * kMixWaves;
         int32_t *small = queue - kSmallQueue;
@@ -319,13 +322,19 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, con
             const uint32_t key = ((xcc & 7u) << 10) | (((hw >> 8) & 0xffu) << 2) | ((hw >> 4) & 3u);
             slot = atomicAdd(small + kSmallSimdSlots + key, 1);
         }
-        const bool short_end = (__shfl(slot, 0, 64) & 1) != 0;
+        const bool second = (__shfl(slot, 0, 64) & 1) != 0;
+        const uint32_t split = (uint32_t)(((uint64_t)units * (uint32_t)sw_split) / 100u);
         for (;;) {
             int u = -1;
             if (lane == 0) {
-                const unsigned long long old = atomicAdd(ends, short_end ? (1ull << 32) : 1ull);
-                const uint32_t h = (uint32_t)old, tl = (uint32_t)(old >> 32);
-                if ((uint64_t)h + tl < units) u = short_end ? (int)(units - 1 - tl) : (int)h;
+                if (!second) {
+                    const uint32_t a = (uint32_t)atomicAdd(ends, 1ull);
+                    if (a < split) u = (int)a;
+                }
+                if (u < 0) {
+                    const uint32_t c = (uint32_t)(atomicAdd(ends, 1ull << 32) >> 32);
+                    if ((uint64_t)split + c < units) u = (int)(split + c);
+                }
             }
             u = __shfl(u, 0, 64);
             if (u < 0) break;
@@ -445,6 +454,7 @@ void tuning_to_array(const MixTuning &t, int32_t *a) {
     a[4] = t.reg_max_band;
     a[5] = t.lane_max_len;
     a[6] = t.sw_dual;
+    a[7] = t.sw_split;
 }
 
 int tuning_from_array(const int32_t *a, MixTuning *t) {
@@ -456,6 +466,7 @@ int tuning_from_array(const int32_t *a, MixTuning *t) {
     v.reg_max_band = a[4];
     v.lane_max_len = a[5];
     v.sw_dual = a[6];
+    v.sw_split = a[7];
     if (v.blocks_per_cu < 1 || v.blocks_per_cu > kMixMaxBlocksPerCu)
         return set_error(CGM_ERR_INVALID, "tuning: blocks_per_cu must be 1 or 2");
     // static LDS of the kernel (a few ints) needs room beside the dynamic part
@@ -469,6 +480,8 @@ int tuning_from_array(const int32_t *a, MixTuning *t) {
     if (v.lane_max_len < 1) return set_error(CGM_ERR_INVALID, "tuning: lane_max_len < 1");
     if (v.sw_dual != 0 && v.sw_dual != 1)
         return set_error(CGM_ERR_INVALID, "tuning: sw_dual must be 0 or 1");
+    if (v.sw_split < 0 || v.sw_split > 100)
+        return set_error(CGM_ERR_INVALID, "tuning: sw_split must be 0..100");
     *t = v;
     return CGM_OK;
 }
@@ -664,7 +677,8 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                        dim3(grid), dim3(kMixThreads),
                        (size_t)tune.lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
                        P->gm, out, P->order, P->tasks, P->task_order, n_tasks, queue,
-                       tune.lds_bytes, tune.tile_steps, tune.sw_dual, P->slab, P->slab_words);
+                       tune.lds_bytes, tune.tile_steps, tune.sw_dual, tune.sw_split, P->slab,
+                       P->slab_words);
     CGM_HIP(hipGetLastError());
     CGM_HIP(hipEventRecord(P->ev_stop, stream));
     // uncommon SW classes: LDS lanes, then one workgroup per pair
@@ -832,6 +846,14 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
         MixTuning c = cur;
         c.sw_dual = 1 - cur.sw_dual;
         try_all(&c, 1);
+    }
+    {   // SW-only batches: share of the units in the long queue
+        const int v[] = {40, 50, 55, 65};
+        MixTuning c[4];
+        int nc = 0;
+        for (int x : v)
+            if (x != cur.sw_split) (c[nc] = cur, c[nc++].sw_split = x);
+        try_all(c, nc);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

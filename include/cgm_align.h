@@ -252,6 +252,11 @@ int cgm_plan_kernel_ms(void *plan, float *ms);
  *                   (packed int16 arithmetic) wherever every value fits int16
  *                   (match * min(qlen, rlen) + match <= 32767, penalties
  *                   >= -8191); 0: int32 lanes only
+ *   SW_SPLIT        SW-only batches: the longest-first list of 64-pair units
+ *                   is cut after this percent (0..100) of the units; the
+ *                   first wavefront on each SIMD takes the long part, then
+ *                   the short part, the second wavefront only the short part,
+ *                   so the last units of the launch are the shortest
  * cgm_plan_autotune times the whole pipeline on the given batch for
  * candidate values of each field in turn (coordinate descent, median of
  * `reps` runs per candidate), keeps a change only if it is >= 2 % faster,
@@ -266,7 +271,8 @@ int cgm_plan_kernel_ms(void *plan, float *ms);
 #define CGM_TUNE_REG_MAX_BAND 4
 #define CGM_TUNE_LANE_MAX_LEN 5
 #define CGM_TUNE_SW_DUAL 6
-#define CGM_TUNE_N 7
+#define CGM_TUNE_SW_SPLIT 7
+#define CGM_TUNE_N 8
 int cgm_plan_get_tuning(void *plan, int32_t *params);
 int cgm_plan_set_tuning(void *plan, const int32_t *params);
 int cgm_tuning_get_default(int32_t *params);

@@ -153,6 +153,22 @@ def test_sw_register_lane_forms_equal_oracle(cgm, dev, orc, gen_mod, dual):
         plan.close()
 
 
+@pytest.mark.parametrize("split", [0, 30, 55, 100])
+def test_sw_unit_queue_split_equals_oracle(cgm, dev, orc, gen_mod, split):
+    """Tuning sw_split cuts the longest-first 64-pair units into a long and
+    a short queue (0: one queue taken by every wavefront; 100: only the
+    first wavefront of each SIMD takes units); every unit is aligned once
+    and the results equal the oracle."""
+    w = gen_mod.generate(2, 20000, seed=77)
+    up = Uploaded(dev, w)
+    plan = cgm.MixedPlan(w.n_pairs, 0)
+    plan.set_tuning(sw_split=split)
+    print(f"[case] unit queue split={split}", flush=True)
+    assert_same(plan_sw(cgm, dev, up, plan, CONFIG_SW), sw_oracle(orc, w, CONFIG_SW),
+                f"split={split}")
+    plan.close()
+
+
 def test_sw_dual_lanes_int16_boundary(cgm, dev, orc):
     """match * min(qlen, rlen) + match at 32766 (dual-row lanes) and 32768
     (int32 lanes), alone and together in one wavefront (which then runs the
