@@ -33,9 +33,9 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # int32 lane-ops/s, MI355X spec
 HBM_PEAK_GBS = 8000.0
 SW_OPS_PER_CELL = 11
 # HBM bytes per launch of the persistent kernel on this workload, from the
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (130860.9 KiB x 2.0 + 38848.5 KiB
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (128439.5 KiB x 2.0 + 38114.8 KiB
 # x 1.0, the scales measured on a 256 MiB device copy).
-TRAFFIC_BYTES = int((130860.9 * 2.0 + 38848.5) * 1024)
+TRAFFIC_BYTES = int((128439.5 * 2.0 + 38114.8) * 1024)
 SW = dict(match=2, mismatch=-4, gap_open=-4, gap_extend=-2)
 
 
