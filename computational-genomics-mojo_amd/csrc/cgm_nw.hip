@@ -137,23 +137,52 @@ __device__ __forceinline__ int nw_reg_dispatch(int band, const uint64_t *q, int 
     }
 }
 
+constexpr uint32_t kNwSplitPercent = 55;  // groups in the long queue
+constexpr int kNwCtlSlots = 22;           // ctl offset of the per-SIMD counters
+constexpr int kNwSimdKeys = 8192;         // XCC (3 bits) x CU (8) x SIMD (2)
+
 // Register lanes over order[range[0] .. range[1]) (every pair there has an
 // effective band <= 64, sorted by band then length, largest first): each
-// wavefront takes the next 64 entries from an atomic counter until the range
-// is exhausted, so the widest and longest groups start first and no
-// wavefront idles while another has work; a group holding two bands runs
-// once per band.
+// wavefront takes groups of 64 entries by itself from a long queue (the first
+// kNwSplitPercent % of the groups) and a short queue (the rest), both
+// largest first.  Of two wavefronts on one SIMD the older one gets most
+// issue slots, so the first one started on a SIMD takes the long queue and
+// then the short one, the second only the short one: the widest groups start
+// first, never on a starved wavefront, and the last groups of the launch are
+// the smallest.  `ctl` holds the 64-bit counter (long-queue claims in the low
+// half, short-queue claims in the high half) and, from ctl + 22, one
+// wavefront counter per SIMD; a group holding two bands runs once per band.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void nw_queue_kernel(
     Batch b, const int32_t *band_arr, int32_t band_scalar, SwScoring sc, SwOut out,
-    const int32_t *order, const int32_t *range, int32_t *next) {
+    const int32_t *order, const int32_t *range, int32_t *ctl) {
     const int64_t start = range[0], end = range[1];
     const int lane = threadIdx.x & 63;
+    const uint32_t units = (uint32_t)((end - start + 63) / 64);
+    const uint32_t split = (uint32_t)(((uint64_t)units * kNwSplitPercent) / 100u);
+    unsigned long long *ends = (unsigned long long *)ctl;
+    int slot = 0;
+    if (lane == 0) {
+        const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_ID
+        const uint32_t xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20);  // XCC_ID
+        const uint32_t key = ((xcc & 7u) << 10) | (((hw >> 8) & 0xffu) << 2) | ((hw >> 4) & 3u);
+        slot = atomicAdd(ctl + kNwCtlSlots + key, 1);
+    }
+    const bool second = (__shfl(slot, 0, 64) & 1) != 0;
     for (;;) {
-        int u = 0;
-        if (lane == 0) u = atomicAdd(next, 1);
+        int u = -1;
+        if (lane == 0) {
+            if (!second) {
+                const uint32_t a = (uint32_t)atomicAdd(ends, 1ull);
+                if (a < split) u = (int)a;
+            }
+            if (u < 0) {
+                const uint32_t c = (uint32_t)(atomicAdd(ends, 1ull << 32) >> 32);
+                if ((uint64_t)split + c < units) u = (int)(split + c);
+            }
+        }
         u = __shfl(u, 0, 64);
+        if (u < 0) break;
         const int64_t base = start + 64 * (int64_t)u;
-        if (base >= end) break;
         const int count = (int)min((int64_t)64, end - base);
         const bool active = lane < count;
         const int64_t pair = active ? order[base + lane] : 0;
@@ -325,9 +354,12 @@ int launch_nw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
     CGM_HIP(hipMallocAsync((void **)&keys, (size_t)n * 4, stream));
     CGM_HIP(hipMallocAsync((void **)&idx, (size_t)n * 4, stream));
     CGM_HIP(hipMallocAsync((void **)&order, (size_t)n * 4, stream));
-    CGM_HIP(hipMallocAsync((void **)&small, 64 * 4, stream));
+    // counts, ranges, the joined register range (40, 41), the queue control
+    // block (from 42: 64-bit counter, then one wavefront counter per SIMD)
+    const int small_words = 42 + kNwCtlSlots + kNwSimdKeys;
+    CGM_HIP(hipMallocAsync((void **)&small, (size_t)small_words * 4, stream));
     int32_t *counts = small, *ranges = small + 16;
-    hipLaunchKernelGGL(nw_zero_kernel, dim3(1), dim3(64), 0, stream, small, 64);
+    hipLaunchKernelGGL(nw_zero_kernel, dim3(1), dim3(256), 0, stream, small, small_words);
     hipLaunchKernelGGL(nw_class_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, b,
                        band, band_scalar, sc, out, keys, idx, counts);
     hipLaunchKernelGGL(nw_ranges_kernel, dim3(1), dim3(64), 0, stream, counts, ranges);
