@@ -157,8 +157,13 @@ constexpr int kTeamUnroll = 4;        // diagonal groups per wavefront and pass 
 // Persistent workgroups take items of `order` from the atomic counter
 // `queue`: all items (part 0), the items before *split (part 1) or those
 // from *split on (part 2).
+// Both launches are LDS-bound at 6 (512 threads, 3 workgroups per CU) and 8
+// (256 threads, 8 per CU) wavefronts per SIMD; the register budget is held
+// to that occupancy (the tiled path for the rare pairs beyond LDS would
+// otherwise raise the whole kernel to ~90 VGPRs, 5 wavefronts per SIMD).
 template <int NT>
-__global__ __launch_bounds__(NT) void wfa_wave_kernel(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 6 : 8)))
+void wfa_wave_kernel(
     Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap, WfaPen p, WfaGeom gm,
     int32_t *out, const int32_t *order, const int32_t *split, int32_t part, int32_t *queue,
     int32_t lds_bytes, int32_t *slab, int64_t slab_words) {
