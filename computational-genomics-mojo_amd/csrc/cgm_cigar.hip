@@ -585,8 +585,9 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
         return set_error(CGM_ERR_INVALID, "cigar: max_error out of range");
     int g = 0, nM = 0, nI = 0;
     if (int rc = wfa_geometry(p, &g, &nM, &nI)) return rc;
-    // 1. costs
-    if (int rc = launch_wfa_auto(b, max_error, max_error_scalar, p, out_cost, stream)) return rc;
+    // 1. costs (the automatic WFA launcher)
+    if (int rc = launch_wfa_wave(b, max_error, max_error_scalar, p, out_cost, nullptr, stream))
+        return rc;
     // 2. history sizes -> host, chunks within the budget
     int64_t *d_bytes = nullptr, *d_off = nullptr;
     CGM_HIP(hipMallocAsync((void **)&d_bytes, (size_t)n_pairs * 8, stream));

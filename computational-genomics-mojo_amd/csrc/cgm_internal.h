@@ -85,9 +85,6 @@ int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error
 // order: optional device permutation of pair indices (work-sorted); NULL = 0..n-1.
 int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
                     WfaPen p, int32_t *out, const int32_t *order, hipStream_t stream);
-// The persistent mixed kernel with every pair a WFA pair (cgm_mixed.hip).
-int launch_wfa_auto(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
-                    int32_t *out_cost, hipStream_t stream);
 
 int wfa_geometry(const WfaPen &p, int *g, int *nM, int *nI);
 // One-workgroup-per-pair baselines restricted to one kind of a mixed batch

@@ -689,27 +689,6 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                                  ranges + 2 * kMixSwPlain, P->cus * 2, stream);
 }
 
-// Automatic WFA layout: the persistent kernel with every pair a WFA pair,
-// on the plan cached per (device, stream), sized for the batch's largest
-// max_error (read back from the device when given per pair).
-int launch_wfa_auto(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
-                    int32_t *out_cost, hipStream_t stream) {
-    if (int rc = check_batch(b)) return rc;
-    if (int rc = check_wfa_pen(p)) return rc;
-    if (b.n_pairs == 0) return CGM_OK;
-    int32_t me_max = max_error_scalar;
-    if (max_error) {
-        if (int rc = device_max_i32(max_error, b.n_pairs, stream, &me_max)) return rc;
-    }
-    me_max = std::max(0, std::min(me_max, kMaxWfaError));
-    MixPlan *P = nullptr;
-    if (int rc = cached_plan(stream, b.n_pairs, me_max, p, &P)) return rc;
-    const SwScoring sc = {2, -3, -5, -2};  // unused: the batch holds no SW pair
-    const SwOut out = {out_cost, nullptr, nullptr};
-    return mix_enqueue(P, b, nullptr, 1, nullptr, 0, max_error, max_error_scalar, sc, p, out,
-                       stream);
-}
-
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                 SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream) {
