@@ -40,7 +40,8 @@ namespace cgm {
 
 namespace {
 
-constexpr int kCigarThreads = 512;
+constexpr int kCigarThreads = 512;       // workgroups of the pairs beyond kCigarSmallLds
+constexpr int kCigarSmallThreads = 256;  // workgroups of the pairs within it
 constexpr int kCigarUnroll = 4;  // diagonal groups per wavefront and pass of a score
 
 // Per-device history workspace kept between calls (see cgm_wfa_cigar_launch).
@@ -52,6 +53,9 @@ struct CigarWorkspace {
 };
 CigarWorkspace g_cigar_ws[kCigarMaxDevices];
 constexpr int kCigarLds = 52 * 1024;  // ring + sequences per workgroup (3 per CU)
+// pairs whose branch-free forward layout fits this run on 256-thread
+// workgroups with it (8 per CU)
+constexpr int kCigarSmallLds = 20 * 1024;
 constexpr int64_t kCigarBudgetMin = int64_t(8) << 30;
 constexpr int64_t kCigarBudgetMax = int64_t(96) << 30;
 
@@ -75,19 +79,27 @@ __host__ __device__ __forceinline__ CigarGeom cigar_geom(int qlen, int rlen, int
     return c;
 }
 
+// History bytes of each pair, and small[i] = 1 when its branch-free forward
+// layout fits small_lds (the pair then runs on a small workgroup).
 __global__ void cigar_size_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p,
-                                  int g, const int32_t *cost, int64_t *bytes) {
+                                  wfadev::WfaGeom gm, int32_t small_lds, const int32_t *cost,
+                                  int64_t *bytes, int32_t *small) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= b.n_pairs) return;
     const uint64_t *q, *r;
     int qlen = 0, rlen = 0;
     int64_t v = 0;
+    int s = 0;
     if (load_pair(b, i, q, r, qlen, rlen)) {
-        const CigarGeom c = cigar_geom(qlen, rlen, me_arr ? me_arr[i] : me_scalar, cost[i], p, g);
+        const CigarGeom c =
+            cigar_geom(qlen, rlen, me_arr ? me_arr[i] : me_scalar, cost[i], p, gm.g);
         v = 3 * c.rows * c.W * c.esize;
         v = (v + 255) & ~(int64_t)255;
+        s = c.esize == 2 && c.W > 0 &&
+            wfadev::team_fast_bytes(qlen, rlen, (int)c.W, gm) <= small_lds;
     }
     bytes[i] = v;
+    small[i] = s;
 }
 
 template <typename T>
@@ -490,11 +502,12 @@ __device__ void cigar_forward(const uint64_t *q, int qlen, const uint64_t *r, in
     }
 }
 
-__global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
+template <int NT, int LDS>
+__global__ __launch_bounds__(NT) void wfa_cigar_kernel(
     Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p, wfadev::WfaGeom gm,
     const int32_t *cost_arr, const int64_t *hist_off, uint8_t *ws, const int32_t *order,
     int64_t p0, uint32_t *cigar, const int64_t *cigar_off, int32_t *cigar_len) {
-    extern __shared__ uint64_t lds[];  // kCigarLds bytes: ring, then sequences
+    extern __shared__ uint64_t lds[];  // LDS bytes: ring, then sequences
     const int g = gm.g;
     const int64_t pair = order[p0 + blockIdx.x];
     const int cost = cost_arr[pair];
@@ -510,9 +523,9 @@ __global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
     const int64_t plane = cg.rows * cg.W;
     uint32_t *out = cigar + cigar_off[pair];
     const int64_t cap = cigar_off[pair + 1] - cigar_off[pair];
-    if (cg.esize == 2 && wfadev::team_fast_bytes(qlen, rlen, (int)cg.W, gm) <= kCigarLds) {
+    if (cg.esize == 2 && wfadev::team_fast_bytes(qlen, rlen, (int)cg.W, gm) <= LDS) {
         int16_t *M = (int16_t *)h;
-        cigar_forward_fast<kCigarUnroll, kCigarThreads>(qg, qlen, rg, rlen, me, cost, p, gm, M,
+        cigar_forward_fast<kCigarUnroll, NT>(qg, qlen, rg, rlen, me, cost, p, gm, M,
                                                         M + plane, M + 2 * plane, (int)cg.K,
                                                         (int)cg.W, (uint8_t *)lds);
         if (threadIdx.x == 0)
@@ -525,9 +538,9 @@ __global__ __launch_bounds__(kCigarThreads) void wfa_cigar_kernel(
     // LDS: the ring when it fits, then the sequences when they fit after it
     const int64_t ring_bytes =
         (((int64_t)gm.nM + 2 * gm.nI) * cg.W * cg.esize + 15) & ~(int64_t)15;
-    const bool ring = ring_bytes <= kCigarLds;
+    const bool ring = ring_bytes <= LDS;
     const int64_t seq_at = ring ? ring_bytes : 0;
-    if (seq_at + 8 * (qw + rw) <= kCigarLds) {
+    if (seq_at + 8 * (qw + rw) <= LDS) {
         uint64_t *seq = lds + seq_at / 8;
         for (int i = threadIdx.x; i < qw + rw; i += blockDim.x) seq[i] = i < qw ? qg[i] : rg[i - qw];
         q = seq;
@@ -590,14 +603,22 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
         return rc;
     // 2. history sizes -> host, chunks within the budget
     int64_t *d_bytes = nullptr, *d_off = nullptr;
+    int32_t *d_small = nullptr;
     CGM_HIP(hipMallocAsync((void **)&d_bytes, (size_t)n_pairs * 8, stream));
+    CGM_HIP(hipMallocAsync((void **)&d_small, (size_t)n_pairs * 4, stream));
     hipLaunchKernelGGL(cigar_size_kernel, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0,
-                       stream, b, max_error, max_error_scalar, p, g, out_cost, d_bytes);
+                       stream, b, max_error, max_error_scalar, p, wfadev::WfaGeom{g, nM, nI},
+                       (int32_t)kCigarSmallLds, out_cost, d_bytes, d_small);
     std::vector<int64_t> bytes((size_t)n_pairs), off((size_t)n_pairs);
+    std::vector<int32_t> small((size_t)n_pairs);
     hipError_t err = hipMemcpyAsync(bytes.data(), d_bytes, (size_t)n_pairs * 8,
                                     hipMemcpyDeviceToHost, stream);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(small.data(), d_small, (size_t)n_pairs * 4, hipMemcpyDeviceToHost,
+                             stream);
     if (err == hipSuccess) err = hipStreamSynchronize(stream);
     (void)hipFreeAsync(d_bytes, stream);
+    (void)hipFreeAsync(d_small, stream);
     if (err != hipSuccess) return hip_error(err, "cigar: history sizes");
     // pairs by history size, largest first; chunks [p0, p1) of that order,
     // a pair larger than the budget forming a chunk of its own
@@ -629,6 +650,10 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
             acc += bytes[(size_t)order[(size_t)j]];
             ++j;
         }
+        // within the chunk: the pairs for the large workgroups first, then
+        // those for the small ones, each still largest first
+        std::stable_partition(order.begin() + i, order.begin() + j,
+                              [&](int32_t a) { return small[(size_t)a] == 0; });
         chunks.push_back({i, j});
         ws_need = std::max(ws_need, acc);
         i = j;
@@ -662,11 +687,20 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
     // 3. forward with history + backtrace, chunk by chunk (stream order
     // keeps a chunk's workspace intact until its kernel is done)
     for (size_t c = 0; !rc && c < chunks.size(); ++c) {
-        const int64_t p0 = chunks[c].first, np = chunks[c].second - p0;
-        hipLaunchKernelGGL(wfa_cigar_kernel, dim3((unsigned)np), dim3(kCigarThreads),
-                           (size_t)kCigarLds, stream, b, max_error, max_error_scalar, p,
-                           wfadev::WfaGeom{g, nM, nI}, out_cost, d_off, ws, d_order, p0, cigar,
-                           cigar_off, cigar_len);
+        const int64_t p0 = chunks[c].first, p1 = chunks[c].second;
+        int64_t nbig = 0;
+        while (p0 + nbig < p1 && small[(size_t)order[(size_t)(p0 + nbig)]] == 0) ++nbig;
+        if (nbig > 0)
+            hipLaunchKernelGGL((wfa_cigar_kernel<kCigarThreads, kCigarLds>), dim3((unsigned)nbig),
+                               dim3(kCigarThreads), (size_t)kCigarLds, stream, b, max_error,
+                               max_error_scalar, p, wfadev::WfaGeom{g, nM, nI}, out_cost, d_off,
+                               ws, d_order, p0, cigar, cigar_off, cigar_len);
+        if (p1 - p0 - nbig > 0)
+            hipLaunchKernelGGL((wfa_cigar_kernel<kCigarSmallThreads, kCigarSmallLds>),
+                               dim3((unsigned)(p1 - p0 - nbig)), dim3(kCigarSmallThreads),
+                               (size_t)kCigarSmallLds, stream, b, max_error, max_error_scalar,
+                               p, wfadev::WfaGeom{g, nM, nI}, out_cost, d_off, ws, d_order,
+                               p0 + nbig, cigar, cigar_off, cigar_len);
         err = hipGetLastError();
         if (err != hipSuccess) rc = hip_error(err, "cigar: kernel launch");
     }
