@@ -416,7 +416,7 @@ __device__ void cigar_forward_fast(const uint64_t *qg, int qlen, const uint64_t 
                 const int lim = min(qlen - row, rlen - col);
                 const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
                 mr[u] = row + n;
-                if (__ballot(row >= 0 && n == 32 && lim > 32 && k <= hi)) more |= 1ull << u;
+                if (__ballot(row >= 0 && n == 32 && lim > 32)) more |= 1ull << u;
             }
             if (more) {
 #pragma unroll

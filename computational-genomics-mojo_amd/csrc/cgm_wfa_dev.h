@@ -256,10 +256,19 @@ __device__ __forceinline__ void window_lsb(const uint32_t *s, int pos, uint32_t 
     w1 = __builtin_amdgcn_alignbit(c, b, sh);
 }
 
-// Leading bases (0..32) on which two 32-base windows agree.
+// Index of the lowest set bit, 0xFFFFFFFF for 0 (the instruction's own
+// result for 0, which the compiler's count-trailing-zeros does not promise).
+__device__ __forceinline__ uint32_t lowest_bit(uint32_t x) {
+    uint32_t r;
+    asm("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Leading bases (0..32) on which two 32-base windows agree: the lowest
+// differing bit over 2, 64 (-> 32) when the windows are equal.
 __device__ __forceinline__ int match32(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
-    const uint64_t x = ((uint64_t)(a1 ^ b1) << 32) | (uint64_t)(a0 ^ b0);
-    return x ? (int)(__builtin_ctzll(x) >> 1) : 32;
+    const uint32_t t1 = min(lowest_bit(a1 ^ b1), 32u);
+    return (int)(min(lowest_bit(a0 ^ b0), 32u + t1) >> 1);
 }
 
 // Furthest row reached from (row, col), col >= 0, following matching bases.
@@ -374,7 +383,7 @@ __device__ int wfa_pair_team_fast(const uint64_t *qg, int qlen, const uint64_t *
                 const int lim = min(qlen - row, rlen - col);
                 const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
                 mr[u] = row + n;
-                if (__ballot(row >= 0 && n == 32 && lim > 32 && k <= hi)) more |= 1ull << u;
+                if (__ballot(row >= 0 && n == 32 && lim > 32)) more |= 1ull << u;
             }
             // the rest of long matches (few lanes)
             if (more) {
@@ -661,7 +670,7 @@ __device__ int wfa_pair_tiled_fast(const uint64_t *qg, int qlen, const uint64_t 
                         const int lim = min(qlen - row, rlen - col);
                         const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
                         mr[u] = row + n;
-                        if (__ballot(row >= 0 && n == 32 && lim > 32 && k <= khi)) more |= 1ull << u;
+                        if (__ballot(row >= 0 && n == 32 && lim > 32)) more |= 1ull << u;
                     }
                     if (more) {
 #pragma unroll

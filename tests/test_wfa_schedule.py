@@ -234,9 +234,15 @@ def window_lsb(s, pos):
     return _alignbit(s[p + 1], s[p], sh), _alignbit(s[p + 2], s[p + 1], sh)
 
 
+def _lowest_bit(x):
+    """v_ffbl_b32: index of the lowest set bit, 0xFFFFFFFF for 0."""
+    return ((x & -x).bit_length() - 1) if x else 0xFFFFFFFF
+
+
 def match32(a, b):
-    x = ((a[1] ^ b[1]) << 32) | (a[0] ^ b[0])
-    return ((x & -x).bit_length() - 1) >> 1 if x else 32
+    """min(ffbl(x0), 32 + min(ffbl(x1), 32)) >> 1, as the kernels count it."""
+    t1 = min(_lowest_bit(a[1] ^ b[1]), 32)
+    return min(_lowest_bit(a[0] ^ b[0]), 32 + t1) >> 1
 
 
 def extend_lsb(qs, ql, rs, rl, row, col):
