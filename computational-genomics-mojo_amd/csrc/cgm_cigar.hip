@@ -411,10 +411,10 @@ __device__ void cigar_forward_fast(const uint64_t *qg, int qlen, const uint64_t 
                 const int k = di0 + u * NT - K;
                 const int row = mv[u], col = row - k;
                 uint32_t a0, a1, b0, b1;
-                window_lsb(qs, min(max(row, 0), qlen), a0, a1);
-                window_lsb(rs, min(max(col, 0), rlen), b0, b1);
+                window_lsb(qs, med3(row, 0, qlen), a0, a1);
+                window_lsb(rs, med3(col, 0, rlen), b0, b1);
                 const int lim = min(qlen - row, rlen - col);
-                const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
+                const int n = med3(match32(a0, a1, b0, b1), lim, 0);  // match >= 0
                 mr[u] = row + n;
                 if (__ballot(row >= 0 && n == 32 && lim > 32)) more |= 1ull << u;
             }

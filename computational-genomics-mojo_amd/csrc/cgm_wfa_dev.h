@@ -264,6 +264,14 @@ __device__ __forceinline__ uint32_t lowest_bit(uint32_t x) {
     return r;
 }
 
+// Middle value of three (one instruction; the compiler builds it only for
+// constant bounds): clamps x to [lo, hi] when lo <= hi.
+__device__ __forceinline__ int med3(int x, int a, int b) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(a), "v"(b));
+    return r;
+}
+
 // Leading bases (0..32) on which two 32-base windows agree: the lowest
 // differing bit over 2, 64 (-> 32) when the windows are equal.
 __device__ __forceinline__ int match32(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
@@ -378,10 +386,10 @@ __device__ int wfa_pair_team_fast(const uint64_t *qg, int qlen, const uint64_t *
                 const int k = di0 + u * NT - K;
                 const int row = mv[u], col = row - k;
                 uint32_t a0, a1, b0, b1;
-                window_lsb(qs, min(max(row, 0), qlen), a0, a1);
-                window_lsb(rs, min(max(col, 0), rlen), b0, b1);
+                window_lsb(qs, med3(row, 0, qlen), a0, a1);
+                window_lsb(rs, med3(col, 0, rlen), b0, b1);
                 const int lim = min(qlen - row, rlen - col);
-                const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
+                const int n = med3(match32(a0, a1, b0, b1), lim, 0);  // match >= 0
                 mr[u] = row + n;
                 if (__ballot(row >= 0 && n == 32 && lim > 32)) more |= 1ull << u;
             }
@@ -665,10 +673,10 @@ __device__ int wfa_pair_tiled_fast(const uint64_t *qg, int qlen, const uint64_t 
                         const int k = j0 + u * NT + base;
                         const int row = mv[u], col = row - k;
                         uint32_t a0, a1, b0, b1;
-                        window_lsb(qs, min(max(row, 0), qlen), a0, a1);
-                        window_lsb(rs, min(max(col, 0), rlen), b0, b1);
+                        window_lsb(qs, med3(row, 0, qlen), a0, a1);
+                        window_lsb(rs, med3(col, 0, rlen), b0, b1);
                         const int lim = min(qlen - row, rlen - col);
-                        const int n = min(match32(a0, a1, b0, b1), max(lim, 0));
+                        const int n = med3(match32(a0, a1, b0, b1), lim, 0);  // match >= 0
                         mr[u] = row + n;
                         if (__ballot(row >= 0 && n == 32 && lim > 32)) more |= 1ull << u;
                     }
