@@ -167,7 +167,7 @@ def mixed_extra(m, dev, gen, n_pairs=100_000):
 
 
 def serving_extra(m, duration=3.0):
-    """Serving loop on config-5 requests (1-64 pairs, log-normal lengths,
+    """Serving loop on requests of 1-64 pairs (log-normal lengths,
     SW band 64 below 2 kb, WFA above): peak sustainable pairs/s in a closed
     loop, then Poisson arrivals at 70% of that peak (p50/p99), then the
     50,000 requests/s arrival rate of the workload definition."""
@@ -314,8 +314,8 @@ def main():
                          "one register kernel per band bucket")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run 3 device-to-device copies of this many bytes (counter scale)")
-    ap.add_argument("--only-extra", choices=["wfa_config3", "ingest_config2", "wfa_cigar_config3",
-                                             "nw_config2"],
+    ap.add_argument("--only-extra", choices=["wfa_20k_1to10kb", "ingest_200k_pairs", "wfa_cigar_20k_1to10kb",
+                                             "nw_200k_100to300bp"],
                     help="run only this extra workload and print its result (for profiling "
                          "its kernels alone)")
     args = ap.parse_args()
@@ -323,13 +323,13 @@ def main():
         m = load_package()
         from cgm_amd import gen
         dev = m.GenomicsDevice(0)
-        if args.only_extra == "nw_config2":
+        if args.only_extra == "nw_200k_100to300bp":
             w = gen.generate(2, args.pairs)
             cells = int(gen.valid_band_cells(w.q_len, w.r_len, w.band).sum())
             res = nw_extra(m, dev, w, cells)
         else:
-            fn = {"wfa_config3": wfa_extra, "ingest_config2": ingest_extra,
-                  "wfa_cigar_config3": cigar_extra}[args.only_extra]
+            fn = {"wfa_20k_1to10kb": wfa_extra, "ingest_200k_pairs": ingest_extra,
+                  "wfa_cigar_20k_1to10kb": cigar_extra}[args.only_extra]
             res = fn(m, dev, gen)
         print(json.dumps({"extra": args.only_extra, **res}), flush=True)
         return
@@ -401,12 +401,12 @@ def main():
 
     extras = {}
     if not args.no_extras:
-        extras["wfa_config3"] = wfa_extra(m, dev, gen)
-        extras["mixed_config4"] = mixed_extra(m, dev, gen)
-        extras["serving_config5"] = serving_extra(m)
-        extras["ingest_config2"] = ingest_extra(m, dev, gen)
-        extras["wfa_cigar_config3"] = cigar_extra(m, dev, gen)
-        extras["nw_config2"] = nw_extra(m, dev, w, total_cells)
+        extras["wfa_20k_1to10kb"] = wfa_extra(m, dev, gen)
+        extras["mixed_100k"] = mixed_extra(m, dev, gen)
+        extras["serving_1to64_pairs"] = serving_extra(m)
+        extras["ingest_200k_pairs"] = ingest_extra(m, dev, gen)
+        extras["wfa_cigar_20k_1to10kb"] = cigar_extra(m, dev, gen)
+        extras["nw_200k_100to300bp"] = nw_extra(m, dev, w, total_cells)
         wall_p, kms_p = time_steps(m, dev, lambda: run.launch(1), max(2, args.steps // 5), 1)
         plain_res = run.results()
         same = all(np.array_equal(a, b) for a, b in zip(plain_res, gpu_res))

@@ -90,13 +90,13 @@ def test_cigar_int32_history_rows(cgm, dev, orc):
     assert (int(cost[0]), cigars[0]) == exp
 
 
-def test_cigar_config3_sample_equals_oracle(cgm, dev, orc, gen_mod):
+def test_cigar_wfa_20k_sample_equals_oracle(cgm, dev, orc, gen_mod):
     w = gen_mod.generate(3, 2000)
     idx = np.nonzero(w.max_error <= 1500)[0]
     idx = idx[np.linspace(0, idx.size - 1, 120).astype(int)]
     s = w.subset(idx)
     cfg = cgm.WFAConfig(4, 6, 2, int(s.max_error.max()))
-    print(f"[case] cigar config3 sample {s.n_pairs} pairs", flush=True)
+    print(f"[case] cigar wfa 20k x 1-10 kb sample {s.n_pairs} pairs", flush=True)
     cost, cigars = cgm.wfa_affine_cigar_batch(s.q_packed, s.q_off, s.q_len, s.r_packed, s.r_off,
                                               s.r_len, cfg, max_error=s.max_error)
     e_cost, e_cig = orc.wfa_cigar_batch(s.q_packed, s.r_packed, s.q_off, s.r_off, s.q_len,
@@ -130,12 +130,12 @@ def codes(packed, off, length):
     return ((w >> (62 - 2 * (i % 32)).astype(np.uint64)) & np.uint64(3)).astype(np.int8)
 
 
-def test_cigar_config3_full_size_properties(cgm, dev, gen_mod):
+def test_cigar_wfa_20k_full_size_properties(cgm, dev, gen_mod):
     """All 20,000 pairs of 1-10 kb: every CIGAR's cost is the pair's WFA cost
     and its operations walk both sequences exactly."""
     w = gen_mod.generate(3, 20_000)
     cfg = cgm.WFAConfig(4, 6, 2, int(w.max_error.max()))
-    print("[case] cigar config3 full 20000 pairs", flush=True)
+    print("[case] cigar wfa 20k x 1-10 kb full 20000 pairs", flush=True)
     up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
                                                   "q_off", "r_off", "max_error")}
     n = w.n_pairs

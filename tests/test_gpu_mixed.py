@@ -46,21 +46,21 @@ def test_mixed_sample_parity(cgm, dev, orc, gen_mod, layout):
     assert_same(got, oracle_mixed(orc, w), f"mixed {layout}")
 
 
-def test_mixed_config4_full_size(cgm, dev, orc, gen_mod):
-    """Config 4 (100,000 pairs incl. 5-20 kb WFA pairs): the persistent
+def test_mixed_100k_full_size(cgm, dev, orc, gen_mod):
+    """Mixed batch (100,000 pairs incl. 5-20 kb WFA pairs): the persistent
     layout equals the per-pair kernels on every pair; a sample of SW pairs
     and of the cheaper WFA pairs equals the oracle."""
     w = gen_mod.generate(4, 100_000)
-    print("[case] config4 full 100000 pairs", flush=True)
+    print("[case] mixed full 100000 pairs", flush=True)
     auto = cgm.align_mixed_batch(dev, w, sw_cfg(cgm), layout=0)
     plain = cgm.align_mixed_batch(dev, w, sw_cfg(cgm), layout=1)
-    assert_same(auto, plain, "config4 persistent vs plain")
+    assert_same(auto, plain, "mixed 100k persistent vs plain")
     rng = np.random.default_rng(1)
     sw_idx = rng.choice(np.nonzero(w.kind == 0)[0], 1500, replace=False)
     wfa_idx = np.nonzero(w.kind == 1)[0]
     wfa_idx = wfa_idx[np.argsort(w.max_error[wfa_idx])[:40]]
     idx = np.concatenate([sw_idx, wfa_idx])
-    assert_same(tuple(a[idx] for a in auto), oracle_mixed(orc, w.subset(idx)), "config4 sample")
+    assert_same(tuple(a[idx] for a in auto), oracle_mixed(orc, w.subset(idx)), "mixed 100k sample")
 
 
 def test_mixed_graph_replay_matches_eager(cgm, dev, gen_mod):

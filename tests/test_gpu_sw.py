@@ -50,33 +50,33 @@ def test_sw_golden_vectors(cgm, dev, layout):
 
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
-def test_sw_config1_parity(cgm, dev, orc, gen_mod, layout):
+def test_sw_1k_150bp_parity(cgm, dev, orc, gen_mod, layout):
     w = gen_mod.generate(1, 1000, seed=101)
-    print(f"[case] config1 1000 pairs layout={layout}", flush=True)
+    print(f"[case] sw 1000 pairs x 150 bp layout={layout}", flush=True)
     got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
-    assert_same(got, sw_oracle(orc, w), f"config1 {layout}")
+    assert_same(got, sw_oracle(orc, w), f"sw 1k x 150 bp {layout}")
 
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
-def test_sw_config2_sample_parity(cgm, dev, orc, gen_mod, layout):
+def test_sw_200k_sample_parity(cgm, dev, orc, gen_mod, layout):
     w = gen_mod.generate(2, 4000, seed=202)
-    print(f"[case] config2 4000 pairs layout={layout}", flush=True)
+    print(f"[case] sw 4000 pairs x 100-300 bp layout={layout}", flush=True)
     got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
-    assert_same(got, sw_oracle(orc, w), f"config2 {layout}")
+    assert_same(got, sw_oracle(orc, w), f"sw 100-300 bp sample {layout}")
 
 
-def test_sw_config2_full_size(cgm, dev, orc, gen_mod):
+def test_sw_200k_full_size(cgm, dev, orc, gen_mod):
     """200,000 pairs: every layout agrees on the whole batch, and a random
     sample of 2,000 pairs equals the oracle."""
     w = gen_mod.generate(2, 200_000)
     up = Uploaded(dev, w)
-    print("[case] config2 full 200000 pairs", flush=True)
+    print("[case] sw full 200000 pairs x 100-300 bp", flush=True)
     ref = sw_gpu(cgm, dev, up, LAYOUTS["plain"])
     for name in ("lane", "auto"):
-        assert_same(sw_gpu(cgm, dev, up, LAYOUTS[name]), ref, f"config2 full {name} vs plain")
+        assert_same(sw_gpu(cgm, dev, up, LAYOUTS[name]), ref, f"sw 200k full {name} vs plain")
     idx = np.random.default_rng(0).choice(w.n_pairs, 2000, replace=False)
     sub = w.subset(idx)
-    assert_same(tuple(a[idx] for a in ref), sw_oracle(orc, sub), "config2 full sample")
+    assert_same(tuple(a[idx] for a in ref), sw_oracle(orc, sub), "sw 200k full sample")
 
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))

@@ -44,15 +44,15 @@ def test_wfa_golden_vectors(cgm, dev, layout):
 
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
-def test_wfa_config3_sample_parity(cgm, dev, orc, gen_mod, layout):
+def test_wfa_20k_sample_parity(cgm, dev, orc, gen_mod, layout):
     w = gen_mod.generate(3, 20000)
     idx = np.random.default_rng(3).choice(w.n_pairs, 160, replace=False)
     sub = w.subset(idx)
-    print(f"[case] config3 sample 160 pairs layout={layout}", flush=True)
+    print(f"[case] wfa 1-10 kb sample 160 pairs layout={layout}", flush=True)
     got = wfa_gpu(cgm, dev, Uploaded(dev, sub), LAYOUTS[layout])
     exp = wfa_oracle(orc, sub)
     assert (exp >= 0).all()
-    assert_same(got, exp, f"config3 {layout}")
+    assert_same(got, exp, f"wfa 1-10 kb {layout}")
 
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
@@ -85,7 +85,7 @@ def test_wfa_is_deterministic(cgm, dev, gen_mod):
 
 
 def big_wfa_pairs(cgm, gen_mod, orc):
-    """Config-4 WFA pairs whose wavefronts exceed LDS (max_error 2.6k-13k,
+    """Mixed-batch WFA pairs whose wavefronts exceed LDS (max_error 2.6k-13k,
     5-20 kb), the same pairs with max_error just below their cost, and a
     34 kb pair whose rows do not fit int16."""
     w = gen_mod.generate(4, 20000, seed=11)
