@@ -87,7 +87,7 @@ def wfa_extra(m, dev, gen, n_pairs=20_000):
     the mixed persistent layout on an autotuned plan, of the automatic
     launcher (the WFA persistent workgroup-per-pair kernel) and of the plain
     one-workgroup-per-pair baseline."""
-    w = gen.generate(3, n_pairs)
+    w = gen.generate(gen.WFA_1_10KB, n_pairs)
     up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
                                                   "q_off", "r_off", "max_error", "kind", "band")}
     out = m.DeviceArray(w.n_pairs, np.int32)
@@ -133,7 +133,7 @@ def mixed_extra(m, dev, gen, n_pairs=100_000):
     WFA 5-20 kb): pairs/s of the persistent task-queue kernel on an
     autotuned plan replayed from a hipGraph, of the same enqueued eagerly,
     and of the one-workgroup-per-pair kernels."""
-    w = gen.generate(4, n_pairs)
+    w = gen.generate(gen.MIXED, n_pairs)
     n = w.n_pairs
     bufs = tuple(dev.upload(a) for a in (w.q_packed, w.r_packed, w.q_len, w.r_len, w.q_off,
                                           w.r_off))
@@ -220,7 +220,7 @@ def cigar_extra(m, dev, gen, n_pairs=20_000, steps=2):
     """WFA with traceback on 20,000 pairs of 1-10 kb (x=4, o=6, e=2): pairs/s
     of cgm_wfa_cigar_launch (costs, history sizing, history + backtrace),
     beside the cost-only automatic launch on the same batch."""
-    w = gen.generate(3, n_pairs)
+    w = gen.generate(gen.WFA_1_10KB, n_pairs)
     n = w.n_pairs
     up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
                                                   "q_off", "r_off", "max_error")}
@@ -258,7 +258,7 @@ def ingest_extra(m, dev, gen, n_pairs=200_000, steps=5):
     to packed words + NMask + word offsets by cgm_encode_ascii_launch;
     GB/s of algorithmic bytes (ASCII in, 12 B per sequence in, 16 B per
     32-base word and 8 B per sequence out) against the 8 TB/s HBM peak."""
-    w = gen.generate(2, n_pairs)
+    w = gen.generate(gen.SW_100_300BP, n_pairs)
     seqs = m.decode_ascii(w.q_packed, w.q_nmask, w.q_off, w.q_len) + \
         m.decode_ascii(w.r_packed, w.r_nmask, w.r_off, w.r_len)
     buf, starts, lens = m.concat_ascii(seqs)
@@ -324,7 +324,7 @@ def main():
         from cgm_amd import gen
         dev = m.GenomicsDevice(0)
         if args.only_extra == "nw_200k_100to300bp":
-            w = gen.generate(2, args.pairs)
+            w = gen.generate(gen.SW_100_300BP, args.pairs)
             cells = int(gen.valid_band_cells(w.q_len, w.r_len, w.band).sum())
             res = nw_extra(m, dev, w, cells)
         else:
@@ -355,7 +355,7 @@ def main():
     info = dev.info()
     log(f"[bench] device {info}")
     t = time.perf_counter()
-    w = gen.generate(2, args.pairs)
+    w = gen.generate(gen.SW_100_300BP, args.pairs)
     cells = gen.valid_band_cells(w.q_len, w.r_len, w.band)
     total_cells = int(cells.sum())
     band_area = int((w.q_len.astype(np.int64) * (2 * w.band.astype(np.int64) + 1)).sum())

@@ -1,7 +1,7 @@
-// cgm_gen.cpp -- seeded synthetic pair batches for the five workload configs.
+// cgm_gen.cpp -- seeded synthetic pair batches for the five CGM_WL_* workloads.
 //
 // Every pair is generated from its own splitmix64 stream keyed by
-// (seed, config, pair index), so a batch is identical whatever the thread
+// (seed, workload, pair index), so a batch is identical whatever the thread
 // count.  Bases are iid uniform A/C/G/T.  The query is the reference mutated
 // position by position: a substitution (one of the 3 other bases), an
 // insertion of `len` random bases before the current base, or a deletion of
@@ -114,32 +114,32 @@ int64_t add_ns(Rng &g, std::vector<uint8_t> &q, double p) {
     return n;
 }
 
-void make_pair(int config, uint64_t seed, int64_t idx, PairData &pd) {
-    Rng g{mix64(mix64(seed ^ ((uint64_t)config * 0xD1B54A32D192ED03ull)) + (uint64_t)idx)};
+void make_pair(int workload, uint64_t seed, int64_t idx, PairData &pd) {
+    Rng g{mix64(mix64(seed ^ ((uint64_t)workload * 0xD1B54A32D192ED03ull)) + (uint64_t)idx)};
     MutModel sw_model{0, 0, 1, 0.5};
     int64_t L = 0;
     bool wfa = false;
     double d = 0;
     MutModel m{};
-    switch (config) {
-        case 1:
+    switch (workload) {
+        case CGM_WL_SW_150BP:
             L = 150;
             m = {0.02, 0.005, 0, 0};
             pd.band = 32;
             break;
-        case 2:
+        case CGM_WL_SW_100_300BP:
             L = g.range(100, 300);
             d = 0.01 + 0.04 * g.uniform();
             m = {0.75 * d, 0.25 * d, 1, 0.5};
             pd.band = (int32_t)g.range(16, 64);
             break;
-        case 3:
+        case CGM_WL_WFA_1_10KB:
             L = llround(exp(log(1000.0) + (log(10000.0) - log(1000.0)) * g.uniform()));
             d = 0.005 + 0.045 * g.uniform();
             m = {0.75 * d, 0.25 * d, 1, 0.3};
             wfa = true;
             break;
-        case 4: {
+        case CGM_WL_MIXED: {
             double u = g.uniform();
             if (u < 0.5) {
                 L = g.range(100, 300);
@@ -155,7 +155,7 @@ void make_pair(int config, uint64_t seed, int64_t idx, PairData &pd) {
             m = {0.75 * d, 0.25 * d, 1, wfa ? 0.3 : 0.5};
             break;
         }
-        case 5: {
+        case CGM_WL_SERVING: {
             double ln = log(250.0) + g.normal();
             L = llround(exp(ln));
             L = std::min<int64_t>(20000, std::max<int64_t>(100, L));
@@ -203,8 +203,8 @@ void pack(const std::vector<uint8_t> &s, uint64_t *w, uint64_t *nm) {
 
 extern "C" {
 
-int cgm_gen_create(int32_t config, int64_t n_pairs, uint64_t seed, void **handle) {
-    if (!handle || n_pairs < 0 || config < 1 || config > 5)
+int cgm_gen_create(int32_t workload, int64_t n_pairs, uint64_t seed, void **handle) {
+    if (!handle || n_pairs < 0 || workload < CGM_WL_SW_150BP || workload > CGM_WL_SERVING)
         return cgm::set_error(CGM_ERR_INVALID, "cgm_gen_create: bad arguments");
     Workload *w = new (std::nothrow) Workload();
     if (!w) return cgm::set_error(CGM_ERR_NOMEM, "cgm_gen_create: out of memory");
@@ -214,7 +214,7 @@ int cgm_gen_create(int32_t config, int64_t n_pairs, uint64_t seed, void **handle
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t)
         th.emplace_back([&, t] {
-            for (int64_t i = t; i < n_pairs; i += nt) make_pair(config, seed, i, pairs[(size_t)i]);
+            for (int64_t i = t; i < n_pairs; i += nt) make_pair(workload, seed, i, pairs[(size_t)i]);
         });
     for (auto &x : th) x.join();
     th.clear();

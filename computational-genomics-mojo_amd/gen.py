@@ -1,15 +1,15 @@
 """Seeded synthetic pair batches (cgm_gen_* in libcgm_align.so).
 
-Configs (sizes of the five benchmark workloads):
-  1  plain SW: 150 bp targets, 2% subs, 0.5% indels of 1-3 bp, band 32
-  2  batched SW: targets 100-300 bp, divergence 1-5% (sub:indel 3:1,
-     indel length geometric p=0.5), band 16-64 per pair
-  3  WFA: targets log-uniform 1-10 kb, divergence 0.5-5% (geometric p=0.3)
-  4  mixed: 50% SW 100-300 bp band 32, 40% SW 1-2 kb band 128,
-     10% WFA 5-20 kb at 1-10% divergence
-  5  serving pair mix: length log-normal (median 250, sigma 1) clipped to
-     100-20000 bp, SW band 64 below 2 kb, WFA above
-All configs put 0.1% N positions into the query (packed as A + NMask bit).
+Workloads (the five benchmark workloads, by id):
+  SW_150BP      plain SW: 150 bp targets, 2% subs, 0.5% indels of 1-3 bp, band 32
+  SW_100_300BP  batched SW: targets 100-300 bp, divergence 1-5% (sub:indel 3:1,
+                indel length geometric p=0.5), band 16-64 per pair
+  WFA_1_10KB    WFA: targets log-uniform 1-10 kb, divergence 0.5-5% (geometric p=0.3)
+  MIXED         mixed: 50% SW 100-300 bp band 32, 40% SW 1-2 kb band 128,
+                10% WFA 5-20 kb at 1-10% divergence
+  SERVING       serving pair mix: length log-normal (median 250, sigma 1) clipped to
+                100-20000 bp, SW band 64 below 2 kb, WFA above
+All workloads put 0.1% N positions into the query (packed as A + NMask bit).
 WFA pairs carry max_error = planted edit cost + 4 * #N + 8.
 """
 from __future__ import annotations
@@ -23,13 +23,16 @@ from . import _lib
 
 SW_SCORING = dict(match=2, mismatch=-4, gap_open=-4, gap_extend=-2)
 WFA_PENALTIES = dict(x=4, o=6, e=2)
-DEFAULT_PAIRS = {1: 1000, 2: 200_000, 3: 20_000, 4: 100_000, 5: 200_000}
+# Workload ids of cgm_gen_create (CGM_WL_* in cgm_align.h).
+SW_150BP, SW_100_300BP, WFA_1_10KB, MIXED, SERVING = 1, 2, 3, 4, 5
+DEFAULT_PAIRS = {SW_150BP: 1000, SW_100_300BP: 200_000, WFA_1_10KB: 20_000, MIXED: 100_000,
+                 SERVING: 200_000}
 DEFAULT_SEED = 20260402
 
 
 @dataclass
 class Workload:
-    config: int
+    workload: int
     seed: int
     q_packed: np.ndarray
     q_nmask: np.ndarray
@@ -67,7 +70,7 @@ class Workload:
 
         qp, qn, qo, ql = take(self.q_packed, self.q_nmask, self.q_off, self.q_len)
         rp, rn, ro, rl = take(self.r_packed, self.r_nmask, self.r_off, self.r_len)
-        return Workload(self.config, self.seed, qp, qn, qo, ql, rp, rn, ro, rl,
+        return Workload(self.workload, self.seed, qp, qn, qo, ql, rp, rn, ro, rl,
                         self.kind[idx].copy(), self.band[idx].copy(), self.max_error[idx].copy())
 
     def sw_cells(self) -> np.ndarray:
@@ -113,10 +116,10 @@ def valid_band_cells(q_len, r_len, band) -> np.ndarray:
     return np.where((n > 0) & (r > 0), total, 0)
 
 
-def generate(config: int, n_pairs: int | None = None, seed: int = DEFAULT_SEED) -> Workload:
-    n = DEFAULT_PAIRS[config] if n_pairs is None else int(n_pairs)
+def generate(workload: int, n_pairs: int | None = None, seed: int = DEFAULT_SEED) -> Workload:
+    n = DEFAULT_PAIRS[workload] if n_pairs is None else int(n_pairs)
     h = ctypes.c_void_p()
-    _lib.call("cgm_gen_create", config, n, seed, ctypes.byref(h))
+    _lib.call("cgm_gen_create", workload, n, seed, ctypes.byref(h))
     try:
         npairs, qw, rw = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _lib.call("cgm_gen_sizes", h, ctypes.byref(npairs), ctypes.byref(qw), ctypes.byref(rw))
@@ -132,4 +135,4 @@ def generate(config: int, n_pairs: int | None = None, seed: int = DEFAULT_SEED) 
                   P(rl), P(kind), P(band), P(me))
     finally:
         _lib.load().cgm_gen_destroy(h)
-    return Workload(config, seed, qp, qn, qo, ql, rp, rn, ro, rl, kind, band, me)
+    return Workload(workload, seed, qp, qn, qo, ql, rp, rn, ro, rl, kind, band, me)

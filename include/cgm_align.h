@@ -383,13 +383,19 @@ int cgm_encode_ascii_host(const uint8_t *ascii, int64_t ascii_bytes, const int64
 /* ------------------------------------------------------- synthetic batches
  * Seeded workload generator (replaces make_random_batch,
  * benchmarks/harness.mojo:55-67, whose queries equal their references).
- * Every pair is generated from splitmix64 keyed by (seed, config, pair
+ * Every pair is generated from splitmix64 keyed by (seed, workload, pair
  * index), so the batch does not depend on thread count or call order.
- * config: 1 = plain SW (150 bp, band 32), 2 = batched SW (100-300 bp,
- * band 16-64), 3 = WFA (1-10 kb), 4 = mixed, 5 = serving pair mix.
+ * workload: one of the CGM_WL_* ids below.
  * kind[i]: 0 = SW pair, 1 = WFA pair.  For WFA pairs max_error[i] = planted
  * edit cost + x * (#N in the query) + 8, an upper bound of the optimum. */
-int cgm_gen_create(int32_t config, int64_t n_pairs, uint64_t seed, void **handle);
+enum {
+    CGM_WL_SW_150BP = 1,     /* SW, 150 bp, band 32 */
+    CGM_WL_SW_100_300BP = 2, /* batched SW, targets 100-300 bp, band 16-64 */
+    CGM_WL_WFA_1_10KB = 3,   /* WFA, 1-10 kb */
+    CGM_WL_MIXED = 4,        /* SW and WFA pairs, WFA up to 20 kb */
+    CGM_WL_SERVING = 5       /* serving pair mix: SW band 64 below 2 kb, WFA above */
+};
+int cgm_gen_create(int32_t workload, int64_t n_pairs, uint64_t seed, void **handle);
 int cgm_gen_sizes(void *handle, int64_t *n_pairs, int64_t *q_words, int64_t *r_words);
 /* Copies the batch into host arrays (any pointer may be NULL to skip it):
  * packed words/NMask words (q_words / r_words entries), offsets and lengths

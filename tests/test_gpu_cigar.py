@@ -91,7 +91,7 @@ def test_cigar_int32_history_rows(cgm, dev, orc):
 
 
 def test_cigar_wfa_20k_sample_equals_oracle(cgm, dev, orc, gen_mod):
-    w = gen_mod.generate(3, 2000)
+    w = gen_mod.generate(gen_mod.WFA_1_10KB, 2000)
     idx = np.nonzero(w.max_error <= 1500)[0]
     idx = idx[np.linspace(0, idx.size - 1, 120).astype(int)]
     s = w.subset(idx)
@@ -133,7 +133,7 @@ def codes(packed, off, length):
 def test_cigar_wfa_20k_full_size_properties(cgm, dev, gen_mod):
     """All 20,000 pairs of 1-10 kb: every CIGAR's cost is the pair's WFA cost
     and its operations walk both sequences exactly."""
-    w = gen_mod.generate(3, 20_000)
+    w = gen_mod.generate(gen_mod.WFA_1_10KB, 20_000)
     cfg = cgm.WFAConfig(4, 6, 2, int(w.max_error.max()))
     print("[case] cigar wfa 20k x 1-10 kb full 20000 pairs", flush=True)
     up = {k: dev.upload(getattr(w, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",

@@ -99,18 +99,18 @@ def test_encode_launch_offsets_bad_entries_and_capacity(cgm, dev, orc):
     assert np.array_equal(p2[:half], e_packed[:half]) and np.all(p2[half:] == sentinel)
 
 
-@pytest.mark.parametrize("config,n", [(2, 200_000), (5, 100_000)])
-def test_encode_round_trip_full_batches(cgm, dev, config, n):
+@pytest.mark.parametrize("workload,n", [("SW_100_300BP", 200_000), ("SERVING", 100_000)])
+def test_encode_round_trip_full_batches(cgm, dev, workload, n):
     """Decoding a whole generated batch to ASCII and encoding it on the GPU
     gives back the generator's words, masks and offsets (queries and
     references)."""
     from cgm_amd import gen
-    w = gen.generate(config, n)
+    w = gen.generate(getattr(gen, workload), n)
     for side, packed, mask, offs, lens in (("query", w.q_packed, w.q_nmask, w.q_off, w.q_len),
                                            ("reference", w.r_packed, w.r_nmask, w.r_off,
                                             w.r_len)):
         seqs = cgm.decode_ascii(packed, mask, offs, lens)
-        print(f"[case] encode round trip config {config} {side} ({len(seqs)} sequences)",
+        print(f"[case] encode round trip {workload} {side} ({len(seqs)} sequences)",
               flush=True)
         got = cgm.encode_batch_gpu(seqs)
         assert np.array_equal(got.offsets, offs), side

@@ -18,10 +18,10 @@ def gen_mod(cgm):
 def mixed_sample(gen, seed=5):
     """SW pairs of 100-300 bp (band 16-64), SW pairs of 1-2 kb (band 128) and
     WFA pairs of 1-10 kb with small costs, interleaved."""
-    sw = gen.generate(2, 3000, seed=seed)
-    long_ = gen.generate(4, 3000, seed=seed)
+    sw = gen.generate(gen.SW_100_300BP, 3000, seed=seed)
+    long_ = gen.generate(gen.MIXED, 3000, seed=seed)
     long_ = long_.subset(np.nonzero((long_.kind == 0) & (long_.band == 128))[0][:300])
-    wfa = gen.generate(3, 600, seed=seed)
+    wfa = gen.generate(gen.WFA_1_10KB, 600, seed=seed)
     wfa = wfa.subset(np.argsort(wfa.max_error)[:250])
     w = gen.concat([sw, long_, wfa])
     perm = np.random.default_rng(seed).permutation(w.n_pairs)
@@ -50,7 +50,7 @@ def test_mixed_100k_full_size(cgm, dev, orc, gen_mod):
     """Mixed batch (100,000 pairs incl. 5-20 kb WFA pairs): the persistent
     layout equals the per-pair kernels on every pair; a sample of SW pairs
     and of the cheaper WFA pairs equals the oracle."""
-    w = gen_mod.generate(4, 100_000)
+    w = gen_mod.generate(gen_mod.MIXED, 100_000)
     print("[case] mixed full 100000 pairs", flush=True)
     auto = cgm.align_mixed_batch(dev, w, sw_cfg(cgm), layout=0)
     plain = cgm.align_mixed_batch(dev, w, sw_cfg(cgm), layout=1)
@@ -135,7 +135,7 @@ def test_mixed_every_tuning_equals_oracle(cgm, dev, orc, gen_mod):
     """Every layout choice (per-wavefront vs per-workgroup WFA, LDS budget,
     tile height, band split, per-lane vs per-workgroup SW) gives the
     oracle's results on a batch that exercises all of them."""
-    big = gen_mod.generate(4, 4000, seed=21)
+    big = gen_mod.generate(gen_mod.MIXED, 4000, seed=21)
     wf = np.nonzero(big.kind == 1)[0]
     big = big.subset(wf[np.argsort(big.max_error[wf])][[0, 5, 20, 60, 150, 250]])
     w = gen_mod.concat([mixed_sample(gen_mod, seed=13), big])

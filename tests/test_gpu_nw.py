@@ -46,7 +46,7 @@ def gen_mod(cgm):
 
 
 def random_pairs(gen_mod, n, seed, band_lo, band_hi):
-    w = gen_mod.generate(2, n, seed=seed)
+    w = gen_mod.generate(gen_mod.SW_100_300BP, n, seed=seed)
     rng = np.random.default_rng(seed)
     w.band[:] = rng.integers(band_lo, band_hi + 1, n)
     return w
@@ -99,7 +99,7 @@ def test_nw_bad_pairs_flagged(cgm, dev):
 def test_nw_batched_workload_full_size(cgm, dev, orc, gen_mod):
     """200,000 pairs (100-300 bp, band 16-64): both layouts agree on every
     pair and a 3,000-pair sample equals the oracle."""
-    w = gen_mod.generate(2, 200_000)
+    w = gen_mod.generate(gen_mod.SW_100_300BP, 200_000)
     sc = CONFIG_SW
     print("[case] nw full 200000 pairs", flush=True)
     auto = cgm.nw_banded_batch(w.q_packed, w.q_off, w.q_len, w.r_packed, w.r_off, w.r_len,

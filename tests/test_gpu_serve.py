@@ -19,7 +19,7 @@ def gen_mod(cgm):
 def serving_pairs(gen_mod):
     """Serving-workload pairs (SW band 64 below 2 kb, WFA above), WFA pairs limited
     to max_error <= 2500 so the oracle check stays short."""
-    w = gen_mod.generate(5, 6000, seed=17)
+    w = gen_mod.generate(gen_mod.SERVING, 6000, seed=17)
     keep = np.nonzero((w.kind == 0) | (w.max_error <= 2500))[0]
     return w.subset(keep)
 

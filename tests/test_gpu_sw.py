@@ -51,7 +51,7 @@ def test_sw_golden_vectors(cgm, dev, layout):
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
 def test_sw_1k_150bp_parity(cgm, dev, orc, gen_mod, layout):
-    w = gen_mod.generate(1, 1000, seed=101)
+    w = gen_mod.generate(gen_mod.SW_150BP, 1000, seed=101)
     print(f"[case] sw 1000 pairs x 150 bp layout={layout}", flush=True)
     got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
     assert_same(got, sw_oracle(orc, w), f"sw 1k x 150 bp {layout}")
@@ -59,7 +59,7 @@ def test_sw_1k_150bp_parity(cgm, dev, orc, gen_mod, layout):
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
 def test_sw_200k_sample_parity(cgm, dev, orc, gen_mod, layout):
-    w = gen_mod.generate(2, 4000, seed=202)
+    w = gen_mod.generate(gen_mod.SW_100_300BP, 4000, seed=202)
     print(f"[case] sw 4000 pairs x 100-300 bp layout={layout}", flush=True)
     got = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
     assert_same(got, sw_oracle(orc, w), f"sw 100-300 bp sample {layout}")
@@ -68,7 +68,7 @@ def test_sw_200k_sample_parity(cgm, dev, orc, gen_mod, layout):
 def test_sw_200k_full_size(cgm, dev, orc, gen_mod):
     """200,000 pairs: every layout agrees on the whole batch, and a random
     sample of 2,000 pairs equals the oracle."""
-    w = gen_mod.generate(2, 200_000)
+    w = gen_mod.generate(gen_mod.SW_100_300BP, 200_000)
     up = Uploaded(dev, w)
     print("[case] sw full 200000 pairs x 100-300 bp", flush=True)
     ref = sw_gpu(cgm, dev, up, LAYOUTS["plain"])
@@ -118,7 +118,7 @@ def test_sw_bad_pairs_are_flagged_not_read(cgm, dev, layout):
 
 
 def test_sw_is_deterministic(cgm, dev, gen_mod):
-    w = gen_mod.generate(2, 20000, seed=9)
+    w = gen_mod.generate(gen_mod.SW_100_300BP, 20000, seed=9)
     up = Uploaded(dev, w)
     a = sw_gpu(cgm, dev, up, LAYOUTS["auto"])
     b = sw_gpu(cgm, dev, up, LAYOUTS["auto"])
@@ -143,7 +143,7 @@ def test_sw_register_lane_forms_equal_oracle(cgm, dev, orc, gen_mod, dual):
                                                         gap_extend=-1),
                             dict(match=3, mismatch=0, gap_open=-2, gap_extend=0),
                             dict(match=5, mismatch=-4, gap_open=0, gap_extend=0))):
-        w = gen_mod.generate(2, 3000, seed=40 + k)
+        w = gen_mod.generate(gen_mod.SW_100_300BP, 3000, seed=40 + k)
         w.band[:] = rng.integers(0, 65, w.n_pairs)
         up = Uploaded(dev, w)
         plan = cgm.MixedPlan(w.n_pairs, 0)
@@ -159,7 +159,7 @@ def test_sw_unit_queue_split_equals_oracle(cgm, dev, orc, gen_mod, split):
     a short queue (0: one queue taken by every wavefront; 100: only the
     first wavefront of each SIMD takes units); every unit is aligned once
     and the results equal the oracle."""
-    w = gen_mod.generate(2, 20000, seed=77)
+    w = gen_mod.generate(gen_mod.SW_100_300BP, 20000, seed=77)
     up = Uploaded(dev, w)
     plan = cgm.MixedPlan(w.n_pairs, 0)
     plan.set_tuning(sw_split=split)

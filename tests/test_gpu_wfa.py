@@ -45,7 +45,7 @@ def test_wfa_golden_vectors(cgm, dev, layout):
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
 def test_wfa_20k_sample_parity(cgm, dev, orc, gen_mod, layout):
-    w = gen_mod.generate(3, 20000)
+    w = gen_mod.generate(gen_mod.WFA_1_10KB, 20000)
     idx = np.random.default_rng(3).choice(w.n_pairs, 160, replace=False)
     sub = w.subset(idx)
     print(f"[case] wfa 1-10 kb sample 160 pairs layout={layout}", flush=True)
@@ -79,7 +79,7 @@ def test_wfa_scalar_max_error_and_bad_pairs(cgm, dev, orc):
 
 
 def test_wfa_is_deterministic(cgm, dev, gen_mod):
-    w = gen_mod.generate(3, 400, seed=4)
+    w = gen_mod.generate(gen_mod.WFA_1_10KB, 400, seed=4)
     up = Uploaded(dev, w)
     assert_same(wfa_gpu(cgm, dev, up, 0), wfa_gpu(cgm, dev, up, 0), "run twice")
 
@@ -88,7 +88,7 @@ def big_wfa_pairs(cgm, gen_mod, orc):
     """Mixed-batch WFA pairs whose wavefronts exceed LDS (max_error 2.6k-13k,
     5-20 kb), the same pairs with max_error just below their cost, and a
     34 kb pair whose rows do not fit int16."""
-    w = gen_mod.generate(4, 20000, seed=11)
+    w = gen_mod.generate(gen_mod.MIXED, 20000, seed=11)
     wf = np.nonzero(w.kind == 1)[0]
     me = w.max_error[wf]
     idx = np.concatenate([wf[(me >= lo) & (me < hi)][:n]
