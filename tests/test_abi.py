@@ -81,6 +81,19 @@ def test_generator_is_deterministic_and_per_pair(cgm):
     assert not np.array_equal(a.r_packed, d.r_packed)
 
 
+def test_workload_ids_match_header(cgm):
+    """The CGM_WL_* ids of cgm_gen_create and the gen module's names agree,
+    and ids outside them are rejected."""
+    from cgm_amd import gen
+    ids = dict(re.findall(r"CGM_WL_(\w+) = (\d+)", open(HEADER).read()))
+    assert len(ids) == 5
+    for name, val in ids.items():
+        assert getattr(gen, name) == int(val), name
+    for bad in (0, max(int(v) for v in ids.values()) + 1):
+        with pytest.raises(cgm._lib.CgmError):
+            gen.generate(bad, 4)
+
+
 def test_generator_matches_workload_definitions(cgm):
     w1 = _gen(cgm, "SW_150BP", 1000)
     assert (w1.r_len == 150).all() and (w1.band == 32).all() and (w1.kind == 0).all()
