@@ -344,7 +344,7 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "bench.py measures one GPU only; --gpus must be 1 "
                           "(pairs are independent: shard a batch across GPUs by calling the "
                           "library once per GPU)"}))
         sys.exit(2)
