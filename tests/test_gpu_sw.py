@@ -117,6 +117,29 @@ def test_sw_bad_pairs_are_flagged_not_read(cgm, dev, layout):
     assert qe.tolist()[1:] == [-1, -1, -1]
 
 
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_sw_band_limit(cgm, dev, orc, layout):
+    """Effective band 1600 (the one-workgroup-per-pair LDS limit) is computed
+    and equals the oracle; 1601 is flagged CGM_BAD_PAIR; a band above both
+    lengths is clamped to the longer length first (5000 -> 1500)."""
+    rng = np.random.default_rng(23)
+    r = "".join(rng.choice(list("ACGT"), 2000))
+    q = list(r)
+    for p in rng.choice(2000, 40, replace=False):
+        q[p] = "ACGT"[("ACGT".index(q[p]) + 1) % 4]
+    q = "".join(q[:700] + q[712:1300]) + "GATTACA" + "".join(q[1300:])
+    pairs = [(q, r), (q, r), (q[:1500], r[:1400])]
+    w = workload_from_strings(pairs, band=[1600, 1601, 5000])
+    print(f"[case] sw band limit layout={layout} lengths {w.q_len.tolist()}/{w.r_len.tolist()}",
+          flush=True)
+    sc, qe, re_ = sw_gpu(cgm, dev, Uploaded(dev, w), LAYOUTS[layout])
+    ref = sw_oracle(orc, w)
+    for i in (0, 2):
+        assert (sc[i], qe[i], re_[i]) == (ref[0][i], ref[1][i], ref[2][i]), i
+    assert sc[0] > 3000
+    assert (sc[1], qe[1], re_[1]) == (-(2 ** 31), -1, -1)
+
+
 def test_sw_is_deterministic(cgm, dev, gen_mod):
     w = gen_mod.generate(gen_mod.SW_100_300BP, 20000, seed=9)
     up = Uploaded(dev, w)
