@@ -43,6 +43,31 @@ def test_library_reports_errors_without_gpu(cgm):
     assert lib.cgm_version() >= 100
 
 
+def test_parameter_limits_checked_before_any_device_call(cgm):
+    """WFA penalties: x, e in [1, 4096], o in [0, 4096]; SW scoring values
+    |v| <= 2^20.  Zero-pair calls at the limits succeed without a GPU."""
+    lib = cgm._lib.load()
+    invalid = cgm._lib.CGM_ERR_INVALID
+
+    def wfa(x, o, e):
+        return lib.cgm_wfa_affine_launch(None, None, None, None, None, None, 0, 0, 0, None, 10,
+                                         x, o, e, None, 0, None)
+
+    def sw(match, mismatch, gap_open, gap_extend):
+        return lib.cgm_sw_banded_launch(None, None, None, None, None, None, 0, 0, 0, None, 4,
+                                        match, mismatch, gap_open, gap_extend, None, None, None,
+                                        0, None)
+
+    assert wfa(4096, 4096, 4096) == 0 and wfa(1, 0, 1) == 0
+    for pen in [(4097, 6, 2), (4, 4097, 2), (4, 6, 4097), (4, -1, 2), (4, 6, 0), (0, 6, 2)]:
+        assert wfa(*pen) == invalid, pen
+    lim = 1 << 20
+    assert sw(lim, -lim, -lim, -lim) == 0
+    for sc in [(lim + 1, -4, -4, -2), (2, -lim - 1, -4, -2), (2, -4, -lim - 1, -2),
+               (2, -4, -4, -lim - 1)]:
+        assert sw(*sc) == invalid, sc
+
+
 def test_sequence_batch_matches_oracle_packing(cgm):
     seqs = ["ACGT", "acgtN-ACGT", "A" * 33, "", "TTTTGGGGCCCCAAAANNNN" * 5]
     b = cgm.SequenceBatch()
