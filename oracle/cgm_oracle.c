@@ -12,6 +12,9 @@
  *   or_wfa_affine        genomics/cpu/align_cpu.mojo:75-226   (wfa_affine_cpu)
  *   or_wfa_affine_cigar  genomics/cpu/align_cpu.mojo:380-623  (wfa_affine_cigar_cpu)
  *   or_nw_banded         genomics/cpu/align_cpu.mojo:312-377  (needleman_wunsch_banded)
+ *   or_sw_banded_start   or_sw_banded plus start coordinates, which the
+ *                        reference hard-wires to 0 (align_cpu.mojo:302-305);
+ *                        the definition is the library's own (see below)
  * Independent cross-checks (not restatements; used to pin the above):
  *   or_sw_antidiag       the same SW recurrence swept by anti-diagonals with
  *                        an explicit (H desc, qi asc, ri asc) reduction
@@ -40,12 +43,19 @@
  *     width entries, which is what the indexing intends.
  *   or_sw_banded: reference known answers test_alignment.mojo:24-58, closed
  *     forms (identical pair, no common base, empty input, band 0, one
- *     substitution), invariants (band monotonicity, single-diagonal Kadane
- *     lower bound), equality with or_sw_antidiag, and a path-model brute force
- *     on tiny inputs.  The non-textbook E/F recurrence itself (E opens from the
- *     cell above and extends from the left; F never extends) can only be
- *     confirmed by reading align_cpu.mojo:259-300, because no Mojo toolchain
- *     is available to run the reference.
+ *     substitution, a deletion or insertion of l bases planted in an
+ *     identical pair), invariants (band monotonicity, single-diagonal Kadane
+ *     lower bound), equality with or_sw_antidiag (visiting order and
+ *     tie-break), and equality with an explicit enumeration of every local
+ *     alignment of tiny pairs under the path model of align_cpu.mojo:277-289
+ *     (E opens with a step down and extends to the right, F is one step
+ *     right that never extends).  Copies of the loop with a planted mistake
+ *     (textbook Gotoh, an extending F, a dropped term, a transposed index, a
+ *     tie-break by >=) fail those checks (tests/test_oracle_sw_gaps.py).
+ *   or_sw_banded_start: the enumeration above, reporting for the end cell
+ *     the latest first diagonal step among its best alignments; closed forms
+ *     (an identical k-mer between flanks that share no base, band 0 =
+ *     Kadane's latest start); a planted earliest-start tie-break fails them.
  *
  * Integer widths: the reference computes in Mojo Int (64-bit); so does this.
  */
@@ -134,6 +144,75 @@ void or_sw_banded(const uint64_t *q, int64_t qlen, const uint64_t *r, int64_t rl
     }
     free(H_prev); free(H_curr); free(E_curr);
     out[0] = best; out[1] = bq; out[2] = br;
+}
+
+/* SW start coordinates.  The reference reports query_start = ref_start = 0
+ * (align_cpu.mojo:302-305) and has no definition of its own, so this one is
+ * the library's: among the alignments of maximal score that end in H at the
+ * cell or_sw_banded reports, the start is the first diagonal step (the
+ * first query and reference base the alignment pairs) of the one whose
+ * start is latest -- largest query start, then largest reference start.
+ * Computed by the row-major loop of or_sw_banded carrying, next to each H
+ * and E value, the start of the path that value belongs to, with ties
+ * between equal values resolved towards the later start; a path with no
+ * diagonal step yet (a fresh start, or gaps only) carries no start and
+ * ranks above every real start of the same value.  Requires the penalties
+ * of every configuration (mismatch, gap_extend, gap_open + gap_extend <= 0),
+ * under which leading gaps never raise a score.  out[0..4] = score,
+ * query_start, ref_start, query_end, ref_end ((0,0,0,0,0) when no cell is
+ * positive). */
+typedef struct { int64_t v, qs, rs; } or_cand;
+#define OR_NO_START INT64_MAX
+
+static inline int or_cand_gt(or_cand a, or_cand b) {
+    if (a.v != b.v) return a.v > b.v;
+    if (a.qs != b.qs) return a.qs > b.qs;
+    return a.rs > b.rs;
+}
+static inline or_cand or_cand_max(or_cand a, or_cand b) { return or_cand_gt(b, a) ? b : a; }
+static inline or_cand or_cand_add(or_cand a, int64_t d) { a.v += d; return a; }
+
+void or_sw_banded_start(const uint64_t *q, int64_t qlen, const uint64_t *r, int64_t rlen,
+                        int64_t match, int64_t mismatch, int64_t gap_open, int64_t gap_extend,
+                        int64_t band, int64_t out[5]) {
+    int64_t width = 2 * band + 1;
+    size_t n = (size_t)(width > 0 ? width : 1);
+    or_cand *Hp = (or_cand *)malloc(n * sizeof(or_cand));
+    or_cand *Hc = (or_cand *)malloc(n * sizeof(or_cand));
+    or_cand *Ec = (or_cand *)malloc(n * sizeof(or_cand));
+    const or_cand zero = {0, OR_NO_START, OR_NO_START};
+    for (int64_t j = 0; j < width; ++j) Hp[j] = zero;
+    or_cand best = zero;
+    int64_t bq = 0, br = 0;
+    const int64_t oe = gap_open + gap_extend;
+    for (int64_t qi = 0; qi < qlen; ++qi) {
+        for (int64_t j = 0; j < width; ++j) { Hc[j] = zero; Ec[j] = zero; }
+        for (int64_t j = 0; j < width; ++j) {
+            int64_t ri = qi - band + j;
+            if (ri < 0 || ri >= rlen) continue;
+            int64_t sub = or_get_base(q, qi) == or_get_base(r, ri) ? match : mismatch;
+            or_cand d = or_cand_add(Hp[j], sub);
+            if (Hp[j].qs == OR_NO_START) { d.qs = qi; d.rs = ri; } /* first diagonal step */
+            or_cand up = (j + 1 < width) ? Hp[j + 1] : zero;
+            or_cand left = (j > 0) ? Hc[j - 1] : zero;
+            or_cand el = (j > 0) ? or_cand_add(Ec[j - 1], gap_extend)
+                                 : (or_cand){OR_NEG_INF, OR_NO_START, OR_NO_START};
+            or_cand e = or_cand_max(el, or_cand_add(up, oe));
+            or_cand f = or_cand_add(left, oe);
+            or_cand h = or_cand_max(or_cand_max(zero, d), or_cand_max(e, f));
+            if (h.v == 0) h = zero; /* a zero-valued path restarts here */
+            Hc[j] = h;
+            Ec[j] = e;
+            if (h.v > best.v) { best = h; bq = qi; br = ri; } /* or_sw_banded's end cell */
+        }
+        or_cand *t = Hp; Hp = Hc; Hc = t;
+    }
+    free(Hp); free(Hc); free(Ec);
+    out[0] = best.v;
+    out[1] = best.v > 0 ? best.qs : 0;
+    out[2] = best.v > 0 ? best.rs : 0;
+    out[3] = bq;
+    out[4] = br;
 }
 
 /* The same recurrence evaluated anti-diagonal by anti-diagonal over a full
@@ -587,6 +666,7 @@ typedef struct {
     int64_t match, mismatch, gap_open, gap_extend;
     int64_t x, o, e;
     int32_t *score, *q_end, *r_end;
+    int32_t *q_start, *r_start; /* mode 3 only */
     int64_t next_pair; pthread_mutex_t *lock; int mode;
 } or_job;
 
@@ -601,6 +681,18 @@ static void or_do_pair(const or_job *j, int64_t i) {
         j->score[i] = (int32_t)out[0];
         if (j->q_end) j->q_end[i] = (int32_t)out[1];
         if (j->r_end) j->r_end[i] = (int32_t)out[2];
+        return;
+    }
+    if (j->mode == 3) {
+        int64_t band = j->band ? j->band[i] : j->band_scalar;
+        int64_t out[5];
+        or_sw_banded_start(q, j->ql[i], r, j->rl[i], j->match, j->mismatch, j->gap_open,
+                           j->gap_extend, band, out);
+        j->score[i] = (int32_t)out[0];
+        j->q_start[i] = (int32_t)out[1];
+        j->r_start[i] = (int32_t)out[2];
+        j->q_end[i] = (int32_t)out[3];
+        j->r_end[i] = (int32_t)out[4];
         return;
     }
     int is_wfa = j->kind ? j->kind[i] != 0 : j->mode == 1;
@@ -633,10 +725,23 @@ static void *or_worker(void *arg) {
     return NULL;
 }
 
+static int or_run_job(or_job *j, int64_t n_pairs, int n_threads) {
+    if (n_threads <= 1) {
+        for (int64_t i = 0; i < n_pairs; ++i) or_do_pair(j, i);
+        return 0;
+    }
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
+    for (int t = 0; t < n_threads; ++t) pthread_create(&th[t], NULL, or_worker, j);
+    for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    return 0;
+}
+
 /* Runs every pair of a batch with n_threads POSIX threads pulling pair
  * indices from a shared counter.  kind[i] (0 = SW, 1 = WFA) may be NULL, in
  * which case mode selects: 0 = all SW, 1 = all WFA; mode 2 = all banded NW
- * (kind ignored). */
+ * (kind ignored); mode 3 = all SW with start coordinates
+ * (or_sw_start_batch). */
 int or_batch(int mode,
              const uint64_t *q_packed, const uint64_t *r_packed,
              const int32_t *q_off, const int32_t *r_off,
@@ -659,13 +764,24 @@ int or_batch(int mode,
     j.x = x; j.o = o; j.e = e;
     j.score = score; j.q_end = q_end; j.r_end = r_end;
     j.next_pair = 0; j.lock = &lock; j.mode = mode;
-    if (n_threads <= 1) {
-        for (int64_t i = 0; i < n_pairs; ++i) or_do_pair(&j, i);
-        return 0;
-    }
-    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
-    for (int t = 0; t < n_threads; ++t) pthread_create(&th[t], NULL, or_worker, &j);
-    for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
-    free(th);
-    return 0;
+    return or_run_job(&j, n_pairs, n_threads);
+}
+
+/* SW with start coordinates (or_sw_banded_start) over a whole batch. */
+int or_sw_start_batch(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_off,
+                      const int32_t *r_off, const int32_t *q_len, const int32_t *r_len,
+                      int64_t n_pairs, const int32_t *band, int32_t band_scalar, int32_t match,
+                      int32_t mismatch, int32_t gap_open, int32_t gap_extend, int32_t *score,
+                      int32_t *q_start, int32_t *r_start, int32_t *q_end, int32_t *r_end,
+                      int n_threads) {
+    pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
+    or_job j;
+    memset(&j, 0, sizeof(j));
+    j.qp = q_packed; j.rp = r_packed; j.qo = q_off; j.ro = r_off; j.ql = q_len; j.rl = r_len;
+    j.band = band; j.band_scalar = band_scalar;
+    j.n = n_pairs;
+    j.match = match; j.mismatch = mismatch; j.gap_open = gap_open; j.gap_extend = gap_extend;
+    j.score = score; j.q_end = q_end; j.r_end = r_end; j.q_start = q_start; j.r_start = r_start;
+    j.next_pair = 0; j.lock = &lock; j.mode = 3;
+    return or_run_job(&j, n_pairs, n_threads);
 }

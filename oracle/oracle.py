@@ -8,7 +8,8 @@ Functions follow the reference Mojo sources:
   sw_banded    genomics/cpu/align_cpu.mojo:229-309 (smith_waterman_banded)
   wfa_affine   genomics/cpu/align_cpu.mojo:75-226  (wfa_affine_cpu)
   pack_ascii   genomics/core/sequence.mojo:38-66   (SequenceBatch.add_sequence)
-plus the independent cross-checks sw_antidiag and gotoh_global.
+plus sw_banded_start (SW with the library's start coordinates) and the
+independent cross-checks sw_antidiag and gotoh_global.
 """
 from __future__ import annotations
 
@@ -59,6 +60,11 @@ def lib():
         L.or_wfa_cigar_batch.restype = ctypes.c_int
         L.or_wfa_cigar_batch.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _i32,
                                          _i32, _vp, _vp, _vp, _vp, ctypes.c_int]
+        L.or_sw_banded_start.restype = None
+        L.or_sw_banded_start.argtypes = [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp]
+        L.or_sw_start_batch.restype = ctypes.c_int
+        L.or_sw_start_batch.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _i32,
+                                        _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
         L.or_batch.restype = ctypes.c_int
         L.or_batch.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
                                _vp, _i32, _i32, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _i32,
@@ -102,6 +108,36 @@ def sw_banded(q, r, match=2, mismatch=-3, gap_open=-5, gap_extend=-2, band=32):
     lib().or_sw_banded(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl,
                        match, mismatch, gap_open, gap_extend, band, _ptr(out))
     return int(out[0]), int(out[1]), int(out[2])
+
+
+def sw_banded_start(q, r, match=2, mismatch=-3, gap_open=-5, gap_extend=-2, band=32):
+    """(score, query_start, ref_start, query_end, ref_end): sw_banded plus the
+    start of the latest-starting best alignment ending at the end cell (the
+    reference reports starts 0, align_cpu.mojo:302-305)."""
+    qw, ql = _words(q)
+    rw, rl = _words(r)
+    out = np.zeros(5, dtype=np.int64)
+    lib().or_sw_banded_start(_ptr(_nonempty(qw)), ql, _ptr(_nonempty(rw)), rl,
+                             match, mismatch, gap_open, gap_extend, band, _ptr(out))
+    return tuple(int(v) for v in out)
+
+
+def sw_start_batch(q_packed, r_packed, q_off, r_off, q_len, r_len, *, band=None, band_scalar=-1,
+                   match=2, mismatch=-4, gap_open=-4, gap_extend=-2, threads=1):
+    """Whole-batch sw_banded_start: (score, q_start, r_start, q_end, r_end)
+    int32 arrays."""
+    n = int(len(q_len))
+    outs = [np.zeros(n, dtype=np.int32) for _ in range(5)]
+    a = [np.ascontiguousarray(x) for x in (q_packed, r_packed)]
+    i32 = [None if x is None else np.ascontiguousarray(x, dtype=np.int32)
+           for x in (q_off, r_off, q_len, r_len, band)]
+    rc = lib().or_sw_start_batch(_ptr(_nonempty(a[0])), _ptr(_nonempty(a[1])), _ptr(i32[0]),
+                                 _ptr(i32[1]), _ptr(i32[2]), _ptr(i32[3]), n, _ptr(i32[4]),
+                                 band_scalar, match, mismatch, gap_open, gap_extend,
+                                 *[_ptr(o) for o in outs], int(threads))
+    if rc != 0:
+        raise RuntimeError("oracle batch failed")
+    return tuple(outs)
 
 
 def sw_antidiag(q, r, match=2, mismatch=-3, gap_open=-5, gap_extend=-2, band=32):
