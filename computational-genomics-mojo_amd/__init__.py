@@ -14,7 +14,8 @@ from .align import (AlignConfig, AlignResult, MixedPlan, SwResult, WFAConfig, al
                     smith_waterman_banded, sw_banded_batch, wfa_affine, wfa_affine_batch,
                     wfa_affine_cpu, WFAResult, cigar_capacity, runs_to_cigar,
                     wfa_affine_cigar_batch, wfa_affine_cigar_cpu, needleman_wunsch_banded,
-                    nw_banded_batch)
+                    nw_banded_batch, SwStartResult, launch_banded_sw_start,
+                    sw_banded_start_batch)
 from .device import DeviceArray, GenomicsDevice, Timer
 from .serve import GeneratorHandle, ServingEngine
 from .sequence import (BASES_PER_WORD, Base, SequenceBatch, SequenceView, ascii_to_2bit,
@@ -31,4 +32,5 @@ __all__ = [
     "pack_codes", "concat_ascii", "decode_ascii", "WFAResult", "cigar_capacity",
     "runs_to_cigar", "wfa_affine_cigar_batch", "wfa_affine_cigar_cpu",
     "needleman_wunsch_banded", "nw_banded_batch", "encode_ascii_launch", "encode_batch_gpu",
+    "SwStartResult", "launch_banded_sw_start", "sw_banded_start_batch",
 ]

@@ -63,6 +63,10 @@ SIGNATURES = {
     "cgm_event_elapsed_ms": [_vp, _vp, _pf],
     "cgm_sw_banded_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                              _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp],
+    "cgm_sw_banded_start_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
+                                   _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp],
+    "cgm_sw_banded_start_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
+                                 _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32],
     "cgm_wfa_affine_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                               _i32, _i32, _i32, _vp, _i32, _vp],
     "cgm_plan_create": [_i64, _i32, _i32, _i32, _i32, _i32, _i32, _pvp],

@@ -76,7 +76,9 @@ def sw_banded_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignCo
                     band=None, layout=_lib.LAYOUT_AUTO):
     """Aligns pair i = (query i, reference i) for every i; host arrays in,
     host arrays (score, query_end, ref_end) out.  band: per-pair array or
-    None for cfg.band_width."""
+    None for cfg.band_width.  A pair the library refuses (lengths/offsets
+    outside the buffers, effective band above 1600, match * min(qlen, rlen)
+    > 2^30) gets score CGM_BAD_PAIR (-2^31) and ends -1."""
     q_packed = np.ascontiguousarray(q_packed, np.uint64)
     r_packed = np.ascontiguousarray(r_packed, np.uint64)
     ql, rl, qo, ro, bd = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(band)
@@ -93,10 +95,32 @@ def sw_banded_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignCo
     return score, qe, re_
 
 
+def sw_banded_start_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignConfig,
+                          band=None, layout=_lib.LAYOUT_AUTO):
+    """sw_banded_batch plus start coordinates: host arrays (score,
+    query_start, ref_start, query_end, ref_end).  The start is that of the
+    latest-starting best alignment ending at the reported end (the
+    reference reports 0, align_cpu.mojo:302-305); refused pairs carry
+    CGM_BAD_PAIR and -1 as in sw_banded_batch."""
+    q_packed = np.ascontiguousarray(q_packed, np.uint64)
+    r_packed = np.ascontiguousarray(r_packed, np.uint64)
+    ql, rl, qo, ro, bd = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(band)
+    n = int(ql.size)
+    outs = [np.zeros(n, np.int32) for _ in range(5)]
+    if n:
+        _lib.call("cgm_sw_banded_start_host", _lib.ptr(_nonempty(q_packed)),
+                  _lib.ptr(_nonempty(r_packed)), _lib.ptr(ql), _lib.ptr(rl), _lib.ptr(qo),
+                  _lib.ptr(ro), n, q_packed.size, r_packed.size, _lib.ptr(bd), cfg.band_width,
+                  cfg.match_score, cfg.mismatch, cfg.gap_open, cfg.gap_extend,
+                  *[_lib.ptr(o) for o in outs], layout)
+    return tuple(outs)
+
+
 def nw_banded_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignConfig,
                     band=None, layout=_lib.LAYOUT_AUTO):
     """Banded NW (needleman_wunsch_banded, align_cpu.mojo:312-377) of every
-    pair on the GPU; host arrays in, (score, query_end, ref_end) out."""
+    pair on the GPU; host arrays in, (score, query_end, ref_end) out.
+    Refused pairs get score CGM_BAD_PAIR (-2^31) and ends -1."""
     q_packed = np.ascontiguousarray(q_packed, np.uint64)
     r_packed = np.ascontiguousarray(r_packed, np.uint64)
     ql, rl, qo, ro, bd = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(band)
@@ -113,7 +137,9 @@ def nw_banded_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: AlignCo
 
 def wfa_affine_batch(q_packed, q_off, q_len, r_packed, r_off, r_len, cfg: WFAConfig,
                      max_error=None, layout=_lib.LAYOUT_AUTO):
-    """Gap-affine WFA cost (or -1) of every pair; host arrays in and out."""
+    """Gap-affine WFA cost (or -1) of every pair; host arrays in and out.
+    Refused pairs (lengths/offsets outside the buffers, max_error < 0 or
+    above 2^20) get CGM_BAD_PAIR (-2^31)."""
     q_packed = np.ascontiguousarray(q_packed, np.uint64)
     r_packed = np.ascontiguousarray(r_packed, np.uint64)
     ql, rl, qo, ro, me = _i32(q_len), _i32(r_len), _i32(q_off), _i32(r_off), _i32(max_error)
@@ -189,12 +215,28 @@ def _view_arrays(v: SequenceView):
         np.array([v.length], np.int32)
 
 
+def _refused(score, what):
+    """The single-pair signatures return a result or raise, as the
+    reference returns a result: a pair the library refuses (effective band
+    above 1600, match * min(qlen, rlen) > 2^30, max_error > 2^20) raises."""
+    if int(score) == _lib.CGM_BAD_PAIR:
+        raise _lib.CgmError(_lib.CGM_ERR_INVALID, f"{what}: pair outside the supported limits")
+
+
 def smith_waterman_banded(query: SequenceView, reference: SequenceView,
-                          cfg: AlignConfig) -> AlignResult:
-    """align_cpu.mojo:229-309, computed by the GPU kernel."""
+                          cfg: AlignConfig, with_start: bool = False) -> AlignResult:
+    """align_cpu.mojo:229-309, computed by the GPU kernel.  query_start and
+    ref_start are 0 as in the reference unless with_start is set; then they
+    hold the start of the latest-starting best alignment that ends at
+    (query_end, ref_end)."""
     qp, qo, ql = _view_arrays(query)
     rp, ro, rl = _view_arrays(reference)
+    if with_start:
+        s, qs, rs, qe, re_ = sw_banded_start_batch(qp, qo, ql, rp, ro, rl, cfg)
+        _refused(s[0], "smith_waterman_banded")
+        return AlignResult(int(s[0]), int(qs[0]), int(rs[0]), int(qe[0]), int(re_[0]), [])
     s, qe, re_ = sw_banded_batch(qp, qo, ql, rp, ro, rl, cfg)
+    _refused(s[0], "smith_waterman_banded")
     return AlignResult(int(s[0]), 0, 0, int(qe[0]), int(re_[0]), [])
 
 
@@ -204,6 +246,7 @@ def needleman_wunsch_banded(query: SequenceView, reference: SequenceView,
     qp, qo, ql = _view_arrays(query)
     rp, ro, rl = _view_arrays(reference)
     s, qe, re_ = nw_banded_batch(qp, qo, ql, rp, ro, rl, cfg)
+    _refused(s[0], "needleman_wunsch_banded")
     return AlignResult(int(s[0]), 0, 0, int(qe[0]), int(re_[0]), [])
 
 
@@ -212,7 +255,9 @@ def wfa_affine_cpu(query: SequenceView, ref_seq: SequenceView, cfg: WFAConfig) -
     kernel."""
     qp, qo, ql = _view_arrays(query)
     rp, ro, rl = _view_arrays(ref_seq)
-    return int(wfa_affine_batch(qp, qo, ql, rp, ro, rl, cfg)[0])
+    c = int(wfa_affine_batch(qp, qo, ql, rp, ro, rl, cfg)[0])
+    _refused(c, "wfa_affine_cpu")
+    return c
 
 
 wfa_affine = wfa_affine_cpu
@@ -245,6 +290,36 @@ def launch_banded_sw(device: GenomicsDevice, queries_buf: DeviceArray, refs_buf:
               rw, band_buf.dptr if band_buf is not None else None, band, cfg.match_score,
               cfg.mismatch, cfg.gap_open, cfg.gap_extend, out.scores.dptr, out.query_end.dptr,
               out.ref_end.dptr, layout, device.stream)
+    return out
+
+
+class SwStartResult(SwResult):
+    """Device outputs of launch_banded_sw_start: SwResult plus starts."""
+
+    def __init__(self, n: int):
+        super().__init__(n)
+        self.query_start = DeviceArray(max(n, 1), np.int32)
+        self.ref_start = DeviceArray(max(n, 1), np.int32)
+
+
+def launch_banded_sw_start(device: GenomicsDevice, queries_buf: DeviceArray,
+                           refs_buf: DeviceArray, q_lengths_buf: DeviceArray,
+                           r_lengths_buf: DeviceArray, q_offsets_buf: DeviceArray,
+                           r_offsets_buf: DeviceArray, n_pairs: int, total_words: int,
+                           band: int = 64, cfg: AlignConfig | None = None,
+                           band_buf: DeviceArray | None = None,
+                           r_total_words: int | None = None,
+                           layout: int = _lib.LAYOUT_AUTO) -> SwStartResult:
+    """launch_banded_sw (align_gpu.mojo:460-493) plus start coordinates
+    (cgm_sw_banded_start_launch).  Asynchronous on device.stream."""
+    cfg = cfg or default_config()
+    out = SwStartResult(n_pairs)
+    rw = total_words if r_total_words is None else r_total_words
+    _lib.call("cgm_sw_banded_start_launch", queries_buf.dptr, refs_buf.dptr, q_lengths_buf.dptr,
+              r_lengths_buf.dptr, q_offsets_buf.dptr, r_offsets_buf.dptr, n_pairs, total_words,
+              rw, band_buf.dptr if band_buf is not None else None, band, cfg.match_score,
+              cfg.mismatch, cfg.gap_open, cfg.gap_extend, out.scores.dptr, out.query_start.dptr,
+              out.ref_start.dptr, out.query_end.dptr, out.ref_end.dptr, layout, device.stream)
     return out
 
 

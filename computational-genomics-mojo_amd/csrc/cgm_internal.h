@@ -203,6 +203,11 @@ int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
                    SwOut out, hipStream_t stream);
 int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                       SwOut out, hipStream_t stream);
+// Start coordinates of SW alignments whose score and ends are already in
+// score / q_end / r_end (cgm_sw_start.hip).
+int launch_sw_start(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                    const int32_t *score, const int32_t *q_end, const int32_t *r_end,
+                    int32_t *q_start, int32_t *r_start, hipStream_t stream);
 int device_cu_count(int *cus);
 
 // Fork/join of auxiliary streams (default priority, non-blocking) so that
@@ -212,7 +217,12 @@ constexpr int kMaxAuxStreams = 4;
 int fork_streams(hipStream_t main, int n, hipStream_t *aux);
 int join_streams(hipStream_t main, int n, hipStream_t *aux);
 
+// SW scoring: |value| <= 2^20 and non-positive penalties (mismatch,
+// gap_extend, gap_open + gap_extend <= 0), as AlignConfig documents
+// (align_cpu.mojo:18-25).
 int check_sw_scoring(const SwScoring &sc);
+// Magnitudes only (banded NW bounds its values per pair instead).
+int check_score_magnitudes(const SwScoring &sc);
 int check_wfa_pen(const WfaPen &p);
 int check_batch(const Batch &b);
 

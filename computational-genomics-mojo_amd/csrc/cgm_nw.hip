@@ -403,7 +403,7 @@ int cgm_nw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed, con
     const Batch b = {q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words, r_words};
     const SwScoring sc = {match, mismatch, gap_open, gap_extend};
     if (int rc = check_batch(b)) return rc;
-    if (int rc = check_sw_scoring(sc)) return rc;
+    if (int rc = check_score_magnitudes(sc)) return rc;
     if (n_pairs == 0) return CGM_OK;
     if (!out_score || !out_q_end || !out_r_end)
         return set_error(CGM_ERR_INVALID, "nw: null output buffer");

@@ -27,11 +27,21 @@ int hip_error(hipError_t e, const char *what) {
                      m);
 }
 
-int check_sw_scoring(const SwScoring &sc) {
+int check_score_magnitudes(const SwScoring &sc) {
     const int32_t v[4] = {sc.match, sc.mismatch, sc.gap_open, sc.gap_extend};
     for (int32_t x : v)
         if (x > kMaxScoreMagnitude || x < -kMaxScoreMagnitude)
             return set_error(CGM_ERR_INVALID, "SW scoring value out of range (|v| <= 2^20)");
+    return CGM_OK;
+}
+
+int check_sw_scoring(const SwScoring &sc) {
+    if (int rc = check_score_magnitudes(sc)) return rc;
+    // Non-positive penalties bound every score by match * min(qlen, rlen),
+    // which the kernels check per pair against int32 range.
+    if (sc.mismatch > 0 || sc.gap_extend > 0 || sc.gap_open + sc.gap_extend > 0)
+        return set_error(CGM_ERR_INVALID,
+                         "SW penalties must be <= 0: mismatch, gap_extend and gap_open + gap_extend");
     return CGM_OK;
 }
 
@@ -577,6 +587,48 @@ int cgm_sw_banded_host(const uint64_t *q_packed, const uint64_t *r_packed, const
     CGM_HIP(hipMemcpyAsync(out_score, os, n_pairs * 4, hipMemcpyDeviceToHost, db.stream));
     CGM_HIP(hipMemcpyAsync(out_q_end, oq, n_pairs * 4, hipMemcpyDeviceToHost, db.stream));
     CGM_HIP(hipMemcpyAsync(out_r_end, orr, n_pairs * 4, hipMemcpyDeviceToHost, db.stream));
+    CGM_HIP(hipStreamSynchronize(db.stream));
+    return CGM_OK;
+}
+
+int cgm_sw_banded_start_host(const uint64_t *q_packed, const uint64_t *r_packed,
+                             const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                             const int32_t *r_off, int64_t n_pairs, int64_t q_words,
+                             int64_t r_words, const int32_t *band, int32_t band_scalar,
+                             int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                             int32_t *out_score, int32_t *out_q_start, int32_t *out_r_start,
+                             int32_t *out_q_end, int32_t *out_r_end, int32_t layout) {
+    if (n_pairs < 0) return set_error(CGM_ERR_INVALID, "n_pairs < 0");
+    if (n_pairs == 0) return CGM_OK;
+    if (!out_score || !out_q_start || !out_r_start || !out_q_end || !out_r_end)
+        return set_error(CGM_ERR_INVALID, "null output array");
+    DeviceBatch db;
+    std::unique_lock<std::mutex> host_lock;
+    if (int rc = host_stream(&db.stream, &host_lock)) return rc;
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamSynchronize(s); }
+    } guard{db.stream};
+    uint64_t *qp, *rp;
+    int32_t *ql, *rl, *qo, *ro, *bd = nullptr, *o[5];
+    int rc = CGM_OK;
+    if ((rc = db.upload(q_packed, q_words, &qp)) || (rc = db.upload(r_packed, r_words, &rp)) ||
+        (rc = db.upload(q_len, n_pairs, &ql)) || (rc = db.upload(r_len, n_pairs, &rl)) ||
+        (rc = db.upload(q_off, n_pairs, &qo)) || (rc = db.upload(r_off, n_pairs, &ro)) ||
+        (rc = db.upload(band, band ? n_pairs : 0, &bd)))
+        return rc;
+    for (int k = 0; k < 5; ++k)
+        if ((rc = db.alloc(n_pairs, &o[k]))) return rc;
+    rc = cgm_sw_banded_start_launch(qp, rp, ql, rl, qo, ro, n_pairs, q_words, r_words, bd,
+                                    band_scalar, match, mismatch, gap_open, gap_extend, o[0], o[1],
+                                    o[2], o[3], o[4], layout, db.stream);
+    if (rc) {
+        (void)hipStreamSynchronize(db.stream);
+        return rc;
+    }
+    int32_t *host[5] = {out_score, out_q_start, out_r_start, out_q_end, out_r_end};
+    for (int k = 0; k < 5; ++k)
+        CGM_HIP(hipMemcpyAsync(host[k], o[k], n_pairs * 4, hipMemcpyDeviceToHost, db.stream));
     CGM_HIP(hipStreamSynchronize(db.stream));
     return CGM_OK;
 }

@@ -98,9 +98,12 @@ int cgm_event_elapsed_ms(void *start, void *stop, float *ms);
  * out_score = best H, out_q_end/out_r_end = first cell reaching it in
  * row-major order (query_start = ref_start = 0 and the CIGAR is empty, as in
  * the reference).  band: per-pair device array, or NULL to use band_scalar
- * for every pair.  Scoring: match > 0 expected, penalties <= 0 (AlignConfig,
- * align_cpu.mojo:18-25); |value| <= 2^20.  The reference's band is a
- * compile-time parameter; here it is a runtime value per pair. */
+ * for every pair.  Scoring: |value| <= 2^20 and penalties <= 0 as AlignConfig
+ * documents (align_cpu.mojo:18-25): mismatch > 0, gap_extend > 0 or
+ * gap_open + gap_extend > 0 is refused with CGM_ERR_INVALID.  A pair with
+ * match * min(qlen, rlen) > 2^30 or an effective band min(band, max(qlen,
+ * rlen)) above 1600 is reported with CGM_BAD_PAIR.  The reference's band is
+ * a compile-time parameter; here it is a runtime value per pair. */
 int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                          const int32_t *q_len, const int32_t *r_len,
                          const int32_t *q_off, const int32_t *r_off,
@@ -109,6 +112,35 @@ int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                          int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
                          int32_t *out_score, int32_t *out_q_end, int32_t *out_r_end,
                          int32_t layout, void *stream);
+
+/* ------------------------------------------- banded SW with start coordinates
+ * cgm_sw_banded_launch plus the start of each alignment, which the
+ * reference reports as query_start = ref_start = 0 (smith_waterman_banded,
+ * genomics/cpu/align_cpu.mojo:302-305) and does not define.  Here: among
+ * the alignments of maximal score that end in H at (out_q_end, out_r_end),
+ * the start of the one whose first diagonal step (the first query and
+ * reference base it pairs) is latest -- largest query start, then largest
+ * reference start.  Score and ends are exactly those of
+ * cgm_sw_banded_launch (same layouts, same arguments).  Starts are (0, 0)
+ * when the score is 0, -1 for a bad pair, and -1 when
+ * (q_end + 2) * (2 * band_eff + 1) exceeds 2^32.  Runs a second pass over
+ * rows 0 .. q_end of each pair; asynchronous on `stream`. */
+int cgm_sw_banded_start_launch(const uint64_t *q_packed, const uint64_t *r_packed,
+                               const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                               const int32_t *r_off, int64_t n_pairs, int64_t q_words,
+                               int64_t r_words, const int32_t *band, int32_t band_scalar,
+                               int32_t match, int32_t mismatch, int32_t gap_open,
+                               int32_t gap_extend, int32_t *out_score, int32_t *out_q_start,
+                               int32_t *out_r_start, int32_t *out_q_end, int32_t *out_r_end,
+                               int32_t layout, void *stream);
+/* Host-pointer form (copies in, runs on an internal stream, copies out). */
+int cgm_sw_banded_start_host(const uint64_t *q_packed, const uint64_t *r_packed,
+                             const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                             const int32_t *r_off, int64_t n_pairs, int64_t q_words,
+                             int64_t r_words, const int32_t *band, int32_t band_scalar,
+                             int32_t match, int32_t mismatch, int32_t gap_open, int32_t gap_extend,
+                             int32_t *out_score, int32_t *out_q_start, int32_t *out_r_start,
+                             int32_t *out_q_end, int32_t *out_r_end, int32_t layout);
 
 /* ------------------------------------------------------- banded NW batch
  * The batch form of needleman_wunsch_banded (genomics/cpu/align_cpu.mojo:

@@ -66,6 +66,26 @@ def test_parameter_limits_checked_before_any_device_call(cgm):
     for sc in [(lim + 1, -4, -4, -2), (2, -lim - 1, -4, -2), (2, -4, -lim - 1, -2),
                (2, -4, -4, -lim - 1)]:
         assert sw(*sc) == invalid, sc
+    # penalties must be <= 0 (AlignConfig, align_cpu.mojo:22-24): a positive
+    # mismatch, extension or opening step would let scores grow past the
+    # per-pair int32 bound the kernels check
+    assert sw(2, 0, 0, 0) == 0 and sw(2, -4, 2, -2) == 0
+    for sc in [(2, 1, -4, -2), (2, -4, -4, 1), (2, -4, 3, -2), (1, 1 << 20, -4, -2)]:
+        assert sw(*sc) == invalid, sc
+
+    def sw_start(match, mismatch, gap_open, gap_extend):
+        return lib.cgm_sw_banded_start_launch(None, None, None, None, None, None, 0, 0, 0, None,
+                                              4, match, mismatch, gap_open, gap_extend, None,
+                                              None, None, None, None, 0, None)
+    assert sw_start(2, -4, -4, -2) == 0
+    assert sw_start(2, -4, -4, 1) == invalid
+
+    def nw(match, mismatch, gap_open, gap_extend):
+        return lib.cgm_nw_banded_launch(None, None, None, None, None, None, 0, 0, 0, None, 4,
+                                        match, mismatch, gap_open, gap_extend, None, None, None,
+                                        0, None)
+    # banded NW bounds every value per pair instead, so any sign is accepted
+    assert nw(2, 1, 3, 1) == 0 and nw(2, -4, -lim - 1, -2) == invalid
 
 
 def test_sequence_batch_matches_oracle_packing(cgm):
@@ -215,3 +235,31 @@ def test_decode_ascii_inverts_oracle_packing(cgm, orc):
     assert [b.decode() for b in back] == seqs
     buf, starts, lens = cgm.concat_ascii(seqs)
     assert [buf[a:a + b].tobytes().decode() for a, b in zip(starts, lens)] == seqs
+
+
+def test_single_pair_signatures_raise_on_refused_pairs(cgm, monkeypatch):
+    """The per-pair signatures return a result or raise (the reference
+    returns a result); the batch forms report refused pairs as
+    CGM_BAD_PAIR.  Checked on the host by handing the wrappers a refused
+    batch result, so no GPU run of a refused case is needed."""
+    import cgm_amd.align as A
+    bad = cgm._lib.CGM_BAD_PAIR
+    one = lambda v: np.array([v], np.int32)  # noqa: E731
+    monkeypatch.setattr(A, "sw_banded_batch", lambda *a, **k: (one(bad), one(-1), one(-1)))
+    monkeypatch.setattr(A, "sw_banded_start_batch",
+                        lambda *a, **k: (one(bad), one(-1), one(-1), one(-1), one(-1)))
+    monkeypatch.setattr(A, "nw_banded_batch", lambda *a, **k: (one(bad), one(-1), one(-1)))
+    monkeypatch.setattr(A, "wfa_affine_batch", lambda *a, **k: one(bad))
+    b = cgm.SequenceBatch()
+    b.add_sequence("ACGT" * 600)
+    v = cgm.get_view(b, 0)
+    cfg = cgm.AlignConfig(2, -4, -4, -2, 2000)
+    for call in (lambda: cgm.smith_waterman_banded(v, v, cfg),
+                 lambda: cgm.smith_waterman_banded(v, v, cfg, with_start=True),
+                 lambda: cgm.needleman_wunsch_banded(v, v, cfg),
+                 lambda: cgm.wfa_affine_cpu(v, v, cgm.WFAConfig(4, 6, 2, 1 << 21))):
+        with pytest.raises(cgm._lib.CgmError):
+            call()
+    monkeypatch.setattr(A, "sw_banded_batch", lambda *a, **k: (one(7), one(3), one(4)))
+    r = cgm.smith_waterman_banded(v, v, cfg)
+    assert (r.score, r.query_start, r.ref_start, r.query_end, r.ref_end) == (7, 0, 0, 3, 4)
