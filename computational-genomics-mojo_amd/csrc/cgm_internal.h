@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <string>
 
 #include "../../include/cgm_align.h"
@@ -139,8 +140,10 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
                  MixTuning *best, float *best_ms, float *start_ms);
 // A plan owned by the library for the drop-in launchers, one per (device,
 // stream), recreated larger when a batch exceeds it (the old plan is freed
-// after the stream drains).
-int cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen, MixPlan **out);
+// after the stream drains).  `use` runs with the plan locked, so calls on
+// one stream from several threads take turns.
+int with_cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen,
+                     const std::function<int(MixPlan *)> &use);
 int64_t mix_plan_capacity(const MixPlan *P, int32_t *me_cap, WfaPen *pen);
 
 
@@ -176,6 +179,13 @@ __device__ __forceinline__ int effective_band(int band, int qlen, int rlen) {
 // descending key.
 int sort_desc_by_key(uint32_t *keys, int32_t *idx, int32_t *order_out, int64_t n,
                      hipStream_t stream);
+// Bytes of temporary storage the radix sort of n keys needs (no allocation).
+int sort_tmp_bytes(int64_t n, size_t *bytes);
+// The same sort on caller-provided storage (keys_out: n entries; tmp:
+// tmp_bytes >= sort_tmp_bytes(n)); allocates nothing.
+int sort_desc_by_key_ws(const uint32_t *keys, uint32_t *keys_out, const int32_t *idx,
+                        int32_t *order_out, int64_t n, void *tmp, size_t tmp_bytes,
+                        hipStream_t stream);
 
 // SW pair classes of the automatic layout.  0..3: register lane kernels for
 // effective bands [0,16], [17,32], [33,48], [49,64] (non-positive penalties,
@@ -190,9 +200,25 @@ constexpr int kPlainMaxBand = 1600;
 // in order[] -- all on the device.
 int sw_classify(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                 SwOut out, int32_t *order, int32_t *ranges, hipStream_t stream);
+// Device workspace of the classification (n = n_pairs entries each).
+struct SwClassWs {
+    int32_t *order = nullptr;      // n: pair indices, sorted
+    int32_t *ranges = nullptr;     // 2 * kSwClasses
+    uint32_t *keys = nullptr;      // n
+    uint32_t *keys_out = nullptr;  // n
+    int32_t *idx = nullptr;        // n
+    int32_t *counts = nullptr;     // 16 (zeroed by the classification itself)
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+};
+// sw_classify on caller-provided workspace; allocates nothing.
+int sw_classify_ws(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                   SwOut out, const SwClassWs &ws, hipStream_t stream);
+// fork: run the four band buckets side by side on forked streams (else one
+// after another on `stream`, creating nothing).
 int launch_sw_reg_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                           SwOut out, const int32_t *order, const int32_t *ranges, int grid,
-                          hipStream_t stream);
+                          hipStream_t stream, bool fork = true);
 int launch_sw_lds_lane(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                        SwOut out, const int32_t *order, const int32_t *range, int grid,
                        hipStream_t stream);
@@ -203,12 +229,21 @@ int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
                    SwOut out, hipStream_t stream);
 int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                       SwOut out, hipStream_t stream);
+// The launches of the band-bucket layout after classification (order and
+// ranges from sw_classify); fork: the register buckets side by side on
+// forked streams.
+int sw_bucket_kernels(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                      SwOut out, const int32_t *order, const int32_t *ranges, bool fork,
+                      hipStream_t stream);
 // Start coordinates of SW alignments whose score and ends are already in
 // score / q_end / r_end (cgm_sw_start.hip).
 int launch_sw_start(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                     const int32_t *score, const int32_t *q_end, const int32_t *r_end,
                     int32_t *q_start, int32_t *r_start, hipStream_t stream);
 int device_cu_count(int *cus);
+// Raises a kernel's dynamic LDS limit to `bytes` once per device
+// (thread-safe; every launch of that kernel uses the same limit).
+int set_kernel_lds(const void *kernel, int bytes);
 
 // Fork/join of auxiliary streams (default priority, non-blocking) so that
 // independent launches of one call overlap on the GPU; both sides are

@@ -26,6 +26,7 @@
 // The class limits and the kernel's shape are the plan's MixTuning.
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <mutex>
 
@@ -562,13 +563,9 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
         P->max_grid /= 2;
     CGM_ALLOC(P->slab, (int64_t)P->max_grid * P->slab_words * 4);
 #undef CGM_ALLOC
-    if (hipFuncSetAttribute((const void *)mix_persistent_kernel<true>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kMixMaxLds - 256) !=
-            hipSuccess ||
-        hipFuncSetAttribute((const void *)mix_persistent_kernel<false>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kMixMaxLds - 256) !=
-            hipSuccess)
-        return fail(set_error(CGM_ERR_HIP, "plan: LDS attribute"));
+    if (set_kernel_lds((const void *)mix_persistent_kernel<true>, kMixMaxLds - 256) ||
+        set_kernel_lds((const void *)mix_persistent_kernel<false>, kMixMaxLds - 256))
+        return fail(CGM_ERR_HIP);
     if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess)
         return fail(set_error(CGM_ERR_HIP, "plan: timing events"));
     *out = P;
@@ -599,38 +596,50 @@ struct PlanKey {
         return device != o.device ? device < o.device : stream < o.stream;
     }
 };
+// One library-owned plan per (device, stream); `mu` is held from the lookup
+// through the enqueue, so a concurrent call on the same stream that needs a
+// larger plan cannot destroy this one while it is in use.
+struct PlanSlot {
+    std::mutex mu;
+    MixPlan *plan = nullptr;
+};
 std::mutex g_plan_mu;
-std::map<PlanKey, MixPlan *> g_plans;
+std::map<PlanKey, PlanSlot> g_plans;
 }  // namespace
 
-int cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen, MixPlan **out) {
+int with_cached_plan(hipStream_t stream, int64_t n_pairs, int32_t me_cap, WfaPen pen,
+                     const std::function<int(MixPlan *)> &use) {
     int dev = 0;
     CGM_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_plan_mu);
-    MixPlan *&slot = g_plans[PlanKey{dev, stream}];
+    PlanSlot *ps = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        ps = &g_plans[PlanKey{dev, stream}];  // map nodes never move
+    }
+    std::lock_guard<std::mutex> lk(ps->mu);
+    MixPlan *&slot = ps->plan;
     if (slot) {
         int32_t cap_me = 0;
         WfaPen cap_pen{};
         const int64_t cap = mix_plan_capacity(slot, &cap_me, &cap_pen);
-        if (cap >= n_pairs && cap_me >= me_cap && cap_pen.x == pen.x && cap_pen.o == pen.o &&
-            cap_pen.e == pen.e) {
-            *out = slot;
-            return CGM_OK;
+        if (!(cap >= n_pairs && cap_me >= me_cap && cap_pen.x == pen.x && cap_pen.o == pen.o &&
+              cap_pen.e == pen.e)) {
+            CGM_HIP(hipStreamSynchronize(stream));
+            mix_plan_destroy(slot);
+            slot = nullptr;
         }
-        CGM_HIP(hipStreamSynchronize(stream));
-        mix_plan_destroy(slot);
-        slot = nullptr;
     }
-    // grow geometrically so a slowly growing batch size does not rebuild often
-    int64_t cap = 1024;
-    while (cap < n_pairs) cap *= 2;
-    int rc = mix_plan_create(cap, me_cap, pen, 0, 0, &slot);
-    if (rc) {
-        slot = nullptr;
-        return rc;
+    if (!slot) {
+        // grow geometrically so a slowly growing batch size does not rebuild often
+        int64_t cap = 1024;
+        while (cap < n_pairs) cap *= 2;
+        int rc = mix_plan_create(cap, me_cap, pen, 0, 0, &slot);
+        if (rc) {
+            slot = nullptr;
+            return rc;
+        }
     }
-    *out = slot;
-    return CGM_OK;
+    return use(slot);
 }
 
 // Enqueues the whole mixed pipeline on `stream` (no allocation, no sync).

@@ -8,6 +8,9 @@
 #include <string>
 #include <vector>
 
+#include <set>
+#include <utility>
+
 #include "cgm_internal.h"
 
 namespace cgm {
@@ -123,6 +126,21 @@ int device_max_band_eff(const Batch &b, const int32_t *band, int32_t band_scalar
     CGM_HIP(e2);
     CGM_HIP(e3);
     *result = h;
+    return CGM_OK;
+}
+
+namespace {
+std::mutex g_lds_attr_mu;
+std::set<std::pair<int, const void *>> g_lds_attr_done;
+}  // namespace
+
+int set_kernel_lds(const void *kernel, int bytes) {
+    int dev = 0;
+    CGM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_lds_attr_mu);
+    if (g_lds_attr_done.count({dev, kernel})) return CGM_OK;
+    CGM_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    g_lds_attr_done.insert({dev, kernel});
     return CGM_OK;
 }
 

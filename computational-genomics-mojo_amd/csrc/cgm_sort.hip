@@ -72,39 +72,70 @@ __global__ void zero_words_kernel(int32_t *p, int n) {
 
 }  // namespace
 
+int sort_tmp_bytes(int64_t n, size_t *bytes) {
+    *bytes = 0;
+    CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
+        nullptr, *bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (int32_t *)nullptr,
+        (int32_t *)nullptr, (int)n, 0, 32, (hipStream_t)0));
+    return CGM_OK;
+}
+
+int sort_desc_by_key_ws(const uint32_t *keys, uint32_t *keys_out, const int32_t *idx,
+                        int32_t *order_out, int64_t n, void *tmp, size_t tmp_bytes,
+                        hipStream_t stream) {
+    size_t need = 0;
+    if (int rc = sort_tmp_bytes(n, &need)) return rc;
+    if (need > tmp_bytes) return set_error(CGM_ERR_INVALID, "sort workspace too small");
+    CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, keys, keys_out, idx,
+                                                         order_out, (int)n, 0, 32, stream));
+    return CGM_OK;
+}
+
 int sort_desc_by_key(uint32_t *keys, int32_t *idx, int32_t *order_out, int64_t n,
                      hipStream_t stream) {
     uint32_t *keys_out = nullptr;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
-    CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, keys, keys, idx,
-                                                         order_out, (int)n, 0, 32, stream));
+    if (int rc = sort_tmp_bytes(n, &tmp_bytes)) return rc;
     CGM_HIP(hipMallocAsync((void **)&keys_out, (size_t)n * 4, stream));
     CGM_HIP(hipMallocAsync(&tmp, tmp_bytes > 0 ? tmp_bytes : 4, stream));
-    hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, keys, keys_out, idx,
-                                                                order_out, (int)n, 0, 32, stream);
+    int rc = sort_desc_by_key_ws(keys, keys_out, idx, order_out, n, tmp,
+                                 tmp_bytes > 0 ? tmp_bytes : 4, stream);
     CGM_HIP(hipFreeAsync(tmp, stream));
     CGM_HIP(hipFreeAsync(keys_out, stream));
-    CGM_HIP(e);
-    return CGM_OK;
+    return rc;
+}
+
+int sw_classify_ws(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                   SwOut out, const SwClassWs &ws, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, ws.counts, 16);
+    const unsigned blocks = (unsigned)((b.n_pairs + 255) / 256);
+    hipLaunchKernelGGL(sw_class_kernel, dim3(blocks), dim3(256), 0, stream, b, band, band_scalar,
+                       sc, out, ws.keys, ws.idx, ws.counts);
+    hipLaunchKernelGGL(sw_ranges_kernel, dim3(1), dim3(64), 0, stream, ws.counts, ws.ranges);
+    CGM_HIP(hipGetLastError());
+    return sort_desc_by_key_ws(ws.keys, ws.keys_out, ws.idx, ws.order, b.n_pairs, ws.sort_tmp,
+                               ws.sort_tmp_bytes, stream);
 }
 
 int sw_classify(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                 SwOut out, int32_t *order, int32_t *ranges, hipStream_t stream) {
-    uint32_t *keys = nullptr;
-    int32_t *idx = nullptr, *counts = nullptr;
-    CGM_HIP(hipMallocAsync((void **)&keys, (size_t)b.n_pairs * 4, stream));
-    CGM_HIP(hipMallocAsync((void **)&idx, (size_t)b.n_pairs * 4, stream));
-    CGM_HIP(hipMallocAsync((void **)&counts, 64, stream));
-    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, counts, 16);
-    const unsigned blocks = (unsigned)((b.n_pairs + 255) / 256);
-    hipLaunchKernelGGL(sw_class_kernel, dim3(blocks), dim3(256), 0, stream, b, band, band_scalar,
-                       sc, out, keys, idx, counts);
-    hipLaunchKernelGGL(sw_ranges_kernel, dim3(1), dim3(64), 0, stream, counts, ranges);
-    int rc = sort_desc_by_key(keys, idx, order, b.n_pairs, stream);
-    CGM_HIP(hipFreeAsync(keys, stream));
-    CGM_HIP(hipFreeAsync(idx, stream));
-    CGM_HIP(hipFreeAsync(counts, stream));
+    SwClassWs ws;
+    ws.order = order;
+    ws.ranges = ranges;
+    if (int rc = sort_tmp_bytes(b.n_pairs, &ws.sort_tmp_bytes)) return rc;
+    if (ws.sort_tmp_bytes == 0) ws.sort_tmp_bytes = 4;
+    CGM_HIP(hipMallocAsync((void **)&ws.keys, (size_t)b.n_pairs * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&ws.keys_out, (size_t)b.n_pairs * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&ws.idx, (size_t)b.n_pairs * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&ws.counts, 64, stream));
+    CGM_HIP(hipMallocAsync(&ws.sort_tmp, ws.sort_tmp_bytes, stream));
+    int rc = sw_classify_ws(b, band, band_scalar, sc, out, ws, stream);
+    CGM_HIP(hipFreeAsync(ws.keys, stream));
+    CGM_HIP(hipFreeAsync(ws.keys_out, stream));
+    CGM_HIP(hipFreeAsync(ws.idx, stream));
+    CGM_HIP(hipFreeAsync(ws.counts, stream));
+    CGM_HIP(hipFreeAsync(ws.sort_tmp, stream));
     return rc;
 }
 

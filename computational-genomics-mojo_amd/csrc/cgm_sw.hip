@@ -13,6 +13,8 @@
 // align_cpu.mojo:293-296), i.e. the key (h desc, qi asc, ri asc); with no
 // positive cell the result is (0, 0, 0).
 
+#include <mutex>
+
 #include "cgm_internal.h"
 
 namespace cgm {
@@ -282,12 +284,7 @@ int launch_sw_lds_lane(const Batch &b, const int32_t *band, int32_t band_scalar,
                        SwOut out, const int32_t *order, const int32_t *range, int grid,
                        hipStream_t stream) {
     const size_t lds_bytes = (size_t)(2 * kLdsLaneMaxBand + 2) * kLaneThreads * sizeof(int32_t);
-    static bool attr_set = false;
-    if (!attr_set) {
-        CGM_HIP(hipFuncSetAttribute((const void *)sw_lane_kernel,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-        attr_set = true;
-    }
+    if (int rc = set_kernel_lds((const void *)sw_lane_kernel, (int)lds_bytes)) return rc;
     hipLaunchKernelGGL(sw_lane_kernel, dim3(grid), dim3(kLaneThreads), lds_bytes, stream, b, band,
                        band_scalar, kLdsLaneMaxBand, sc, out, order, range);
     CGM_HIP(hipGetLastError());
@@ -301,12 +298,17 @@ int launch_sw_lds_lane(const Batch &b, const int32_t *band, int32_t band_scalar,
 // memory from the driver again.
 constexpr uint64_t kAsyncPoolKeepBytes = uint64_t(8) << 30;
 
+namespace {
+std::mutex g_cu_mu;
+int g_cu_cached[64] = {0};
+}  // namespace
+
 int device_cu_count(int *cus) {
-    static int cached[64] = {0};
     int dev = 0;
     CGM_HIP(hipGetDevice(&dev));
-    if (dev < 64 && cached[dev]) {
-        *cus = cached[dev];
+    std::lock_guard<std::mutex> lk(g_cu_mu);
+    if (dev < 64 && g_cu_cached[dev]) {
+        *cus = g_cu_cached[dev];
         return CGM_OK;
     }
     int n = 0;
@@ -317,34 +319,42 @@ int device_cu_count(int *cus) {
         uint64_t keep = kAsyncPoolKeepBytes;
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
-    if (dev < 64) cached[dev] = n;
+    if (dev < 64) g_cu_cached[dev] = n;
     *cus = n;
     return CGM_OK;
 }
 
 // Band-bucket layout: classify + sort on the device, then one launch per
-// class (register buckets on forked streams, LDS lanes, one workgroup per
-// pair).
-int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
-                      SwOut out, hipStream_t stream) {
-    if (int rc = check_batch(b)) return rc;
-    if (int rc = check_sw_scoring(sc)) return rc;
-    if (b.n_pairs == 0) return CGM_OK;
+// class (register buckets on forked streams when `fork`, LDS lanes, one
+// workgroup per pair).
+int sw_bucket_kernels(const Batch &b, const int32_t *band, int32_t band_scalar,
+                             SwScoring sc, SwOut out, const int32_t *order, const int32_t *ranges,
+                             bool fork, hipStream_t stream) {
     int cus = 0;
     if (int rc = device_cu_count(&cus)) return rc;
-    int32_t *order = nullptr, *ranges = nullptr;
-    CGM_HIP(hipMallocAsync((void **)&order, (size_t)b.n_pairs * 4, stream));
-    CGM_HIP(hipMallocAsync((void **)&ranges, 2 * kSwClasses * 4, stream));
-    int rc = sw_classify(b, band, band_scalar, sc, out, order, ranges, stream);
     const int64_t waves = (b.n_pairs + 63) / 64;
     const int reg_grid = (int)(waves < (int64_t)cus * 16 ? waves : (int64_t)cus * 16);
-    if (!rc) rc = launch_sw_reg_buckets(b, band, band_scalar, sc, out, order, ranges, reg_grid, stream);
+    int rc = launch_sw_reg_buckets(b, band, band_scalar, sc, out, order, ranges, reg_grid, stream,
+                                   fork);
     if (!rc)
         rc = launch_sw_lds_lane(b, band, band_scalar, sc, out, order, ranges + 2 * kSwClassLds,
                                 reg_grid < cus * 2 ? reg_grid : cus * 2, stream);
     if (!rc)
         rc = launch_sw_plain_range(b, band, band_scalar, sc, out, order,
                                    ranges + 2 * kSwClassPlain, cus * 2, stream);
+    return rc;
+}
+
+int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                      SwOut out, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    if (int rc = check_sw_scoring(sc)) return rc;
+    if (b.n_pairs == 0) return CGM_OK;
+    int32_t *order = nullptr, *ranges = nullptr;
+    CGM_HIP(hipMallocAsync((void **)&order, (size_t)b.n_pairs * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&ranges, 2 * kSwClasses * 4, stream));
+    int rc = sw_classify(b, band, band_scalar, sc, out, order, ranges, stream);
+    if (!rc) rc = sw_bucket_kernels(b, band, band_scalar, sc, out, order, ranges, true, stream);
     CGM_HIP(hipFreeAsync(order, stream));
     CGM_HIP(hipFreeAsync(ranges, stream));
     return rc;
@@ -358,10 +368,10 @@ int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
     if (int rc = check_batch(b)) return rc;
     if (int rc = check_sw_scoring(sc)) return rc;
     if (b.n_pairs == 0) return CGM_OK;
-    MixPlan *P = nullptr;
     const WfaPen pen = {4, 6, 2};  // unused: the batch holds no WFA pair
-    if (int rc = cached_plan(stream, b.n_pairs, 0, pen, &P)) return rc;
-    return mix_enqueue(P, b, nullptr, 0, band, band_scalar, nullptr, 0, sc, pen, out, stream);
+    return with_cached_plan(stream, b.n_pairs, 0, pen, [&](MixPlan *P) {
+        return mix_enqueue(P, b, nullptr, 0, band, band_scalar, nullptr, 0, sc, pen, out, stream);
+    });
 }
 
 }  // namespace cgm

@@ -27,11 +27,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
 // [49,64] (the classifier in cgm_sort.hip uses the same bounds).
 int launch_sw_reg_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                           SwOut out, const int32_t *order, const int32_t *ranges, int grid,
-                          hipStream_t stream) {
+                          hipStream_t stream, bool fork) {
     // The buckets are independent: run them side by side on forked streams
     // so that together they fill the GPU.
-    hipStream_t aux[4];
-    if (int rc = fork_streams(stream, 4, aux)) return rc;
+    hipStream_t aux[4] = {stream, stream, stream, stream};
+    if (fork)
+        if (int rc = fork_streams(stream, 4, aux)) return rc;
     // waves per SIMD each bucket fits without spills: 5 / 4 / 3 / 2
 #define CGM_REG(K, LO, HI, WPE)                                                              \
     hipLaunchKernelGGL((sw_reg_kernel<LO, HI, WPE>), dim3(grid), dim3(64), 0, aux[3 - (K)], b, \
@@ -42,7 +43,7 @@ int launch_sw_reg_buckets(const Batch &b, const int32_t *band, int32_t band_scal
     CGM_REG(0, 0, 16, 5)
 #undef CGM_REG
     CGM_HIP(hipGetLastError());
-    return join_streams(stream, 4, aux);
+    return fork ? join_streams(stream, 4, aux) : CGM_OK;
 }
 
 

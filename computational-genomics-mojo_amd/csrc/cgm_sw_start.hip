@@ -28,8 +28,6 @@
 // score < 2^(30-K) - 1, |mismatch|, |open + extend|, |extend| <= 2^(29-K));
 // one 128-thread workgroup per pair with int64 packing (K = 32) otherwise.
 
-#include <mutex>
-
 #include "cgm_internal.h"
 #include "cgm_sw_dev.h"
 
@@ -397,14 +395,7 @@ int launch_sw_start(const Batch &b, const int32_t *band, int32_t band_scalar, Sw
             rc = join_streams(stream, 4, aux);
         }
         const size_t lds = (size_t)5 * (2 * kPlainMaxBand + 1) * sizeof(int64_t);
-        static std::once_flag attr_once;
-        static hipError_t attr_err = hipSuccess;
-        std::call_once(attr_once, [&] {
-            attr_err = hipFuncSetAttribute((const void *)sw_start_plain_kernel,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        });
-        if (!rc && attr_err != hipSuccess)
-            rc = hip_error(attr_err, "hipFuncSetAttribute(sw_start_plain)");
+        if (!rc) rc = set_kernel_lds((const void *)sw_start_plain_kernel, (int)lds);
         if (!rc) {
             hipLaunchKernelGGL(sw_start_plain_kernel, dim3(cus * 2), dim3(kStartPlainThreads), lds,
                                stream, b, band, band_scalar, sc, io, order,

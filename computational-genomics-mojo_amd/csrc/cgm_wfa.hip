@@ -301,14 +301,10 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
     while (grid > 1 && grid * slab_words * 4 > (int64_t(4) << 30)) grid /= 2;
     int64_t grid_small = (int64_t)cus * kWfaSmallBlocksPerCu;
     if (grid_small > b.n_pairs) grid_small = b.n_pairs;
-    static bool attr_set = false;
-    if (!attr_set) {
-        CGM_HIP(hipFuncSetAttribute((const void *)wfa_wave_kernel<kTeamThreadsBig>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, kWfaLdsBytes));
-        CGM_HIP(hipFuncSetAttribute((const void *)wfa_wave_kernel<kTeamThreads>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, kWfaSmallLdsBytes));
-        attr_set = true;
-    }
+    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreadsBig>, kWfaLdsBytes))
+        return rc;
+    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreads>, kWfaSmallLdsBytes))
+        return rc;
     int32_t *slab = nullptr, *queue = nullptr, *own_order = nullptr;
     uint32_t *keys = nullptr;
     int32_t *idx = nullptr;
