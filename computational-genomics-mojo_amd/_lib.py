@@ -63,6 +63,8 @@ SIGNATURES = {
     "cgm_event_elapsed_ms": [_vp, _vp, _pf],
     "cgm_sw_banded_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                              _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp],
+    "cgm_sw_banded_launch_noalloc": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
+                                     _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
     "cgm_sw_banded_start_launch": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                                    _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp],
     "cgm_sw_banded_start_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,

@@ -113,6 +113,25 @@ int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                          int32_t *out_score, int32_t *out_q_end, int32_t *out_r_end,
                          int32_t layout, void *stream);
 
+/* -------------------------------------------- banded SW, allocation-free
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory during a call (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): cgm_sw_banded_launch with
+ * layout CGM_LAYOUT_LANE, same arguments and results, but every temporary
+ * (sort keys, pair order, class counts, radix-sort storage) lives in one
+ * static 12 MiB device array per GPU, held until the process ends, and no
+ * stream is created.  Limit: n_pairs <= 262144 per call (CGM_ERR_INVALID
+ * above).  Calls on one GPU share that array: calls that overlap in time
+ * on one GPU (two streams, or two host threads) race and must not be
+ * issued; consecutive calls on one stream are safe. */
+int cgm_sw_banded_launch_noalloc(const uint64_t *q_packed, const uint64_t *r_packed,
+                                 const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                                 const int32_t *r_off, int64_t n_pairs, int64_t q_words,
+                                 int64_t r_words, const int32_t *band, int32_t band_scalar,
+                                 int32_t match, int32_t mismatch, int32_t gap_open,
+                                 int32_t gap_extend, int32_t *out_score, int32_t *out_q_end,
+                                 int32_t *out_r_end, void *stream);
+
 /* ------------------------------------------- banded SW with start coordinates
  * cgm_sw_banded_launch plus the start of each alignment, which the
  * reference reports as query_start = ref_start = 0 (smith_waterman_banded,
