@@ -80,6 +80,7 @@ SIGNATURES = {
                          _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
     "cgm_plan_replay": [_vp, _vp],
     "cgm_plan_kernel_ms": [_vp, _pf],
+    "cgm_plan_wfa_kernel_ms": [_vp, _pf],
     "cgm_plan_get_tuning": [_vp, _vp],
     "cgm_plan_set_tuning": [_vp, _vp],
     "cgm_tuning_get_default": [_vp],

@@ -387,10 +387,18 @@ class MixedPlan:
         _lib.call("cgm_plan_replay", self.handle, device.stream)
 
     def kernel_ms(self) -> float:
-        """Duration of the persistent kernel of the last call or replay."""
+        """Duration of the SW persistent kernel of the last call or replay."""
         import ctypes
         ms = ctypes.c_float()
         _lib.call("cgm_plan_kernel_ms", self.handle, ctypes.byref(ms))
+        return float(ms.value)
+
+    def wfa_kernel_ms(self) -> float:
+        """Duration of the WFA kernels of the last call or replay (0 when it
+        had no WFA pair)."""
+        import ctypes
+        ms = ctypes.c_float()
+        _lib.call("cgm_plan_wfa_kernel_ms", self.handle, ctypes.byref(ms))
         return float(ms.value)
 
     def tuning(self) -> dict:

@@ -86,6 +86,16 @@ int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error
 // order: optional device permutation of pair indices (work-sorted); NULL = 0..n-1.
 int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
                     WfaPen p, int32_t *out, const int32_t *order, hipStream_t stream);
+// The two persistent WFA launches over device ranges of `order` (pairs
+// order[range[0]] .. order[range[1] - 1]): range_big on 512-thread
+// workgroups (slab: grid_big x slab_words int32 for pairs beyond LDS),
+// range_small on 256-thread ones; queues[0], queues[1]: zeroed counters.
+// Pairs with max_error above me_cap get CGM_BAD_PAIR.  Allocates nothing.
+int launch_wfa_wave_ranges(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                           int32_t me_cap, WfaPen p, int32_t *out, const int32_t *order,
+                           const int32_t *range_big, const int32_t *range_small, int32_t *queues,
+                           int32_t *slab, int64_t slab_words, int grid_big, int grid_small,
+                           hipStream_t stream);
 
 int wfa_geometry(const WfaPen &p, int *g, int *nM, int *nI);
 // One-workgroup-per-pair baselines restricted to one kind of a mixed batch
@@ -103,6 +113,8 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
 void mix_plan_destroy(MixPlan *P);
 int64_t mix_plan_bytes(const MixPlan *P);
 int mix_plan_kernel_ms(MixPlan *P, float *ms);
+// Duration of the last call's WFA kernels (0 when it had no WFA pair).
+int mix_plan_wfa_ms(MixPlan *P, float *ms);
 int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                 SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream);
@@ -117,8 +129,8 @@ int mix_replay(MixPlan *P, hipStream_t stream);
 struct MixTuning {
     int32_t blocks_per_cu = 2;         // four-wavefront workgroups per CU (1 or 2)
     int32_t lds_bytes = 64 * 1024;     // LDS per workgroup
-    int32_t tile_steps = 32;           // WFA scores per block of the tiled layout
-    int32_t wave_max_width = 1 << 30;  // WFA pairs up to this many diagonals: one per wavefront
+    int32_t tile_steps = 32;           // kept for the ABI; WFA pairs run the WFA launcher's layout
+    int32_t wave_max_width = 1 << 30;  // kept for the ABI; as tile_steps
     int32_t reg_max_band = 64;         // SW bands up to this: register lanes, wider: LDS lanes
     int32_t lane_max_len = 1 << 30;    // SW pairs longer than this: one workgroup per pair
     int32_t sw_dual = 1;               // register SW lanes: dual-row int16 where exact (1) or int32 (0)

@@ -4,25 +4,28 @@
 // performs no allocation and no host synchronisation and can be captured
 // into a hipGraph and replayed.
 //
-// Pipeline (all device work, one stream):
+// Pipeline (all device work; WFA pairs on a forked stream):
 //  1. classify each pair (kind, effective band, lengths, max_error) into
 //     0 SW register lanes (band <= reg_max_band <= 64), 1 SW packed register
-//     lanes (band 128), 2 WFA one workgroup per pair, 3 SW LDS lanes, 4 SW
+//     lanes (band 128), 2 WFA on 512-thread workgroups, 3 SW LDS lanes, 4 SW
 //     one workgroup per pair (also SW pairs longer than lane_max_len), 5 bad,
-//     6 WFA one wavefront per pair (at most wave_max_width diagonals and
-//     fitting a quarter of the workgroup's LDS); bad pairs get their outputs
-//     written here;
+//     6 WFA on 256-thread workgroups (int16 rows, branch-free LDS layout
+//     within 20 KB); bad pairs get their outputs written here, WFA pairs
+//     their ends (-1);
 //  2. radix-sort pair indices by (class, band, length) or (class, WFA work);
-//  3. build tasks: 256 consecutive SW pairs of one register class per task
-//     (one 64-lane group per wavefront of the workgroup), one class-2 WFA
-//     pair per task (the whole workgroup), four class-6 WFA pairs per task
-//     (one per wavefront), each with an estimated cost;
-//  4. radix-sort tasks by cost, largest first (longest-processing-time
-//     order, so the tail of the launch is made of small tasks);
-//  5. the persistent kernel: CUs x blocks_per_cu four-wavefront workgroups
-//     take task indices from an atomic counter until the queue is empty;
+//  3. the WFA classes run the WFA launcher's two persistent kernels over
+//     their ranges of the order, longest first, on a stream forked from the
+//     caller's (its kernels keep their own register and LDS budgets:
+//     6 and 8 wavefronts per SIMD);
+//  4. build SW tasks: 256 consecutive SW pairs of one register class per
+//     task, each with an estimated cost, and radix-sort them largest first;
+//  5. the SW persistent kernel: CUs x blocks_per_cu four-wavefront
+//     workgroups whose wavefronts take 64-pair units from a long and a short
+//     queue; it is launched after the WFA kernels so that its workgroups
+//     fill the CUs the WFA tail leaves idle;
 //  6. classes 3 and 4 (unusual scorings, bands or lengths) run the LDS-lane
-//     and one-workgroup-per-pair kernels over their ranges.
+//     and one-workgroup-per-pair kernels over their ranges;
+//  7. the caller's stream joins the forked one.
 // The class limits and the kernel's shape are the plan's MixTuning.
 
 #include <algorithm>
@@ -41,13 +44,13 @@ namespace {
 
 constexpr int kMixClasses = 7;
 constexpr int kMixSwReg = 0, kMixSwPacked = 1, kMixWfa = 2, kMixSwLds = 3, kMixSwPlain = 4,
-              kMixBad = 5, kMixWfaWave = 6;
+              kMixBad = 5, kMixWfaSmall = 6;
 constexpr int kMixWaves = 4;              // wavefronts per workgroup
 constexpr int kMixMaxBlocksPerCu = 2;     // 256 VGPRs: two workgroups per CU at most
 constexpr int kMixMaxLds = 160 * 1024;    // LDS of a CU
 constexpr int kPackedBand = 128;
 // One workgroup = four wavefronts: an SW task holds up to 4 x 64 pairs (one
-// lane-group per wavefront), a WFA task is one pair for the whole workgroup.
+// lane-group per wavefront).
 constexpr int kMixThreads = 256;
 constexpr int kMixGroup = kMixThreads;
 
@@ -59,40 +62,21 @@ constexpr int kRegBands = 65;                // effective bands 0..64
 constexpr int kSmallCounts = 0;              // pairs per class [kMixClasses]
 constexpr int kSmallRanges = 8;              // [start, end) per class in order[]
 constexpr int kSmallNTasks = 24;             // tasks in the queue
-constexpr int kSmallQueue = 25;              // next task (atomic counter)
 constexpr int kSmallRegTasks = 26;           // register-lane SW tasks
 constexpr int kSmallBandCount = 32;          // register-lane pairs per band [kRegBands]
 constexpr int kSmallBandTask = 32 + 72;      // first task of each band, widest first [kRegBands+1]
 constexpr int kSmallBandPair = 32 + 2 * 72;  // first pair (in the class) of each band [kRegBands]
-constexpr int kSmallUnits = 28;              // SW-only kernel: two-ended unit counter (64 bit)
+constexpr int kSmallUnits = 28;              // SW kernel: long/short unit counters (64 bit)
+constexpr int kSmallWfaQueues = 30;          // counters of the two WFA launches
 constexpr int kSmallSimdSlots = 256;         // wavefronts started per SIMD [kSimdKeys]
 constexpr int kSimdKeys = 8 << 10;           // XCC (3 bits), CU bits of HW_ID (8), SIMD (2)
 constexpr int kSmallWords = kSmallSimdSlots + kSimdKeys;
 
-// Diagonals per thread and pass in WFA tasks: 4 would spill at the 256 VGPRs
-// the register SW layout leaves.
-constexpr int kMixWfaUnroll = 2;
-
-enum TaskType : int32_t { kTaskSwReg = 0, kTaskSwPacked = 1, kTaskWfa = 2, kTaskWfaWave = 3 };
-
-// LDS part of one wavefront in class-6 tasks.
-__host__ __device__ __forceinline__ int wave_part_bytes(int lds_bytes) {
-    return (lds_bytes / kMixWaves) & ~15;
-}
+enum TaskType : int32_t { kTaskSwReg = 0, kTaskSwPacked = 1 };
 
 struct Task {
     int32_t type, start, count, pad;
 };
-
-__device__ __forceinline__ uint32_t wfa_work_estimate(int me, int qlen, int rlen,
-                                                      const WfaPen &p, int g) {
-    // sum over scores of ceil(W(s) / 64) wave steps, ~25 units each, plus
-    // the extension of the longer sequence
-    const int64_t R = me >= p.o + p.e ? (me - p.o) / p.e : 0;
-    const int64_t steps = (int64_t)(me / g) * (R / 32 + 1);
-    const int64_t est = 25 * steps + (max(qlen, rlen) >> 3) + 1;
-    return (uint32_t)min(est, (int64_t)((1 << 28) - 1));
-}
 
 // Classification, one pair per thread in a grid-stride loop over a grid of
 // at most one workgroup per CU: class counts and register-band counts are
@@ -149,13 +133,8 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
             } else if (k == 1) {
                 const int me = me_arr ? me_arr[i] : me_scalar;
                 if (me >= 0 && me <= me_cap) {
-                    const int K = min(min(me, max(ql, rl)), wfadev::reach(me, p));
-                    const bool wave = 2 * K + 1 <= tune.wave_max_width &&
-                                      wfadev::rows_fit_int16(ql, me, p) &&
-                                      wfadev::wave_pair_bytes(ql, rl, me, p, gm) <=
-                                          wave_part_bytes(tune.lds_bytes);
-                    cls = wave ? kMixWfaWave : kMixWfa;
-                    payload = wfa_work_estimate(me, ql, rl, p, gm.g);
+                    cls = wfadev::wfa_fits_small(ql, rl, me, p, gm) ? kMixWfaSmall : kMixWfa;
+                    payload = wfadev::wfa_work_key(me, p, gm);
                 }
             }
         }
@@ -164,8 +143,8 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
             if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&cls_count[c], __popcll(m));
         }
         if (in) {
-            if (cls == kMixBad) {
-                out.score[i] = CGM_BAD_PAIR;
+            if (cls == kMixBad || cls == kMixWfa || cls == kMixWfaSmall) {
+                if (cls == kMixBad) out.score[i] = CGM_BAD_PAIR;
                 if (out.q_end) out.q_end[i] = -1;
                 if (out.r_end) out.r_end[i] = -1;
             }
@@ -203,8 +182,7 @@ __global__ void mix_ranges_kernel(int32_t *small) {
     }
     small[kSmallBandTask + kRegBands] = tasks;
     small[kSmallRegTasks] = tasks;
-    small[kSmallNTasks] = tasks + (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup +
-                          counts[kMixWfa] + (counts[kMixWfaWave] + kMixWaves - 1) / kMixWaves;
+    small[kSmallNTasks] = tasks + (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup;
 }
 
 __global__ void mix_tasks_kernel(const int32_t *small, const uint32_t *sorted_keys, int64_t cap,
@@ -213,195 +191,99 @@ __global__ void mix_tasks_kernel(const int32_t *small, const uint32_t *sorted_ke
     if (t >= cap) return;
     const int32_t *ranges = small + kSmallRanges;
     const int32_t c1 = ranges[2 * kMixSwPacked + 1] - ranges[2 * kMixSwPacked];
-    const int64_t n0 = small[kSmallRegTasks], n1 = (c1 + kMixGroup - 1) / kMixGroup;
+    const int64_t n0 = small[kSmallRegTasks];
     Task task = {0, 0, 0, 0};
     uint32_t key = 0;
     if (t < small[kSmallNTasks]) {
-        uint32_t payload;
-        if (t < n0 + n1) {
-            if (t < n0) {
-                // band of task t: the last band (widest first) whose first task is <= t
-                int lo = 0, hi = kRegBands - 1;  // band task starts decrease with the band
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (small[kSmallBandTask + mid] <= t) hi = mid;
-                    else lo = mid + 1;
-                }
-                const int bd = lo;
-                const int64_t g = t - small[kSmallBandTask + bd];
-                task.type = kTaskSwReg;
-                task.start = ranges[2 * kMixSwReg] + small[kSmallBandPair + bd] +
-                             (int32_t)(kMixGroup * g);
-                task.count = (int32_t)min((int64_t)kMixGroup,
-                                          small[kSmallBandCount + bd] - kMixGroup * g);
-            } else {
-                const int64_t g = t - n0;
-                task.type = kTaskSwPacked;
-                task.start = ranges[2 * kMixSwPacked] + (int32_t)(kMixGroup * g);
-                task.count = (int32_t)min((int64_t)kMixGroup, c1 - kMixGroup * g);
+        if (t < n0) {
+            // band of task t: the last band (widest first) whose first task is <= t
+            int lo = 0, hi = kRegBands - 1;  // band task starts decrease with the band
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (small[kSmallBandTask + mid] <= t) hi = mid;
+                else lo = mid + 1;
             }
-            payload = sorted_keys[task.start] & 0x0fffffffu;
-            const uint32_t band = payload >> 17, len = payload & 0x1ffffu;
-            // ~10 VALU ops x 4 cycles per wave-step of 64 cells, units of 16 cycles
-            key = (uint32_t)min((uint64_t)len * (2 * band + 1) * 3 + 1, (uint64_t)0xffffffffu);
-        } else if (t < n0 + n1 + (ranges[2 * kMixWfa + 1] - ranges[2 * kMixWfa])) {
-            task.type = kTaskWfa;
-            task.start = ranges[2 * kMixWfa] + (int32_t)(t - n0 - n1);
-            task.count = 1;
-            key = (sorted_keys[task.start] & 0x0fffffffu) + 1;
+            const int bd = lo;
+            const int64_t g = t - small[kSmallBandTask + bd];
+            task.type = kTaskSwReg;
+            task.start = ranges[2 * kMixSwReg] + small[kSmallBandPair + bd] +
+                         (int32_t)(kMixGroup * g);
+            task.count = (int32_t)min((int64_t)kMixGroup,
+                                      small[kSmallBandCount + bd] - kMixGroup * g);
         } else {
-            // four class-6 pairs, one per wavefront; the first is the largest
-            const int64_t g = t - n0 - n1 - (ranges[2 * kMixWfa + 1] - ranges[2 * kMixWfa]);
-            const int32_t cnt = ranges[2 * kMixWfaWave + 1] - ranges[2 * kMixWfaWave];
-            task.type = kTaskWfaWave;
-            task.start = ranges[2 * kMixWfaWave] + (int32_t)(kMixWaves * g);
-            task.count = (int32_t)min((int64_t)kMixWaves, cnt - kMixWaves * g);
-            key = (sorted_keys[task.start] & 0x0fffffffu) + 1;
+            const int64_t g = t - n0;
+            task.type = kTaskSwPacked;
+            task.start = ranges[2 * kMixSwPacked] + (int32_t)(kMixGroup * g);
+            task.count = (int32_t)min((int64_t)kMixGroup, c1 - kMixGroup * g);
         }
+        const uint32_t payload = sorted_keys[task.start] & 0x0fffffffu;
+        const uint32_t band = payload >> 17, len = payload & 0x1ffffu;
+        // ~10 VALU ops x 4 cycles per wave-step of 64 cells, units of 16 cycles
+        key = (uint32_t)min((uint64_t)len * (2 * band + 1) * 3 + 1, (uint64_t)0xffffffffu);
     }
     tasks[t] = task;
     tkeys[t] = key;
     tidx[t] = (int32_t)t;
 }
 
-// The WFA paths run once per task, so they are called out of line: inlined
-// beside the register SW layout they would push the kernel past 256 VGPRs.
-__device__ __noinline__ int mix_wfa_block(const uint64_t *q, int qlen, const uint64_t *r, int rlen,
-                                          int me, WfaPen p, wfadev::WfaGeom gm, uint8_t *lds,
-                                          int lds_bytes, int32_t *slab, int64_t slab_words,
-                                          int tile_steps) {
-    return wfadev::wfa_pair_block<kMixWfaUnroll, kMixThreads>(q, qlen, r, rlen, me, p, gm, lds, lds_bytes,
-                                                 slab, slab_words, tile_steps);
-}
-
-__device__ __noinline__ int mix_wfa_wave(const uint64_t *q, int qlen, const uint64_t *r, int rlen,
-                                         int me, WfaPen p, wfadev::WfaGeom gm, uint8_t *part) {
-    return wfadev::wfa_pair_wave<kMixWfaUnroll>(q, qlen, r, rlen, me, p, gm, part);
-}
-
 __global__ void mix_zero_kernel(int32_t *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-// The persistent kernel: four-wavefront workgroups take tasks in order.
-// WITH_WFA = false drops the WFA task code for batches known to hold SW
-// pairs only, so the register SW layout keeps its registers to itself.
-template <bool WITH_WFA>
+// The SW persistent kernel.  Every wavefront takes 64-pair units of the
+// longest-first task list by itself (unit u = part u % 4 of task u / 4), so
+// no wavefront waits for the others of its workgroup.  The list is cut in
+// two queues, both taken longest first: the first sw_split percent of the
+// units (the long queue) and the rest (the short queue).  Of two wavefronts
+// on one SIMD the older one gets most issue slots, so the first wavefront
+// started on a SIMD takes the long queue and, once it is empty, the short
+// one; the second takes only the short queue.  The longest units never go
+// to a starved wavefront, and the last units of the launch are the shortest
+// (one 64-bit counter: long-queue claims in the low half, short-queue
+// claims in the high half).
 __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void
-mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, const int32_t *me_arr,
-                      int32_t me_scalar, SwScoring sc, WfaPen p, wfadev::WfaGeom gm, SwOut out,
-                      const int32_t *order, const Task *tasks, const int32_t *task_order,
-                      const int32_t *n_tasks, int32_t *queue, int32_t lds_bytes,
-                      int32_t tile_steps, int32_t sw_dual, int32_t sw_split, int32_t *slab,
-                      int64_t slab_words) {
-    extern __shared__ uint8_t lds_raw[];
-    __shared__ int next_task;
-    int32_t *my_slab = slab + (int64_t)blockIdx.x * slab_words;
+mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, SwScoring sc,
+                      SwOut out, const int32_t *order, const Task *tasks,
+                      const int32_t *task_order, const int32_t *n_tasks, int32_t *small,
+                      int32_t sw_dual, int32_t sw_split) {
     const int total = *n_tasks;
-    const int wave = threadIdx.x >> 6;
-    if constexpr (!WITH_WFA) {
-        // SW only: every wavefront takes 64-pair units of the longest-first
-        // task list by itself (unit u = part u % 4 of task u / 4), so no
-        // wavefront waits for the others of its workgroup.  The list is cut
-        // in two queues, both taken longest first: the first sw_split
-        // percent of the units (the long queue) and the rest (the short
-        // queue).  Of two wavefronts on one SIMD the older one gets most
-        // issue slots, so the first wavefront started on a SIMD takes the
-        // long queue and, once it is empty, the short one; the second takes
-        // only the short queue.  The longest units never go to a starved
-        // wavefront, and the last units of the launch are the shortest (one
-        // 64-bit counter: long-queue claims in the low half, short-queue
-        // claims in the high half).
-        const int lane = threadIdx.x & 63;
-        const uint32_t units = (uint32_t)total * kMixWaves;
-        int32_t *small = queue - kSmallQueue;
-        unsigned long long *ends = (unsigned long long *)(small + kSmallUnits);
-        int slot = 0;
-        if (lane == 0) {
-            const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_ID
-            const uint32_t xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20);  // XCC_ID
-            const uint32_t key = ((xcc & 7u) << 10) | (((hw >> 8) & 0xffu) << 2) | ((hw >> 4) & 3u);
-            slot = atomicAdd(small + kSmallSimdSlots + key, 1);
-        }
-        const bool second = (__shfl(slot, 0, 64) & 1) != 0;
-        const uint32_t split = (uint32_t)(((uint64_t)units * (uint32_t)sw_split) / 100u);
-        for (;;) {
-            int u = -1;
-            if (lane == 0) {
-                if (!second) {
-                    const uint32_t a = (uint32_t)atomicAdd(ends, 1ull);
-                    if (a < split) u = (int)a;
-                }
-                if (u < 0) {
-                    const uint32_t c = (uint32_t)(atomicAdd(ends, 1ull << 32) >> 32);
-                    if ((uint64_t)split + c < units) u = (int)(split + c);
-                }
-            }
-            u = __shfl(u, 0, 64);
-            if (u < 0) break;
-            const Task task = tasks[task_order[u / kMixWaves]];
-            const int part = u % kMixWaves;
-            const int64_t start = task.start + 64 * part;
-            const int count = min(64, task.count - 64 * part);
-            if (count <= 0) continue;
-            if (task.type == kTaskSwReg)
-                swdev::sw_reg_group<0, 64, false, true>(b, band_arr, band_scalar, sc, out, order,
-                                                        start, count, sw_dual != 0);
-            else
-                swdev::sw_reg_group<kPackedBand, kPackedBand, true>(b, band_arr, band_scalar, sc,
-                                                                    out, order, start, count);
-        }
-        return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t units = (uint32_t)total * kMixWaves;
+    unsigned long long *ends = (unsigned long long *)(small + kSmallUnits);
+    int slot = 0;
+    if (lane == 0) {
+        const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_ID
+        const uint32_t xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20);  // XCC_ID
+        const uint32_t key = ((xcc & 7u) << 10) | (((hw >> 8) & 0xffu) << 2) | ((hw >> 4) & 3u);
+        slot = atomicAdd(small + kSmallSimdSlots + key, 1);
     }
+    const bool second = (__shfl(slot, 0, 64) & 1) != 0;
+    const uint32_t split = (uint32_t)(((uint64_t)units * (uint32_t)sw_split) / 100u);
     for (;;) {
-        if (threadIdx.x == 0) next_task = atomicAdd(queue, 1);
-        __syncthreads();
-        const int t = next_task;
-        __syncthreads();
-        if (t >= total) break;
-        const Task task = tasks[task_order[t]];
-        if (task.type == kTaskSwReg || task.type == kTaskSwPacked) {
-            const int64_t start = task.start + 64 * wave;
-            const int count = min(64, task.count - 64 * wave);
-            if (count > 0) {
-                if (task.type == kTaskSwReg)
-                    swdev::sw_reg_group<0, 64, false, true>(b, band_arr, band_scalar, sc, out,
-                                                            order, start, count, sw_dual != 0);
-                else
-                    swdev::sw_reg_group<kPackedBand, kPackedBand, true>(
-                        b, band_arr, band_scalar, sc, out, order, start, count);
+        int u = -1;
+        if (lane == 0) {
+            if (!second) {
+                const uint32_t a = (uint32_t)atomicAdd(ends, 1ull);
+                if (a < split) u = (int)a;
             }
-        } else if (!WITH_WFA) {
-            // unreachable: WFA tasks only exist when the batch has WFA pairs
-        } else if (task.type == kTaskWfa) {
-            const int64_t pair = order[task.start];
-            const uint64_t *q, *r;
-            int qlen, rlen;
-            load_pair(b, pair, q, r, qlen, rlen);  // validated by the classifier
-            const int me = me_arr ? me_arr[pair] : me_scalar;
-            const int res = mix_wfa_block(q, qlen, r, rlen, me, p, gm, lds_raw, lds_bytes, my_slab,
-                                          slab_words, tile_steps);
-            if (threadIdx.x == 0) {
-                out.score[pair] = res;
-                if (out.q_end) out.q_end[pair] = -1;
-                if (out.r_end) out.r_end[pair] = -1;
-            }
-        } else if (wave < task.count) {
-            // class 6: each wavefront aligns its own pair in its LDS part
-            const int64_t pair = order[task.start + wave];
-            const uint64_t *q, *r;
-            int qlen, rlen;
-            load_pair(b, pair, q, r, qlen, rlen);  // validated by the classifier
-            const int me = me_arr ? me_arr[pair] : me_scalar;
-            const int res =
-                mix_wfa_wave(q, qlen, r, rlen, me, p, gm, lds_raw + wave * wave_part_bytes(lds_bytes));
-            if ((threadIdx.x & 63) == 0) {
-                out.score[pair] = res;
-                if (out.q_end) out.q_end[pair] = -1;
-                if (out.r_end) out.r_end[pair] = -1;
+            if (u < 0) {
+                const uint32_t c = (uint32_t)(atomicAdd(ends, 1ull << 32) >> 32);
+                if ((uint64_t)split + c < units) u = (int)(split + c);
             }
         }
-        __syncthreads();
+        u = __shfl(u, 0, 64);
+        if (u < 0) break;
+        const Task task = tasks[task_order[u / kMixWaves]];
+        const int part = u % kMixWaves;
+        const int64_t start = task.start + 64 * part;
+        const int count = min(64, task.count - 64 * part);
+        if (count <= 0) continue;
+        if (task.type == kTaskSwReg)
+            swdev::sw_reg_group<0, 64, false, true>(b, band_arr, band_scalar, sc, out, order,
+                                                    start, count, sw_dual != 0);
+        else
+            swdev::sw_reg_group<kPackedBand, kPackedBand, true>(b, band_arr, band_scalar, sc, out,
+                                                                order, start, count);
     }
 }
 
@@ -415,7 +297,7 @@ struct MixPlan {
     WfaPen pen{};
     wfadev::WfaGeom gm{};
     int cus = 0;
-    int max_grid = 0;  // workgroups that have a slab (the launch grid's cap)
+    int wfa_grid = 0;  // 512-thread WFA workgroups, each with a slab
     MixTuning tune{};
     // workspace
     uint32_t *keys = nullptr, *keys_out = nullptr, *tkeys = nullptr, *tkeys_out = nullptr;
@@ -429,10 +311,12 @@ struct MixPlan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t bytes = 0;  // device memory held by the plan
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // around the persistent kernel
+    bool had_wfa = false;  // the last enqueue launched the WFA kernels
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // around the SW persistent kernel
+    hipEvent_t ev_wfa_start = nullptr, ev_wfa_stop = nullptr;  // around the WFA kernels
     ~MixPlan() {
-        if (ev_start) (void)hipEventDestroy(ev_start);
-        if (ev_stop) (void)hipEventDestroy(ev_stop);
+        for (hipEvent_t e : {ev_start, ev_stop, ev_wfa_start, ev_wfa_stop})
+            if (e) (void)hipEventDestroy(e);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         void *ptrs[] = {keys, keys_out, tkeys, tkeys_out, idx, order, tidx, task_order, small,
@@ -554,19 +438,18 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
         return fail(set_error(CGM_ERR_HIP, "plan: sort size query"));
     P->sort_tmp_bytes = b1 > b2 ? b1 : b2;
     CGM_ALLOC(P->sort_tmp, P->sort_tmp_bytes + 256);
-    // HBM wavefront slab per workgroup (for the largest grid any tuning
-    // launches, within 4 GiB of slabs) for WFA pairs too large for LDS.
+    // HBM wavefront slab per 512-thread WFA workgroup (within 4 GiB of
+    // slabs) for WFA pairs too large for LDS.
     const int K = me_cap < reach_host(me_cap, pen) ? me_cap : reach_host(me_cap, pen);
     P->slab_words = (int64_t)(P->gm.nM + 2 * P->gm.nI) * (2 * (int64_t)K + 1);
-    P->max_grid = P->cus * kMixMaxBlocksPerCu;
-    while (P->max_grid > 1 && (int64_t)P->max_grid * P->slab_words * 4 > (int64_t(4) << 30))
-        P->max_grid /= 2;
-    CGM_ALLOC(P->slab, (int64_t)P->max_grid * P->slab_words * 4);
+    P->wfa_grid = P->cus * kWfaBlocksPerCu;
+    while (P->wfa_grid > 1 && (int64_t)P->wfa_grid * P->slab_words * 4 > (int64_t(4) << 30))
+        P->wfa_grid /= 2;
+    CGM_ALLOC(P->slab, (int64_t)P->wfa_grid * P->slab_words * 4);
 #undef CGM_ALLOC
-    if (set_kernel_lds((const void *)mix_persistent_kernel<true>, kMixMaxLds - 256) ||
-        set_kernel_lds((const void *)mix_persistent_kernel<false>, kMixMaxLds - 256))
-        return fail(CGM_ERR_HIP);
-    if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess)
+    if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess ||
+        hipEventCreate(&P->ev_wfa_start) != hipSuccess ||
+        hipEventCreate(&P->ev_wfa_stop) != hipSuccess)
         return fail(set_error(CGM_ERR_HIP, "plan: timing events"));
     *out = P;
     return CGM_OK;
@@ -579,6 +462,16 @@ int64_t mix_plan_bytes(const MixPlan *P) { return P->bytes; }
 int mix_plan_kernel_ms(MixPlan *P, float *ms) {
     CGM_HIP(hipEventSynchronize(P->ev_stop));
     CGM_HIP(hipEventElapsedTime(ms, P->ev_start, P->ev_stop));
+    return CGM_OK;
+}
+
+int mix_plan_wfa_ms(MixPlan *P, float *ms) {
+    if (!P->had_wfa) {
+        *ms = 0.0f;
+        return CGM_OK;
+    }
+    CGM_HIP(hipEventSynchronize(P->ev_wfa_stop));
+    CGM_HIP(hipEventElapsedTime(ms, P->ev_wfa_start, P->ev_wfa_stop));
     return CGM_OK;
 }
 
@@ -656,7 +549,7 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     CGM_HIP(hipGetDevice(&dev));
     if (dev != P->device) return set_error(CGM_ERR_INVALID, "plan belongs to another device");
     int32_t *counts = P->small + kSmallCounts, *ranges = P->small + kSmallRanges,
-            *n_tasks = P->small + kSmallNTasks, *queue = P->small + kSmallQueue;
+            *n_tasks = P->small + kSmallNTasks;
     const int64_t n = b.n_pairs, nt = P->task_cap;
     hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(256), 0, stream, P->small, kSmallWords);
     const MixTuning tune = P->tune;
@@ -668,34 +561,46 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     size_t tb = P->sort_tmp_bytes;
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->keys, P->keys_out,
                                                          P->idx, P->order, (int)n, 0, 32, stream));
-    // SW-only batches hold at most ceil(n / kMixGroup) + kRegBands + 1 tasks
-    // (band-aligned register tasks, packed tasks); mixed ones at most n + ...
-    const bool sw_only = kind == nullptr && kind_scalar == 0;
-    const int64_t tasks_needed = sw_only ? (n + kMixGroup - 1) / kMixGroup + kRegBands + 1
-                                         : n + (n + kMixGroup - 1) / kMixGroup + 1;
+    // WFA pairs: the WFA launcher's kernels on a forked stream, launched
+    // first so that the SW persistent kernel fills the CUs their tail frees
+    const bool has_wfa = kind != nullptr || kind_scalar == 1;
+    hipStream_t aux = stream;
+    if (has_wfa) {
+        if (int rc = fork_streams(stream, 1, &aux)) return rc;
+        const int grid_small = P->cus * kWfaSmallBlocksPerCu;
+        CGM_HIP(hipEventRecord(P->ev_wfa_start, aux));
+        if (int rc = launch_wfa_wave_ranges(b, me, me_scalar, P->me_cap, pen, out.score, P->order,
+                                            ranges + 2 * kMixWfa, ranges + 2 * kMixWfaSmall,
+                                            P->small + kSmallWfaQueues, P->slab, P->slab_words,
+                                            P->wfa_grid, grid_small, aux))
+            return rc;
+        CGM_HIP(hipEventRecord(P->ev_wfa_stop, aux));
+    }
+    P->had_wfa = has_wfa;
+    // SW tasks: at most ceil(n / kMixGroup) + kRegBands + 1 (band-aligned
+    // register tasks, packed tasks)
+    const int64_t tasks_needed = (n + kMixGroup - 1) / kMixGroup + kRegBands + 1;
     const int64_t tn = tasks_needed < nt ? tasks_needed : nt;
     hipLaunchKernelGGL(mix_tasks_kernel, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, stream,
                        P->small, P->keys_out, tn, P->tasks, P->tkeys, P->tidx);
-    tb = P->sort_tmp_bytes;
-    CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->tkeys, P->tkeys_out,
+    size_t tb2 = P->sort_tmp_bytes;
+    CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb2, P->tkeys, P->tkeys_out,
                                                          P->tidx, P->task_order, (int)tn, 0, 32,
                                                          stream));
     CGM_HIP(hipEventRecord(P->ev_start, stream));
-    const int grid = std::min(P->cus * tune.blocks_per_cu, P->max_grid);
-    hipLaunchKernelGGL(sw_only ? mix_persistent_kernel<false> : mix_persistent_kernel<true>,
-                       dim3(grid), dim3(kMixThreads),
-                       (size_t)tune.lds_bytes, stream, b, band, band_scalar, me, me_scalar, sc, pen,
-                       P->gm, out, P->order, P->tasks, P->task_order, n_tasks, queue,
-                       tune.lds_bytes, tune.tile_steps, tune.sw_dual, tune.sw_split, P->slab,
-                       P->slab_words);
+    hipLaunchKernelGGL(mix_persistent_kernel, dim3(P->cus * tune.blocks_per_cu), dim3(kMixThreads),
+                       0, stream, b, band, band_scalar, sc, out, P->order, P->tasks,
+                       P->task_order, n_tasks, P->small, tune.sw_dual, tune.sw_split);
     CGM_HIP(hipGetLastError());
     CGM_HIP(hipEventRecord(P->ev_stop, stream));
     // uncommon SW classes: LDS lanes, then one workgroup per pair
     if (int rc = launch_sw_lds_lane(b, band, band_scalar, sc, out, P->order, ranges + 2 * kMixSwLds,
                                     P->cus * 2, stream))
         return rc;
-    return launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
-                                 ranges + 2 * kMixSwPlain, P->cus * 2, stream);
+    if (int rc = launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
+                                       ranges + 2 * kMixSwPlain, P->cus * 2, stream))
+        return rc;
+    return has_wfa ? join_streams(stream, 1, &aux) : CGM_OK;
 }
 
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
@@ -784,35 +689,10 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
             }
         }
     };
-    {   // workgroup shape: workgroups per CU and LDS per workgroup
-        const int shapes[][2] = {{2, 48 * 1024}, {2, 64 * 1024}, {2, 79 * 1024},
-                                 {1, 96 * 1024}, {1, 128 * 1024}, {1, 159 * 1024}};
-        MixTuning c[6];
-        int nc = 0;
-        for (auto &sh : shapes) {
-            if (sh[0] == cur.blocks_per_cu && sh[1] == cur.lds_bytes) continue;
-            c[nc] = cur;
-            c[nc].blocks_per_cu = sh[0];
-            c[nc].lds_bytes = sh[1];
-            ++nc;
-        }
-        try_all(c, nc);
-    }
-    {   // WFA tile height (scores per block of the tiled layout)
-        const int v[] = {16, 32, 64};
-        MixTuning c[3];
-        int nc = 0;
-        for (int x : v)
-            if (x != cur.tile_steps) (c[nc] = cur, c[nc++].tile_steps = x);
-        try_all(c, nc);
-    }
-    {   // switch-over between one wavefront and one workgroup per WFA pair
-        const int v[] = {0, 256, 512, 1024, 2048, 1 << 30};
-        MixTuning c[6];
-        int nc = 0;
-        for (int x : v)
-            if (x != cur.wave_max_width) (c[nc] = cur, c[nc++].wave_max_width = x);
-        try_all(c, nc);
+    {   // four-wavefront SW workgroups per CU (two waves per SIMD at most)
+        MixTuning c = cur;
+        c.blocks_per_cu = cur.blocks_per_cu == 1 ? 2 : 1;
+        try_all(&c, 1);
     }
     {   // band split between register lanes and LDS lanes
         const int v[] = {32, 48, 64};

@@ -515,6 +515,11 @@ int cgm_plan_kernel_ms(void *plan, float *ms) {
     return mix_plan_kernel_ms((MixPlan *)plan, ms);
 }
 
+int cgm_plan_wfa_kernel_ms(void *plan, float *ms) {
+    if (!plan || !ms) return set_error(CGM_ERR_INVALID, "null argument");
+    return mix_plan_wfa_ms((MixPlan *)plan, ms);
+}
+
 int cgm_plan_replay(void *plan, void *stream) {
     if (!plan) return set_error(CGM_ERR_INVALID, "null plan");
     return mix_replay((MixPlan *)plan, (hipStream_t)stream);

@@ -165,12 +165,12 @@ template <int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 6 : 8)))
 void wfa_wave_kernel(
     Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap, WfaPen p, WfaGeom gm,
-    int32_t *out, const int32_t *order, const int32_t *split, int32_t part, int32_t *queue,
+    int32_t *out, const int32_t *order, const int32_t *range, int32_t *queue,
     int32_t lds_bytes, int32_t *slab, int64_t slab_words) {
     extern __shared__ uint8_t lds_raw[];
     __shared__ int next_item;
-    const int64_t first = part == 2 ? *split : 0;
-    const int64_t last = part == 1 ? *split : b.n_pairs;
+    const int64_t first = range[0];
+    const int64_t last = range[1];
     int32_t *my_slab = slab ? slab + (int64_t)blockIdx.x * slab_words : nullptr;
     for (;;) {
         if (threadIdx.x == 0) next_item = atomicAdd(queue, 1);
@@ -197,24 +197,28 @@ void wfa_wave_kernel(
 // set for pairs that need the large layout (they come first).  *nbig counts
 // those pairs.
 __global__ void wfa_key_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap,
-                               WfaPen p, WfaGeom gm, int32_t small_lds, uint32_t *keys,
-                               int32_t *idx, int32_t *nbig) {
+                               WfaPen p, WfaGeom gm, uint32_t *keys, int32_t *idx,
+                               int32_t *nbig) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= b.n_pairs) return;
     const int me = me_arr ? me_arr[i] : me_scalar;
-    const int64_t R = me >= p.o + p.e ? (me - p.o) / p.e : 0;
-    const int64_t steps = (me > 0 ? me / gm.g : 0) * (2 * R + 1);
     const int ql = b.q_len[i], rl = b.r_len[i];
-    bool small = false;
-    if (me >= 0 && me <= me_cap && ql >= 0 && rl >= 0) {
-        const int K = min(min(me, max(ql, rl)), reach(me, p));
-        small = rows_fit_int16(ql, me, p) && team_fast_bytes(ql, rl, 2 * K + 1, gm) <= small_lds;
-    }
-    keys[i] = (small ? 0u : 0x80000000u) | (uint32_t)min(steps, (int64_t)0x7fffffff);
+    const bool small = me >= 0 && me <= me_cap && ql >= 0 && rl >= 0 &&
+                       wfa_fits_small(ql, rl, me, p, gm);
+    keys[i] = (small ? 0u : 0x80000000u) | wfa_work_key(me, p, gm);
     idx[i] = (int32_t)i;
     const uint64_t m = __ballot(!small);
     if (m && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
         atomicAdd(nbig, __popcll(m));
+}
+
+// ranges[0..1] = [0, nbig), ranges[2..3] = [nbig, n) of the sorted order
+__global__ void wfa_split_ranges_kernel(const int32_t *nbig, int64_t n, int32_t *ranges) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ranges[0] = 0;
+    ranges[1] = *nbig;
+    ranges[2] = *nbig;
+    ranges[3] = (int32_t)n;
 }
 
 __global__ void zero_i32_kernel(int32_t *p, int n) {
@@ -277,6 +281,31 @@ int wfa_geometry(const WfaPen &p, int *g, int *nM, int *nI) {
     return CGM_OK;
 }
 
+int launch_wfa_wave_ranges(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                           int32_t me_cap, WfaPen p, int32_t *out, const int32_t *order,
+                           const int32_t *range_big, const int32_t *range_small, int32_t *queues,
+                           int32_t *slab, int64_t slab_words, int grid_big, int grid_small,
+                           hipStream_t stream) {
+    WfaGeom gm;
+    wfa_geometry(p, &gm.g, &gm.nM, &gm.nI);
+    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreadsBig>, kWfaLdsBytes))
+        return rc;
+    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreads>, kWfaSmallLdsBytes))
+        return rc;
+    if (range_big && grid_big > 0)
+        hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreadsBig>, dim3((unsigned)grid_big),
+                           dim3(kTeamThreadsBig), (size_t)kWfaLdsBytes, stream, b, max_error,
+                           max_error_scalar, me_cap, p, gm, out, order, range_big, queues, kWfaLdsBytes,
+                           slab, slab_words);
+    if (range_small && grid_small > 0)
+        hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreads>, dim3((unsigned)grid_small),
+                           dim3(kTeamThreads), (size_t)kWfaSmallLdsBytes, stream, b, max_error,
+                           max_error_scalar, me_cap, p, gm, out, order, range_small, queues + 1,
+                           kWfaSmallLdsBytes, nullptr, (int64_t)0);
+    CGM_HIP(hipGetLastError());
+    return CGM_OK;
+}
+
 int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
                     int32_t *out, const int32_t *order, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
@@ -301,17 +330,13 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
     while (grid > 1 && grid * slab_words * 4 > (int64_t(4) << 30)) grid /= 2;
     int64_t grid_small = (int64_t)cus * kWfaSmallBlocksPerCu;
     if (grid_small > b.n_pairs) grid_small = b.n_pairs;
-    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreadsBig>, kWfaLdsBytes))
-        return rc;
-    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreads>, kWfaSmallLdsBytes))
-        return rc;
     int32_t *slab = nullptr, *queue = nullptr, *own_order = nullptr;
     uint32_t *keys = nullptr;
     int32_t *idx = nullptr;
     CGM_HIP(hipMallocAsync((void **)&queue, 64, stream));
     CGM_HIP(hipMallocAsync((void **)&slab, (size_t)(grid * slab_words * 4), stream));
-    // queue[0] / queue[1]: counters of the two launches; queue[2]: pairs
-    // of the large layout
+    // queue[0] / queue[1]: counters of the two launches; queue[2]: pairs of
+    // the large layout; queue[4..7]: the two launches' ranges
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, queue, 16);
     int rc = CGM_OK;
     if (!order) {
@@ -322,24 +347,23 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
         CGM_HIP(hipMallocAsync((void **)&keys, (size_t)b.n_pairs * 4, stream));
         CGM_HIP(hipMallocAsync((void **)&idx, (size_t)b.n_pairs * 4, stream));
         hipLaunchKernelGGL(wfa_key_kernel, dim3((unsigned)((b.n_pairs + 255) / 256)), dim3(256), 0,
-                           stream, b, max_error, max_error_scalar, me_max, p, gm,
-                           (int32_t)kWfaSmallLdsBytes, keys, idx, queue + 2);
+                           stream, b, max_error, max_error_scalar, me_max, p, gm, keys, idx,
+                           queue + 2);
         rc = sort_desc_by_key(keys, idx, own_order, b.n_pairs, stream);
-        if (rc == CGM_OK) {
-            hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreadsBig>, dim3((unsigned)grid),
-                               dim3(kTeamThreadsBig), (size_t)kWfaLdsBytes, stream, b, max_error,
-                               max_error_scalar, me_max, p, gm, out, own_order, queue + 2, 1,
-                               queue, kWfaLdsBytes, slab, slab_words);
-            hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreads>, dim3((unsigned)grid_small),
-                               dim3(kTeamThreads), (size_t)kWfaSmallLdsBytes, stream, b,
-                               max_error, max_error_scalar, me_max, p, gm, out, own_order,
-                               queue + 2, 2, queue + 1, kWfaSmallLdsBytes, nullptr, (int64_t)0);
-        }
+        hipLaunchKernelGGL(wfa_split_ranges_kernel, dim3(1), dim3(64), 0, stream, queue + 2,
+                           b.n_pairs, queue + 4);
+        if (rc == CGM_OK)
+            rc = launch_wfa_wave_ranges(b, max_error, max_error_scalar, me_max, p, out, own_order,
+                                        queue + 4, queue + 6, queue, slab, slab_words, (int)grid,
+                                        (int)grid_small, stream);
     } else {
-        hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreadsBig>, dim3((unsigned)grid),
-                           dim3(kTeamThreadsBig), (size_t)kWfaLdsBytes, stream, b, max_error,
-                           max_error_scalar, me_max, p, gm, out, order, nullptr, 0, queue,
-                           kWfaLdsBytes, slab, slab_words);
+        // caller's order: every pair on the large layout, in that order
+        // (queue[3] is 0, so queue[6..7] = [0, n))
+        hipLaunchKernelGGL(wfa_split_ranges_kernel, dim3(1), dim3(64), 0, stream, queue + 3,
+                           b.n_pairs, queue + 4);
+        rc = launch_wfa_wave_ranges(b, max_error, max_error_scalar, me_max, p, out, order,
+                                    queue + 6, nullptr, queue, slab, slab_words, (int)grid, 0,
+                                    stream);
     }
     hipError_t le = hipGetLastError();
     CGM_HIP(hipFreeAsync(slab, stream));

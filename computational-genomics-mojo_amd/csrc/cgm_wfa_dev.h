@@ -246,6 +246,25 @@ __device__ __forceinline__ int64_t team_fast_bytes(int qlen, int rlen, int W, co
     return 4 * (nq + nr) + 2 * m_ents + 4 * gm.nI * WP + kFastSlack;
 }
 
+// Whether a pair runs on the small workgroups of the WFA launchers: int16
+// rows and the branch-free LDS layout within kWfaSmallLdsBytes.  The other
+// pairs take the 512-thread workgroups (52 KB of LDS, tiled over an HBM
+// slab beyond that).
+__device__ __forceinline__ bool wfa_fits_small(int qlen, int rlen, int me, const WfaPen &p,
+                                               const WfaGeom &gm) {
+    const int K = min(min(me, max(qlen, rlen)), reach(me, p));
+    return rows_fit_int16(qlen, me, p) &&
+           team_fast_bytes(qlen, rlen, 2 * K + 1, gm) <= kWfaSmallLdsBytes;
+}
+
+// Work key of a WFA pair for the longest-first order: diagonal count summed
+// over the scores up to max_error (an upper bound of the cost), < 2^27.
+__device__ __forceinline__ uint32_t wfa_work_key(int me, const WfaPen &p, const WfaGeom &gm) {
+    const int64_t R = me >= p.o + p.e ? (me - p.o) / p.e : 0;
+    const int64_t steps = (me > 0 ? me / gm.g : 0) * (2 * R + 1);
+    return (uint32_t)min(steps, (int64_t)((1 << 27) - 1));
+}
+
 // Bases pos .. pos + 31 of an LSB-first sequence as two words (base pos + i
 // at bits 2i of w0 for i < 16, of w1 for i >= 16).
 __device__ __forceinline__ void window_lsb(const uint32_t *s, int pos, uint32_t &w0, uint32_t &w1) {

@@ -245,6 +245,10 @@ int cgm_wfa_cigar_host(const uint64_t *q_packed, const uint64_t *r_packed, const
  * The batched persistent form of the two launchers above for a batch that
  * mixes SW pairs (kind[i] == 0) and WFA pairs (kind[i] == 1): one call
  * aligns every pair, writing the SW score/ends or the WFA cost (ends -1).
+ * SW pairs run one persistent kernel over a work-sorted task queue; WFA
+ * pairs run the two longest-first kernels of cgm_wfa_affine_launch's
+ * automatic layout on a stream forked from the call's (and joined back),
+ * over the plan's buffers.
  * No reference counterpart takes a mixed batch; per pair the results are
  * those of launch_banded_sw / launch_wfa_affine (align_gpu.mojo:460-535),
  * i.e. of smith_waterman_banded / wfa_affine_cpu.
@@ -279,21 +283,26 @@ int cgm_plan_capture(void *plan, const uint64_t *q_packed, const uint64_t *r_pac
                      int32_t x, int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
                      int32_t *out_r_end, void *stream);
 int cgm_plan_replay(void *plan, void *stream);
-/* Duration of the persistent kernel of the plan's last call (or replay),
- * from events recorded around it on the call's stream; waits for it. */
+/* Duration of the SW persistent kernel of the plan's last call (or
+ * replay), from events recorded around it on the call's stream; waits for
+ * it. */
 int cgm_plan_kernel_ms(void *plan, float *ms);
+/* Duration of the last call's WFA kernels (the WFA pairs run the kernels
+ * of cgm_wfa_affine_launch's automatic layout on a stream forked from the
+ * call's and joined back), from events around them; 0 when the call had no
+ * WFA pair; waits for them. */
+int cgm_plan_wfa_kernel_ms(void *plan, float *ms);
 
 /* ------------------------------------------------------------ autotuning
  * Layout choices of the persistent kernel, as an int32 array of
  * CGM_TUNE_N entries (no reference counterpart: the reference has one
  * launch shape per kernel, align_gpu.mojo:24-38).  None changes a result.
- *   BLOCKS_PER_CU   four-wavefront workgroups per CU (1 or 2)
- *   LDS_BYTES       LDS per workgroup (blocks_per_cu x (lds + 256) <= 160 KiB)
- *   TILE_STEPS      WFA scores per block of the tiled layout (pairs whose
- *                   wavefronts exceed LDS)
- *   WAVE_MAX_WIDTH  switch-over: WFA pairs with at most this many diagonals
- *                   (2 R(max_error) + 1) that fit a quarter of the LDS run one
- *                   per wavefront, the others one per workgroup (0: never)
+ *   BLOCKS_PER_CU   four-wavefront SW workgroups per CU (1 or 2)
+ *   LDS_BYTES       validated (blocks_per_cu x (lds + 256) <= 160 KiB) and
+ *                   kept for the ABI; the SW persistent kernel uses no LDS
+ *   TILE_STEPS      kept for the ABI (1..1024); WFA pairs run the WFA
+ *                   launcher's own layout
+ *   WAVE_MAX_WIDTH  kept for the ABI (>= 0); as TILE_STEPS
  *   REG_MAX_BAND    band split: SW bands up to this (<= 64) run in register
  *                   lanes, wider ones in LDS lanes (band 128: packed lanes)
  *   LANE_MAX_LEN    switch-over: SW pairs longer than this run one workgroup
