@@ -599,7 +599,7 @@ int cgm_wfa_cigar_launch(const uint64_t *q_packed, const uint64_t *r_packed,
     int g = 0, nM = 0, nI = 0;
     if (int rc = wfa_geometry(p, &g, &nM, &nI)) return rc;
     // 1. costs (the automatic WFA launcher)
-    if (int rc = launch_wfa_wave(b, max_error, max_error_scalar, p, out_cost, nullptr, stream))
+    if (int rc = launch_wfa_wave(b, max_error, max_error_scalar, p, out_cost, true, stream))
         return rc;
     // 2. history sizes -> host, chunks within the budget
     int64_t *d_bytes = nullptr, *d_off = nullptr;

@@ -84,18 +84,27 @@ int launch_sw_plain(const Batch &b, const int32_t *band, int32_t band_scalar, Sw
 int launch_wfa_plain(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
                      WfaPen p, int32_t *out, hipStream_t stream);
 // order: optional device permutation of pair indices (work-sorted); NULL = 0..n-1.
+// reg: register layouts where they apply (else the team layouts only).
 int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
-                    WfaPen p, int32_t *out, const int32_t *order, hipStream_t stream);
-// The two persistent WFA launches over device ranges of `order` (pairs
-// order[range[0]] .. order[range[1] - 1]): range_big on 512-thread
-// workgroups (slab: grid_big x slab_words int32 for pairs beyond LDS),
-// range_small on 256-thread ones; queues[0], queues[1]: zeroed counters.
-// Pairs with max_error above me_cap get CGM_BAD_PAIR.  Allocates nothing.
-int launch_wfa_wave_ranges(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
-                           int32_t me_cap, WfaPen p, int32_t *out, const int32_t *order,
-                           const int32_t *range_big, const int32_t *range_small, int32_t *queues,
-                           int32_t *slab, int64_t slab_words, int grid_big, int grid_small,
-                           hipStream_t stream);
+                    WfaPen p, int32_t *out, bool reg, hipStream_t stream);
+// The pieces of the automatic WFA layout, on caller-provided buffers (no
+// allocation): classify every pair into the 8 layout classes (counts: 8
+// zeroed ints; bad pairs get CGM_BAD_PAIR), class ranges of the sorted
+// order from the counts (start: device offset of the first class, or
+// null for 0; ranges: 16 ints), and the class kernels over those ranges
+// (queues: 8 zeroed counters; slab: grid_big x slab_words int32 for the
+// largest pairs), split over two streams.
+bool wfa_reg_penalties(const WfaPen &p);
+int wfa_slab_words(int32_t me_cap, const WfaPen &p);
+int launch_wfa_classify(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                        int32_t me_cap, WfaPen p, bool reg, int32_t *out, uint32_t *keys,
+                        int32_t *idx, int32_t *counts, hipStream_t stream);
+int launch_wfa_class_ranges(const int32_t *counts, const int32_t *start, int32_t *ranges,
+                            hipStream_t stream);
+int launch_wfa_classes(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                       int32_t me_cap, WfaPen p, bool reg, int32_t *out, const int32_t *order,
+                       const int32_t *ranges, int32_t *queues, int32_t *slab, int64_t slab_words,
+                       int grid_big, hipStream_t s_long, hipStream_t s_short);
 
 int wfa_geometry(const WfaPen &p, int *g, int *nM, int *nI);
 // One-workgroup-per-pair baselines restricted to one kind of a mixed batch

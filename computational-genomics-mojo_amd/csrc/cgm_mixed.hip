@@ -7,16 +7,16 @@
 // Pipeline (all device work; WFA pairs on a forked stream):
 //  1. classify each pair (kind, effective band, lengths, max_error) into
 //     0 SW register lanes (band <= reg_max_band <= 64), 1 SW packed register
-//     lanes (band 128), 2 WFA on 512-thread workgroups, 3 SW LDS lanes, 4 SW
-//     one workgroup per pair (also SW pairs longer than lane_max_len), 5 bad,
-//     6 WFA on 256-thread workgroups (int16 rows, branch-free LDS layout
-//     within 20 KB); bad pairs get their outputs written here, WFA pairs
+//     lanes (band 128), 2 WFA, 3 SW LDS lanes, 4 SW one workgroup per pair
+//     (also SW pairs longer than lane_max_len), 5 bad; WFA pairs also get
+//     the WFA launcher's layout class (register layouts by width, team
+//     layouts beyond); bad pairs get their outputs written here, WFA pairs
 //     their ends (-1);
-//  2. radix-sort pair indices by (class, band, length) or (class, WFA work);
-//  3. the WFA classes run the WFA launcher's two persistent kernels over
-//     their ranges of the order, longest first, on a stream forked from the
-//     caller's (its kernels keep their own register and LDS budgets:
-//     6 and 8 wavefronts per SIMD);
+//  2. radix-sort pair indices by (class, band, length) or (class, WFA
+//     layout class, WFA work);
+//  3. the WFA pairs run the WFA launcher's class kernels over their ranges
+//     of the order, longest first, on two streams forked from the caller's
+//     (its kernels keep their own register and LDS budgets);
 //  4. build SW tasks: 256 consecutive SW pairs of one register class per
 //     task, each with an estimated cost, and radix-sort them largest first;
 //  5. the SW persistent kernel: CUs x blocks_per_cu four-wavefront
@@ -36,15 +36,15 @@
 #include <hipcub/hipcub.hpp>
 
 #include "cgm_sw_dev.h"
-#include "cgm_wfa_dev.h"
+#include "cgm_wfa_reg.h"
 
 namespace cgm {
 
 namespace {
 
-constexpr int kMixClasses = 7;
+constexpr int kMixClasses = 6;
 constexpr int kMixSwReg = 0, kMixSwPacked = 1, kMixWfa = 2, kMixSwLds = 3, kMixSwPlain = 4,
-              kMixBad = 5, kMixWfaSmall = 6;
+              kMixBad = 5;
 constexpr int kMixWaves = 4;              // wavefronts per workgroup
 constexpr int kMixMaxBlocksPerCu = 2;     // 256 VGPRs: two workgroups per CU at most
 constexpr int kMixMaxLds = 160 * 1024;    // LDS of a CU
@@ -67,8 +67,10 @@ constexpr int kSmallBandCount = 32;          // register-lane pairs per band [kR
 constexpr int kSmallBandTask = 32 + 72;      // first task of each band, widest first [kRegBands+1]
 constexpr int kSmallBandPair = 32 + 2 * 72;  // first pair (in the class) of each band [kRegBands]
 constexpr int kSmallUnits = 28;              // SW kernel: long/short unit counters (64 bit)
-constexpr int kSmallWfaQueues = 30;          // counters of the two WFA launches
-constexpr int kSmallSimdSlots = 256;         // wavefronts started per SIMD [kSimdKeys]
+constexpr int kSmallWfaCounts = 256;         // WFA pairs per layout class [8]
+constexpr int kSmallWfaRanges = 264;         // [start, end) per WFA layout class [16]
+constexpr int kSmallWfaQueues = 280;         // WFA class kernels' counters [8]
+constexpr int kSmallSimdSlots = 512;         // wavefronts started per SIMD [kSimdKeys]
 constexpr int kSimdKeys = 8 << 10;           // XCC (3 bits), CU bits of HW_ID (8), SIMD (2)
 constexpr int kSmallWords = kSmallSimdSlots + kSimdKeys;
 
@@ -86,8 +88,9 @@ struct Task {
 __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scalar,
                                  const int32_t *band, int32_t band_scalar, const int32_t *me_arr,
                                  int32_t me_scalar, int32_t me_cap, SwScoring sc, WfaPen p,
-                                 wfadev::WfaGeom gm, MixTuning tune, SwOut out, uint32_t *keys,
-                                 int32_t *idx, int32_t *counts, int32_t *band_counts) {
+                                 wfadev::WfaGeom gm, int32_t reg_pen, MixTuning tune, SwOut out,
+                                 uint32_t *keys, int32_t *idx, int32_t *counts,
+                                 int32_t *band_counts, int32_t *wfa_counts) {
     __shared__ int32_t hist[kRegBands];
     __shared__ int32_t cls_count[kMixClasses];
     for (int j = threadIdx.x; j < kRegBands; j += blockDim.x) hist[j] = 0;
@@ -101,6 +104,7 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
         const uint64_t *q, *r;
         int ql = 0, rl = 0;
         int cls = in ? kMixBad : -1;
+        int wcls = -1;
         uint32_t payload = 0;
         if (in && load_pair(b, i, q, r, ql, rl)) {
             const int k = kind ? kind[i] : kind_scalar;
@@ -133,8 +137,9 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
             } else if (k == 1) {
                 const int me = me_arr ? me_arr[i] : me_scalar;
                 if (me >= 0 && me <= me_cap) {
-                    cls = wfadev::wfa_fits_small(ql, rl, me, p, gm) ? kMixWfaSmall : kMixWfa;
-                    payload = wfadev::wfa_work_key(me, p, gm);
+                    cls = kMixWfa;
+                    wcls = wfareg::wfa_pair_class(ql, rl, me, p, gm, reg_pen != 0);
+                    payload = wfareg::wfa_class_key(wcls, wfadev::wfa_work_key(me, p, gm));
                 }
             }
         }
@@ -142,8 +147,15 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
             const uint64_t m = __ballot(cls == c);
             if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&cls_count[c], __popcll(m));
         }
+        if (__ballot(wcls >= 0)) {
+            for (int c = 0; c < wfareg::kWfaClasses; ++c) {
+                const uint64_t m = __ballot(wcls == c);
+                if (m && lane == __ffsll((unsigned long long)m) - 1)
+                    atomicAdd(&wfa_counts[c], __popcll(m));
+            }
+        }
         if (in) {
-            if (cls == kMixBad || cls == kMixWfa || cls == kMixWfaSmall) {
+            if (cls == kMixBad || cls == kMixWfa) {
                 if (cls == kMixBad) out.score[i] = CGM_BAD_PAIR;
                 if (out.q_end) out.q_end[i] = -1;
                 if (out.r_end) out.r_end[i] = -1;
@@ -183,6 +195,13 @@ __global__ void mix_ranges_kernel(int32_t *small) {
     small[kSmallBandTask + kRegBands] = tasks;
     small[kSmallRegTasks] = tasks;
     small[kSmallNTasks] = tasks + (counts[kMixSwPacked] + kMixGroup - 1) / kMixGroup;
+    // WFA layout classes inside the WFA class's range (sorted class first)
+    int32_t w = ranges[2 * kMixWfa];
+    for (int c = 0; c < wfareg::kWfaClasses; ++c) {
+        small[kSmallWfaRanges + 2 * c] = w;
+        w += small[kSmallWfaCounts + c];
+        small[kSmallWfaRanges + 2 * c + 1] = w;
+    }
 }
 
 __global__ void mix_tasks_kernel(const int32_t *small, const uint32_t *sorted_keys, int64_t cap,
@@ -314,8 +333,9 @@ struct MixPlan {
     bool had_wfa = false;  // the last enqueue launched the WFA kernels
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // around the SW persistent kernel
     hipEvent_t ev_wfa_start = nullptr, ev_wfa_stop = nullptr;  // around the WFA kernels
+    hipEvent_t ev_wfa_mid = nullptr;  // end of the second WFA stream (no timing)
     ~MixPlan() {
-        for (hipEvent_t e : {ev_start, ev_stop, ev_wfa_start, ev_wfa_stop})
+        for (hipEvent_t e : {ev_start, ev_stop, ev_wfa_start, ev_wfa_stop, ev_wfa_mid})
             if (e) (void)hipEventDestroy(e);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -449,7 +469,8 @@ int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per
 #undef CGM_ALLOC
     if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess ||
         hipEventCreate(&P->ev_wfa_start) != hipSuccess ||
-        hipEventCreate(&P->ev_wfa_stop) != hipSuccess)
+        hipEventCreate(&P->ev_wfa_stop) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_wfa_mid, hipEventDisableTiming) != hipSuccess)
         return fail(set_error(CGM_ERR_HIP, "plan: timing events"));
     *out = P;
     return CGM_OK;
@@ -556,25 +577,28 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     const unsigned class_grid = (unsigned)std::min<int64_t>((n + 255) / 256, P->cus);
     hipLaunchKernelGGL(mix_class_kernel, dim3(class_grid), dim3(256), 0, stream, b,
                        kind, kind_scalar, band, band_scalar, me, me_scalar, P->me_cap, sc, pen,
-                       P->gm, tune, out, P->keys, P->idx, counts, P->small + kSmallBandCount);
+                       P->gm, (int32_t)wfa_reg_penalties(pen), tune, out, P->keys, P->idx, counts,
+                       P->small + kSmallBandCount, P->small + kSmallWfaCounts);
     hipLaunchKernelGGL(mix_ranges_kernel, dim3(1), dim3(64), 0, stream, P->small);
     size_t tb = P->sort_tmp_bytes;
     CGM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(P->sort_tmp, tb, P->keys, P->keys_out,
                                                          P->idx, P->order, (int)n, 0, 32, stream));
-    // WFA pairs: the WFA launcher's kernels on a forked stream, launched
-    // first so that the SW persistent kernel fills the CUs their tail frees
+    // WFA pairs: the WFA launcher's class kernels on two forked streams,
+    // launched first so that the SW persistent kernel fills the CUs their
+    // tails free
     const bool has_wfa = kind != nullptr || kind_scalar == 1;
-    hipStream_t aux = stream;
+    hipStream_t aux[2] = {stream, stream};
     if (has_wfa) {
-        if (int rc = fork_streams(stream, 1, &aux)) return rc;
-        const int grid_small = P->cus * kWfaSmallBlocksPerCu;
-        CGM_HIP(hipEventRecord(P->ev_wfa_start, aux));
-        if (int rc = launch_wfa_wave_ranges(b, me, me_scalar, P->me_cap, pen, out.score, P->order,
-                                            ranges + 2 * kMixWfa, ranges + 2 * kMixWfaSmall,
-                                            P->small + kSmallWfaQueues, P->slab, P->slab_words,
-                                            P->wfa_grid, grid_small, aux))
+        if (int rc = fork_streams(stream, 2, aux)) return rc;
+        CGM_HIP(hipEventRecord(P->ev_wfa_start, aux[0]));
+        if (int rc = launch_wfa_classes(b, me, me_scalar, P->me_cap, pen, true, out.score, P->order,
+                                        P->small + kSmallWfaRanges, P->small + kSmallWfaQueues,
+                                        P->slab, P->slab_words, P->wfa_grid, aux[0], aux[1]))
             return rc;
-        CGM_HIP(hipEventRecord(P->ev_wfa_stop, aux));
+        // the WFA end: both streams done
+        CGM_HIP(hipEventRecord(P->ev_wfa_mid, aux[1]));
+        CGM_HIP(hipStreamWaitEvent(aux[0], P->ev_wfa_mid, 0));
+        CGM_HIP(hipEventRecord(P->ev_wfa_stop, aux[0]));
     }
     P->had_wfa = has_wfa;
     // SW tasks: at most ceil(n / kMixGroup) + kRegBands + 1 (band-aligned
@@ -600,7 +624,7 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     if (int rc = launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
                                        ranges + 2 * kMixSwPlain, P->cus * 2, stream))
         return rc;
-    return has_wfa ? join_streams(stream, 1, &aux) : CGM_OK;
+    return has_wfa ? join_streams(stream, 2, aux) : CGM_OK;
 }
 
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,

@@ -434,8 +434,10 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed, co
             return launch_wfa_plain(b, max_error, max_error_scalar, p, out_cost,
                                     (hipStream_t)stream);
         case CGM_LAYOUT_AUTO:
+            return launch_wfa_wave(b, max_error, max_error_scalar, p, out_cost, true,
+                                   (hipStream_t)stream);
         case CGM_LAYOUT_WAVE:
-            return launch_wfa_wave(b, max_error, max_error_scalar, p, out_cost, nullptr,
+            return launch_wfa_wave(b, max_error, max_error_scalar, p, out_cost, false,
                                    (hipStream_t)stream);
         default:
             return set_error(CGM_ERR_INVALID, "unknown WFA layout");

@@ -11,7 +11,9 @@
 // the target diagonal qlen - rlen and clamped to +-max_error, and the result
 // is the first s with M[s, qlen-rlen] >= qlen, or -1 past max_error.
 
-#include "cgm_wfa_dev.h"
+#include <algorithm>
+
+#include "cgm_wfa_reg.h"
 
 namespace cgm {
 
@@ -192,33 +194,75 @@ void wfa_wave_kernel(
     }
 }
 
-// WFA work key for the longest-first order: sum over scores of the
-// diagonal count, from max_error (an upper bound of the cost), with bit 31
-// set for pairs that need the large layout (they come first).  *nbig counts
-// those pairs.
-__global__ void wfa_key_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, int32_t me_cap,
-                               WfaPen p, WfaGeom gm, uint32_t *keys, int32_t *idx,
-                               int32_t *nbig) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= b.n_pairs) return;
-    const int me = me_arr ? me_arr[i] : me_scalar;
-    const int ql = b.q_len[i], rl = b.r_len[i];
-    const bool small = me >= 0 && me <= me_cap && ql >= 0 && rl >= 0 &&
-                       wfa_fits_small(ql, rl, me, p, gm);
-    keys[i] = (small ? 0u : 0x80000000u) | wfa_work_key(me, p, gm);
-    idx[i] = (int32_t)i;
-    const uint64_t m = __ballot(!small);
-    if (m && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1)
-        atomicAdd(nbig, __popcll(m));
+// Register layout: one pair per workgroup of NW wavefronts (NB blocks of
+// 64 diagonals each), pairs order[range[0] .. range[1]) from an atomic queue.
+template <int NB, int NW, int WPE>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void wfa_reg_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar, WfaPen p, int32_t g,
+                    int32_t *out, const int32_t *order, const int32_t *range, int32_t *queue) {
+    extern __shared__ uint8_t lds_raw[];
+    __shared__ int next_item;
+    const int64_t first = range[0];
+    const int64_t last = range[1];
+    for (;;) {
+        if (threadIdx.x == 0) next_item = atomicAdd(queue, 1);
+        __syncthreads();
+        const int64_t item = first + next_item;
+        __syncthreads();
+        if (item >= last) break;
+        const int64_t pair = order[item];
+        const uint64_t *q, *r;
+        int qlen, rlen;
+        load_pair(b, pair, q, r, qlen, rlen);  // checked by the classifier
+        const int me = me_arr ? me_arr[pair] : me_scalar;
+        const int res = wfareg::wfa_reg_pair<NB, NW, wfareg::kRegDX, wfareg::kRegDO,
+                                             wfareg::kRegDE>(q, qlen, r, rlen, me, p, g, lds_raw);
+        if (threadIdx.x == 0) out[pair] = res;
+        __syncthreads();
+    }
 }
 
-// ranges[0..1] = [0, nbig), ranges[2..3] = [nbig, n) of the sorted order
-__global__ void wfa_split_ranges_kernel(const int32_t *nbig, int64_t n, int32_t *ranges) {
+// Layout class and longest-first key of every pair (class first); bad
+// pairs get CGM_BAD_PAIR here and a key that sorts them last, outside every
+// class range.  counts[c]: pairs of class c.
+__global__ void wfa_class_kernel(Batch b, const int32_t *me_arr, int32_t me_scalar,
+                                 int32_t me_cap, WfaPen p, WfaGeom gm, int32_t reg_pen,
+                                 int32_t *out, uint32_t *keys, int32_t *idx, int32_t *counts) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool in = i < b.n_pairs;
+    int cls = -1;
+    uint32_t key = 0;
+    if (in) {
+        const uint64_t *q, *r;
+        int ql, rl;
+        const int me = me_arr ? me_arr[i] : me_scalar;
+        if (load_pair(b, i, q, r, ql, rl) && me >= 0 && me <= me_cap) {
+            cls = wfareg::wfa_pair_class(ql, rl, me, p, gm, reg_pen != 0);
+            key = wfareg::wfa_class_key(cls, wfa_work_key(me, p, gm));
+        } else {
+            out[i] = CGM_BAD_PAIR;
+        }
+        keys[i] = key;
+        idx[i] = (int32_t)i;
+    }
+    const int lane = threadIdx.x & 63;
+    for (int c = 0; c < wfareg::kWfaClasses; ++c) {
+        const uint64_t m = __ballot(cls == c);
+        if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&counts[c], __popcll(m));
+    }
+}
+
+// ranges[2c], ranges[2c + 1]: class c's part of the sorted order, which
+// starts at `start` (a device value, or 0 when null)
+__global__ void wfa_class_ranges_kernel(const int32_t *counts, const int32_t *start,
+                                        int32_t *ranges) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    ranges[0] = 0;
-    ranges[1] = *nbig;
-    ranges[2] = *nbig;
-    ranges[3] = (int32_t)n;
+    int32_t s = start ? *start : 0;
+    for (int c = 0; c < wfareg::kWfaClasses; ++c) {
+        ranges[2 * c] = s;
+        s += counts[c];
+        ranges[2 * c + 1] = s;
+    }
 }
 
 __global__ void zero_i32_kernel(int32_t *p, int n) {
@@ -281,33 +325,91 @@ int wfa_geometry(const WfaPen &p, int *g, int *nM, int *nI) {
     return CGM_OK;
 }
 
-int launch_wfa_wave_ranges(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
-                           int32_t me_cap, WfaPen p, int32_t *out, const int32_t *order,
-                           const int32_t *range_big, const int32_t *range_small, int32_t *queues,
-                           int32_t *slab, int64_t slab_words, int grid_big, int grid_small,
-                           hipStream_t stream) {
+bool wfa_reg_penalties(const WfaPen &p) {
+    int g = 0, nM = 0, nI = 0;
+    wfa_geometry(p, &g, &nM, &nI);
+    return p.x == wfareg::kRegDX * g && p.o + p.e == wfareg::kRegDO * g &&
+           p.e == wfareg::kRegDE * g;
+}
+
+int launch_wfa_classify(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                        int32_t me_cap, WfaPen p, bool reg, int32_t *out, uint32_t *keys,
+                        int32_t *idx, int32_t *counts, hipStream_t stream) {
     WfaGeom gm;
     wfa_geometry(p, &gm.g, &gm.nM, &gm.nI);
-    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreadsBig>, kWfaLdsBytes))
-        return rc;
-    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreads>, kWfaSmallLdsBytes))
-        return rc;
-    if (range_big && grid_big > 0)
-        hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreadsBig>, dim3((unsigned)grid_big),
-                           dim3(kTeamThreadsBig), (size_t)kWfaLdsBytes, stream, b, max_error,
-                           max_error_scalar, me_cap, p, gm, out, order, range_big, queues, kWfaLdsBytes,
-                           slab, slab_words);
-    if (range_small && grid_small > 0)
-        hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreads>, dim3((unsigned)grid_small),
-                           dim3(kTeamThreads), (size_t)kWfaSmallLdsBytes, stream, b, max_error,
-                           max_error_scalar, me_cap, p, gm, out, order, range_small, queues + 1,
-                           kWfaSmallLdsBytes, nullptr, (int64_t)0);
+    hipLaunchKernelGGL(wfa_class_kernel, dim3((unsigned)((b.n_pairs + 255) / 256)), dim3(256), 0,
+                       stream, b, max_error, max_error_scalar, me_cap, p, gm,
+                       (int32_t)(reg && wfa_reg_penalties(p)), out, keys, idx, counts);
     CGM_HIP(hipGetLastError());
     return CGM_OK;
 }
 
+int launch_wfa_class_ranges(const int32_t *counts, const int32_t *start, int32_t *ranges,
+                            hipStream_t stream) {
+    hipLaunchKernelGGL(wfa_class_ranges_kernel, dim3(1), dim3(64), 0, stream, counts, start,
+                       ranges);
+    CGM_HIP(hipGetLastError());
+    return CGM_OK;
+}
+
+int launch_wfa_classes(const Batch &b, const int32_t *max_error, int32_t max_error_scalar,
+                       int32_t me_cap, WfaPen p, bool reg, int32_t *out, const int32_t *order,
+                       const int32_t *ranges, int32_t *queues, int32_t *slab, int64_t slab_words,
+                       int grid_big, hipStream_t s_long, hipStream_t s_short) {
+    using namespace wfareg;
+    WfaGeom gm;
+    wfa_geometry(p, &gm.g, &gm.nM, &gm.nI);
+    int cus = 0;
+    if (int rc = device_cu_count(&cus)) return rc;
+    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreadsBig>, kWfaLdsBytes))
+        return rc;
+    if (int rc = set_kernel_lds((const void *)wfa_wave_kernel<kTeamThreads>, kWfaSmallLdsBytes))
+        return rc;
+    // the longest pairs (team layout beyond the register classes, then the
+    // widest register classes) on one stream, the rest on the other, so that
+    // the two halves fill each other's tails
+    if (grid_big > 0)
+        hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreadsBig>, dim3((unsigned)grid_big),
+                           dim3(kTeamThreadsBig), (size_t)kWfaLdsBytes, s_long, b, max_error,
+                           max_error_scalar, me_cap, p, gm, out, order, ranges + 2 * kWfaTeamBig,
+                           queues + kWfaTeamBig, kWfaLdsBytes, slab, slab_words);
+#define CGM_REG_CLASS(C, STREAM)                                                                 \
+    do {                                                                                         \
+        constexpr int NB = reg_class_nb(C), NW = reg_class_nw(C), WPE = reg_class_wpe(C);       \
+        constexpr int LDS = reg_class_lds(C);                                                    \
+        const void *fn = (const void *)wfa_reg_kernel<NB, NW, WPE>;                             \
+        if (int rc = set_kernel_lds(fn, LDS)) return rc;                                         \
+        const int per_cu = std::min(4 * WPE / NW, 160 * 1024 / LDS);                            \
+        hipLaunchKernelGGL((wfa_reg_kernel<NB, NW, WPE>), dim3((unsigned)(cus * per_cu)),       \
+                           dim3(NW * 64), (size_t)LDS, STREAM, b, max_error, max_error_scalar, p, \
+                           gm.g, out, order, ranges + 2 * (C), queues + (C));                    \
+    } while (0)
+    if (reg && wfa_reg_penalties(p)) {
+        CGM_REG_CLASS(5, s_long);
+        CGM_REG_CLASS(4, s_long);
+        CGM_REG_CLASS(3, s_long);
+        CGM_REG_CLASS(2, s_short);
+        CGM_REG_CLASS(1, s_short);
+        CGM_REG_CLASS(0, s_short);
+    }
+#undef CGM_REG_CLASS
+    hipLaunchKernelGGL(wfa_wave_kernel<kTeamThreads>, dim3((unsigned)(cus * kWfaSmallBlocksPerCu)),
+                       dim3(kTeamThreads), (size_t)kWfaSmallLdsBytes, s_short, b, max_error,
+                       max_error_scalar, me_cap, p, gm, out, order, ranges + 2 * kWfaTeamSmall,
+                       queues + kWfaTeamSmall, kWfaSmallLdsBytes, nullptr, (int64_t)0);
+    CGM_HIP(hipGetLastError());
+    return CGM_OK;
+}
+
+int wfa_slab_words(int32_t me_cap, const WfaPen &p) {
+    int g = 0, nM = 0, nI = 0;
+    wfa_geometry(p, &g, &nM, &nI);
+    const int K = me_cap < reach_host(me_cap, p) ? me_cap : reach_host(me_cap, p);
+    return (nM + 2 * nI) * (2 * K + 1);
+}
+
 int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_scalar, WfaPen p,
-                    int32_t *out, const int32_t *order, hipStream_t stream) {
+                    int32_t *out, bool reg, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
     if (int rc = check_wfa_pen(p)) return rc;
     if (b.n_pairs == 0) return CGM_OK;
@@ -317,64 +419,43 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
     }
     if (me_max < 0) me_max = 0;
     if (me_max > kMaxWfaError) me_max = kMaxWfaError;
-    WfaGeom gm;
-    wfa_geometry(p, &gm.g, &gm.nM, &gm.nI);
-    const int slots = gm.nM + 2 * gm.nI;
-    const int Kmax = me_max < reach_host(me_max, p) ? me_max : reach_host(me_max, p);
-    const int64_t slab_words = (int64_t)slots * (2 * (int64_t)Kmax + 1);
+    const int64_t slab_words = wfa_slab_words(me_max, p);
     int cus = 0;
     if (int rc = device_cu_count(&cus)) return rc;
     int64_t grid = (int64_t)cus * kWfaBlocksPerCu;
-    if (grid > b.n_pairs) grid = b.n_pairs;
     // keep the HBM slabs within 4 GiB
     while (grid > 1 && grid * slab_words * 4 > (int64_t(4) << 30)) grid /= 2;
-    int64_t grid_small = (int64_t)cus * kWfaSmallBlocksPerCu;
-    if (grid_small > b.n_pairs) grid_small = b.n_pairs;
-    int32_t *slab = nullptr, *queue = nullptr, *own_order = nullptr;
+    // small: queue counters [0, 8), class counts [8, 16), class ranges [16, 32)
+    int32_t *slab = nullptr, *small = nullptr, *order = nullptr, *idx = nullptr;
     uint32_t *keys = nullptr;
-    int32_t *idx = nullptr;
-    CGM_HIP(hipMallocAsync((void **)&queue, 64, stream));
+    CGM_HIP(hipMallocAsync((void **)&small, 32 * 4, stream));
     CGM_HIP(hipMallocAsync((void **)&slab, (size_t)(grid * slab_words * 4), stream));
-    // queue[0] / queue[1]: counters of the two launches; queue[2]: pairs of
-    // the large layout; queue[4..7]: the two launches' ranges
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, queue, 16);
-    int rc = CGM_OK;
-    if (!order) {
-        // Longest first (a late long pair would otherwise leave the GPU
-        // idle), the large-layout pairs ahead of the others: they run first,
-        // on 512-thread workgroups, then the rest on small ones.
-        CGM_HIP(hipMallocAsync((void **)&own_order, (size_t)b.n_pairs * 4, stream));
-        CGM_HIP(hipMallocAsync((void **)&keys, (size_t)b.n_pairs * 4, stream));
-        CGM_HIP(hipMallocAsync((void **)&idx, (size_t)b.n_pairs * 4, stream));
-        hipLaunchKernelGGL(wfa_key_kernel, dim3((unsigned)((b.n_pairs + 255) / 256)), dim3(256), 0,
-                           stream, b, max_error, max_error_scalar, me_max, p, gm, keys, idx,
-                           queue + 2);
-        rc = sort_desc_by_key(keys, idx, own_order, b.n_pairs, stream);
-        hipLaunchKernelGGL(wfa_split_ranges_kernel, dim3(1), dim3(64), 0, stream, queue + 2,
-                           b.n_pairs, queue + 4);
-        if (rc == CGM_OK)
-            rc = launch_wfa_wave_ranges(b, max_error, max_error_scalar, me_max, p, out, own_order,
-                                        queue + 4, queue + 6, queue, slab, slab_words, (int)grid,
-                                        (int)grid_small, stream);
-    } else {
-        // caller's order: every pair on the large layout, in that order
-        // (queue[3] is 0, so queue[6..7] = [0, n))
-        hipLaunchKernelGGL(wfa_split_ranges_kernel, dim3(1), dim3(64), 0, stream, queue + 3,
-                           b.n_pairs, queue + 4);
-        rc = launch_wfa_wave_ranges(b, max_error, max_error_scalar, me_max, p, out, order,
-                                    queue + 6, nullptr, queue, slab, slab_words, (int)grid, 0,
-                                    stream);
+    CGM_HIP(hipMallocAsync((void **)&order, (size_t)b.n_pairs * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&keys, (size_t)b.n_pairs * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&idx, (size_t)b.n_pairs * 4, stream));
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, small, 32);
+    // every class longest first (a late long pair would leave the GPU idle)
+    int rc = launch_wfa_classify(b, max_error, max_error_scalar, me_max, p, reg, out, keys, idx,
+                                 small + 8, stream);
+    if (!rc) rc = sort_desc_by_key(keys, idx, order, b.n_pairs, stream);
+    if (!rc) rc = launch_wfa_class_ranges(small + 8, nullptr, small + 16, stream);
+    hipStream_t aux[2];
+    if (!rc) rc = fork_streams(stream, 2, aux);
+    if (!rc) {
+        rc = launch_wfa_classes(b, max_error, max_error_scalar, me_max, p, reg, out, order,
+                                small + 16, small, slab, slab_words, (int)grid, aux[0], aux[1]);
+        int rc2 = join_streams(stream, 2, aux);
+        if (!rc) rc = rc2;
     }
-    hipError_t le = hipGetLastError();
-    CGM_HIP(hipFreeAsync(slab, stream));
-    CGM_HIP(hipFreeAsync(queue, stream));
-    if (own_order) {
-        CGM_HIP(hipFreeAsync(own_order, stream));
-        CGM_HIP(hipFreeAsync(keys, stream));
-        CGM_HIP(hipFreeAsync(idx, stream));
-    }
+    hipError_t f1 = hipFreeAsync(slab, stream), f2 = hipFreeAsync(small, stream);
+    hipError_t f3 = hipFreeAsync(order, stream), f4 = hipFreeAsync(keys, stream);
+    hipError_t f5 = hipFreeAsync(idx, stream);
     if (rc) return rc;
-    CGM_HIP(le);
+    CGM_HIP(f1);
+    CGM_HIP(f2);
+    CGM_HIP(f3);
+    CGM_HIP(f4);
+    CGM_HIP(f5);
     return CGM_OK;
 }
 

@@ -1,0 +1,321 @@
+// cgm_wfa_reg.h -- register-resident gap-affine WFA layout (gfx950).
+//
+// Same recurrence and exact reductions as the wave layout (cgm_wfa_dev.h:
+// scores that are multiples of g only, diagonals |k| <= R(s) within
+// [-rlen, qlen], slots never cleared, NEXT and EXTEND fused), one more exact
+// reduction (cells that max_error rules out, reg_span below), and a
+// different home for the wavefronts.  Each lane owns one diagonal per
+// block: block gb holds the 64 diagonals kbase + 64 gb .. + 63 of the
+// pair's alive span [kbase, kmax], and the lane keeps that diagonal's M, I
+// and D history in VGPRs as rings of NI = DE + 1 and NM >= DM + 1 slots
+// (DM = max(x, o + e) / g, DE = e / g, NM a multiple of NI).  The score loop
+// is unrolled by the rings' common period NM, so every slot index is a
+// compile-time register; a slot is read without an existence test because
+// before its score it still holds the invalid value it started with.
+//   I[s,k] = max(M[s-o-e,k-1], I[s-e,k-1]) + 1   -> own max, DPP wave_shr:1
+//   D[s,k] = max(M[s-o-e,k+1], D[s-e,k+1])       -> own max, DPP wave_shl:1
+//   M[s,k] = extend(max(M[s-x,k] + 1, I, D))
+// so NEXT costs no memory access at all.  Lane 0 and lane 63 take the
+// neighbour across the block edge from the adjacent block's edge value of
+// the step (max(M_o, I_e) of its lane 63, max(M_o, D_e) of its lane 0),
+// published through LDS at the end of the previous step (two buffers by
+// step parity, one barrier per step; the two outer edges read the invalid
+// value).  Blocks are interleaved over the workgroup's NW wavefronts
+// (gb = b NW + w), so every wavefront gets work from the first scores on.
+// Per score the alive diagonals form one range [lo(s), hi(s)]: lanes outside
+// it are held invalid, a block wholly outside it skips the step (a scalar
+// test) once its slots and published edges hold the invalid value, and a
+// block that leaves the range writes the invalid value for NM more scores
+// before it skips.  Sequences sit in LDS as LSB-first 32-bit words, and the
+// extension is the wave layout's 32-bases-per-step comparison.
+//
+// Pinned on the CPU by tests/test_wfa_reg_schedule.py (this data flow
+// restated in Python equals the oracle, also at tight max_error; planted
+// slips in the edge side, the ring offset, the outer edge value, the block
+// skip, the gap budget and the leaving blocks' flush each fail; the span's
+// bisection equals a scan over all scores).
+
+#pragma once
+
+#include "cgm_wfa_dev.h"
+
+namespace cgm {
+namespace wfareg {
+
+using wfadev::extend_lsb;
+using wfadev::match32;
+using wfadev::med3;
+using wfadev::window_lsb;
+
+constexpr int kNeg = -(1 << 30);
+
+__host__ __device__ constexpr int gcd_c(int a, int b) { return b ? gcd_c(b, a % b) : a; }
+__host__ __device__ constexpr int pos_mod(int a, int n) { return ((a % n) + n) % n; }
+
+// Penalties in units of g: x / g, (o + e) / g, e / g.  The instantiated
+// set is x:o+e:e = 2:4:1 (the reference's defaults 4, 6, 2 and every
+// multiple of them); other penalties take the wave layout.
+constexpr int kRegDX = 2, kRegDO = 4, kRegDE = 1;
+
+// LDS bytes a pair needs: the two sequences as LSB-first words, the edge
+// buffers of nblk_cap blocks (+2 outer entries) and two done flags.
+__host__ __device__ inline int64_t reg_lds_bytes(int qlen, int rlen, int nblk_cap) {
+    const int64_t nq = 2 * (int64_t)((qlen + 31) >> 5) + 4, nr = 2 * (int64_t)((rlen + 31) >> 5) + 4;
+    return 4 * (nq + nr) + 4 * 4 * ((int64_t)nblk_cap + 2) + 16;
+}
+
+// Exact pruning by max_error: from a cell at score s on diagonal k every
+// completion still needs |k - t| gap characters (t = qlen - rlen) at e or
+// more each, so a cell with e |k - t| > me - s lies on no alignment within
+// max_error and is held invalid.  Diagonal k is then alive at score s for
+// lo(s) <= k <= hi(s), lo(s) = max(-rlen, -R(s), t - P(s)),
+// hi(s) = min(qlen, R(s), t + P(s)), P(s) = (me - s) / e; and a pair with
+// e |t| > me has no alignment within max_error at all.
+//
+// G(u) = max over s = 0, g, .. <= me of min(R(s), u + (me - s) / e), for
+// u + me / e >= 0: R rises and the second term falls with s, so the
+// maximum sits where they cross, found by bisection on s / g.
+__host__ __device__ inline int reg_span_side(int u, int me, int o, int e, int g) {
+    auto R = [&](int s) { return s >= o + e ? (s - o) / e : 0; };
+    auto H = [&](int s) { return u + (me - s) / e; };
+    int a = 0, b = me / g;  // largest j with R(j g) <= H(j g); j = 0 holds
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (R(m * g) <= H(m * g)) a = m;
+        else b = m - 1;
+    }
+    const int v = R(a * g);
+    return a < me / g ? max(v, H((a + 1) * g)) : v;
+}
+
+// The alive diagonal span [lo, hi] over all scores of a pair with
+// e |qlen - rlen| <= me.
+__host__ __device__ inline void reg_span(int qlen, int rlen, int me, int o, int e, int g,
+                                         int &lo, int &hi) {
+    const int t = qlen - rlen;
+    hi = min(qlen, reg_span_side(t, me, o, e, g));
+    lo = max(-rlen, -reg_span_side(-t, me, o, e, g));
+}
+
+// The number of 64-diagonal blocks of a pair (1 for a pair with no
+// alignment within max_error, which ends before its first step).
+__host__ __device__ inline int reg_blocks(int qlen, int rlen, int me, int o, int e, int g) {
+    const int64_t t = (int64_t)qlen - rlen;
+    if ((t < 0 ? -t : t) * e > me) return 1;
+    int lo, hi;
+    reg_span(qlen, rlen, me, o, e, g, lo, hi);
+    return (hi - lo + 64) / 64;
+}
+
+// Layout classes of a WFA pair, used by every WFA launcher: 0..5 the
+// register layout with (NB blocks per wavefront, NW wavefronts) =
+// (2, 1), (4, 1), (8, 1), (8, 4), (8, 8), (8, 16), each within an LDS
+// budget; 6 the 512-thread team layout (52 KB, tiled over an HBM slab
+// beyond it), 7 the 256-thread team layout (int16 rows within 20 KB).
+constexpr int kWfaClasses = 8;
+constexpr int kWfaRegClasses = 6;
+constexpr int kWfaTeamBig = 6, kWfaTeamSmall = 7;
+__host__ __device__ constexpr int reg_class_nb(int c) { return c == 0 ? 2 : c == 1 ? 4 : 8; }
+__host__ __device__ constexpr int reg_class_nw(int c) {
+    return c <= 2 ? 1 : c == 3 ? 4 : c == 4 ? 8 : 16;
+}
+__host__ __device__ constexpr int reg_class_lds(int c) {
+    return c == 0 ? 4096 : c == 1 ? 6144 : c == 2 ? 8192 : c == 3 ? 16384 : c == 4 ? 24576 : 32768;
+}
+// waves per SIMD each class's registers allow (kept by amdgpu_waves_per_eu)
+__host__ __device__ constexpr int reg_class_wpe(int c) { return c == 0 ? 8 : c == 1 ? 6 : 4; }
+
+__device__ __forceinline__ int wfa_pair_class(int qlen, int rlen, int me, const WfaPen &p,
+                                              const wfadev::WfaGeom &gm, bool reg_pen) {
+    if (reg_pen) {
+        const int nblk = reg_blocks(qlen, rlen, me, p.o, p.e, gm.g);
+#pragma unroll
+        for (int c = 0; c < kWfaRegClasses; ++c)
+            if (nblk <= reg_class_nb(c) * reg_class_nw(c) &&
+                reg_lds_bytes(qlen, rlen, reg_class_nb(c) * reg_class_nw(c)) <= reg_class_lds(c))
+                return c;
+    }
+    return wfadev::wfa_fits_small(qlen, rlen, me, p, gm) ? kWfaTeamSmall : kWfaTeamBig;
+}
+
+// Sort key of a WFA pair: its class (first classes first in a descending
+// sort), then its work (longest first); never 0, the key of excluded pairs.
+__device__ __forceinline__ uint32_t wfa_class_key(int cls, uint32_t work) {
+    return ((uint32_t)(kWfaClasses - 1 - cls) << 25) | (min(work, (uint32_t)((1 << 25) - 2)) + 1);
+}
+
+// Ring sizes: NI = DE + 1 I and D slots, and M slots rounded up from
+// max(DX, DO) + 1 to a multiple of NI, so that both rings repeat with the
+// M ring's period P (a shorter unrolled score loop; the extra M slot is
+// never read).
+template <int DX, int DO, int DE>
+struct RegRing {
+    static constexpr int NI = DE + 1;
+    static constexpr int NM = ((DX > DO ? DX : DO) + 1 + NI - 1) / NI * NI;
+    static constexpr int P = NM;
+};
+
+// One pair by a workgroup of NW wavefronts with NB blocks per wavefront
+// (nblk <= NB * NW).  lds: at least reg_lds_bytes(qlen, rlen, NB * NW).
+template <int NB, int NW, int DX, int DO, int DE>
+__device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, int rlen, int me,
+                            const WfaPen &p, int g, uint8_t *lds) {
+    using Ring = RegRing<DX, DO, DE>;
+    constexpr int NM = Ring::NM, NI = Ring::NI, P = Ring::P;
+    constexpr int NBLK = NB * NW;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int qw = (qlen + 31) >> 5, rw = (rlen + 31) >> 5;
+    const int nq = 2 * qw + 4, nr = 2 * rw + 4;
+    uint32_t *qs = (uint32_t *)lds;
+    uint32_t *rs = qs + nq;
+    // edge buffers: ER[buf][gb + 1] = max(M_o, I_e) of block gb's lane 63,
+    // EL[buf][gb + 1] = max(M_o, D_e) of its lane 0; entries 0 and nblk + 1
+    // are the outer edges and stay invalid
+    int *ER = (int *)(rs + nr);
+    int *EL = ER + 2 * (NBLK + 2);
+    int *done = EL + 2 * (NBLK + 2);
+    for (int i = tid; i < nq + nr; i += NW * 64) {
+        const bool isq = i < nq;
+        const int j = isq ? i : i - nq;
+        uint32_t v = 0;
+        if ((j >> 1) < (isq ? qw : rw)) {
+            const uint64_t wd = (isq ? qg : rg)[j >> 1];
+            v = __builtin_bitreverse32((j & 1) ? (uint32_t)wd : (uint32_t)(wd >> 32));
+        }
+        qs[i] = v;
+    }
+    for (int i = tid; i < 4 * (NBLK + 2); i += NW * 64) ER[i] = kNeg;  // ER and EL
+    if (tid == 0) done[0] = done[1] = -1;
+    __syncthreads();
+
+    const int target = qlen - rlen;
+    if ((int64_t)(target < 0 ? -target : target) * p.e > me) return -1;
+    int kbase, kmax;
+    reg_span(qlen, rlen, me, p.o, p.e, g, kbase, kmax);
+    const int nblk = (kmax - kbase + 64) / 64;
+
+    int Mr[NB][NM], Ir[NB][NI], Dr[NB][NI];
+    // quiet[b]: scores block b has been wholly outside the alive range, up to
+    // NM + 1; for the first NM of them it still runs, every lane held
+    // invalid, so that its slots and published edges hold the invalid value,
+    // and from then on it skips
+    int quiet[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Mr[b][j] = kNeg;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) Ir[b][j] = Dr[b][j] = kNeg;
+        quiet[b] = NM + 1;
+    }
+    // score 0: M[0, 0] = the matching prefix, in the lane owning diagonal 0
+    {
+        const int g0 = -kbase / 64, l0 = -kbase % 64;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (b * NW + w == g0 && lane == l0) Mr[b][0] = extend_lsb(qs, qlen, rs, rlen, 0, 0);
+    }
+    // score 0 alone reaches the target only on diagonal 0
+    if (target == 0) {
+        int m0 = kNeg;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (b * NW + w == -kbase / 64 && lane == -kbase % 64) m0 = Mr[b][0];
+        if (__ballot(m0 >= qlen)) done[0] = 0;
+        __syncthreads();
+        if (done[0] == 0) return 0;
+    }
+    // edges for step 1, from the block holding diagonal 0
+    {
+        constexpr int mo1 = pos_mod(1 - DO, NM), ie1 = pos_mod(1 - DE, NI);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int gb = b * NW + w, kb0 = kbase + 64 * gb;
+            if (gb < nblk && kb0 <= 0 && kb0 + 63 >= 0) {
+                quiet[b] = 0;
+                if (lane == 63) ER[(NBLK + 2) + gb + 1] = max(Mr[b][mo1], Ir[b][ie1]);
+                if (lane == 0) EL[(NBLK + 2) + gb + 1] = max(Mr[b][mo1], Dr[b][ie1]);
+            }
+        }
+    }
+    __syncthreads();
+    int Rs = 0, Ps = me / p.e, seen = -1;
+    for (int base = 0;; base += P) {
+#pragma unroll
+        for (int ph = 1; ph <= P; ++ph) {
+            const int si = base + ph;
+            const int s = si * g;
+            if (s > me) return seen;
+            while ((Rs + 1) * p.e + p.o <= s) ++Rs;
+            while (Ps * p.e > me - s) --Ps;
+            // the alive diagonals of this score
+            const int lo_s = max(max(-rlen, -Rs), target - Ps);
+            const int hi_s = min(min(qlen, Rs), target + Ps);
+            const int ms = ph % NM, is = ph % NI;
+            const int mx = pos_mod(ph - DX, NM);
+            const int mo = pos_mod(ph - DO, NM), ie = pos_mod(ph - DE, NI);
+            const int mo2 = pos_mod(ph + 1 - DO, NM), ie2 = pos_mod(ph + 1 - DE, NI);
+            const int cur = (si & 1) * (NBLK + 2), nxt = (NBLK + 2) - cur;
+            int found = 0;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int gb = b * NW + w, kb0 = kbase + 64 * gb;
+                // a block that left the alive range less than NM scores ago
+                // runs with every lane held invalid
+                const bool alive = kb0 <= hi_s && kb0 + 63 >= lo_s;
+                quiet[b] = alive ? 0 : min(quiet[b] + 1, NM + 1);
+                if (gb < nblk && quiet[b] <= NM) {
+                    const int ti_in = ER[cur + gb];      // block gb - 1's lane 63
+                    const int td_in = EL[cur + gb + 2];  // block gb + 1's lane 0
+                    // each lane's max(M_o, I_e) and max(M_o, D_e), shifted
+                    // one lane up and one lane down
+                    const int ti = max(Mr[b][mo], Ir[b][ie]), td = max(Mr[b][mo], Dr[b][ie]);
+                    int iv = __builtin_amdgcn_update_dpp(ti_in, ti, 0x138, 0xf, 0xf, false) + 1;
+                    int dv = __builtin_amdgcn_update_dpp(td_in, td, 0x130, 0xf, 0xf, false);
+                    int mv = max(max(Mr[b][mx] + 1, iv), dv);
+                    const int k = kb0 + lane;
+                    if (kb0 < lo_s || kb0 + 63 > hi_s) {  // a block across the range's ends
+                        if (k < lo_s || k > hi_s) iv = dv = mv = kNeg;
+                    }
+                    // EXTEND: the first 32 bases without a branch, the rest
+                    // for the few lanes that matched all of them
+                    const int row = mv, col = row - k;
+                    uint32_t a0, a1, b0, b1;
+                    window_lsb(qs, med3(row, 0, qlen), a0, a1);
+                    window_lsb(rs, med3(col, 0, rlen), b0, b1);
+                    const int lim = min(qlen - row, rlen - col);
+                    const int n = med3(match32(a0, a1, b0, b1), lim, 0);
+                    int mr = row >= 0 ? row + n : kNeg;
+                    const bool longer = row >= 0 && n == 32 && lim > 32;
+                    if (__ballot(longer))
+                        if (longer) mr = extend_lsb(qs, qlen, rs, rlen, row + 32, col + 32);
+                    Ir[b][is] = iv;
+                    Dr[b][is] = dv;
+                    Mr[b][ms] = mr;
+                    if (kb0 <= target && target <= kb0 + 63)
+                        found |= (k == target && mr >= qlen);
+                    // this block's edge values for the next step
+                    if (lane == 63) ER[nxt + gb + 1] = max(Mr[b][mo2], Ir[b][ie2]);
+                    if (lane == 0) EL[nxt + gb + 1] = max(Mr[b][mo2], Dr[b][ie2]);
+                }
+            }
+            if (NW == 1) {
+                // one wavefront: LDS accesses of a wavefront are ordered, no
+                // barrier; the pair ends as soon as its target lane reaches
+                // the query end
+                if (__ballot(found)) return s;
+            } else {
+                // one barrier per step; the flag of step si is acted on after
+                // the next step's barrier (one extra step, harmless)
+                if (found) done[si & 1] = s;
+                __syncthreads();
+                if (seen >= 0) return seen;
+                seen = done[si & 1];
+            }
+        }
+    }
+}
+
+}  // namespace wfareg
+}  // namespace cgm

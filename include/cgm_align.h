@@ -52,11 +52,14 @@ extern "C" {
 /* Kernel layouts for the launch entry points. */
 #define CGM_LAYOUT_AUTO 0   /* SW: persistent task-queue kernel, work-sorted, one pair per
                                lane, on a plan the library keeps per (device, stream);
-                               WFA: as CGM_LAYOUT_WAVE */
+                               WFA: register layout (wavefronts in VGPRs, one diagonal per
+                               lane and block) where the penalties are x : o+e : e =
+                               2 : 4 : 1 and the pair's diagonals fit, else as
+                               CGM_LAYOUT_WAVE; longest pairs first */
 #define CGM_LAYOUT_PLAIN 1  /* one workgroup per pair (baseline) */
 #define CGM_LAYOUT_LANE 2   /* SW: one pair per lane, one kernel per band bucket */
-#define CGM_LAYOUT_WAVE 3   /* WFA: persistent kernel, one four-wavefront workgroup per pair
-                               (LDS, or tiled over an HBM slab), longest pairs first */
+#define CGM_LAYOUT_WAVE 3   /* WFA: team layouts only: one workgroup per pair, wavefronts in
+                               LDS (or tiled over an HBM slab), longest pairs first */
 
 /* ------------------------------------------------------------ errors, info */
 

@@ -126,3 +126,55 @@ def test_wfa_large_pairs_tiled_and_slab(cgm, dev, orc, gen_mod):
         dev.synchronize()
         assert_same(dev.download(out.scores, w.n_pairs), exp, f"large mixed lds={lds}")
         plan.close()
+
+
+def _layout_class_pairs(seed):
+    """Pairs whose max_error spreads them over every layout class: register
+    layouts of 128 to 8192 diagonals, the team layouts beyond."""
+    rng = np.random.default_rng(seed)
+    pairs, mes = [], []
+    for i, (L, me) in enumerate([(L, me) for L in (60, 400, 1500, 4000, 9000)
+                                 for me in (8, 40, 120, 250, 600, 1500, 4000, 9000)] * 3):
+        a = "".join(rng.choice(list("ACGT"), L))
+        b = "".join(c if rng.random() > 0.03 else rng.choice(list("ACGT")) for c in a
+                    if rng.random() > 0.01)
+        if i % 3 == 1:
+            b = b[: len(b) * 2 // 3]
+        pairs.append((a, b) if i % 2 else (b, a))
+        mes.append(me)
+    return pairs, mes
+
+
+@pytest.mark.parametrize("pen", [dict(x=4, o=6, e=2), dict(x=8, o=12, e=4), dict(x=3, o=5, e=2)])
+def test_wfa_every_layout_class(cgm, dev, orc, pen):
+    """The automatic layout (register classes where the penalties are
+    2 : 4 : 1 multiples, team layouts otherwise) and the team-only layout
+    equal the oracle on pairs spread over every class."""
+    pairs, mes = _layout_class_pairs(11)
+    w = workload_from_strings(pairs, max_error=mes)
+    up = Uploaded(dev, w)
+    exp = wfa_oracle(orc, w, pen=pen)
+    for layout in ("auto", "wave"):
+        print(f"[case] wfa every layout class {pen} layout={layout}", flush=True)
+        assert_same(wfa_gpu(cgm, dev, up, LAYOUTS[layout], pen=pen), exp,
+                    f"wfa classes {pen} {layout}")
+
+
+@pytest.mark.parametrize("pen", [dict(x=4, o=6, e=2), dict(x=8, o=12, e=4)])
+def test_wfa_max_error_at_the_cost(cgm, dev, orc, pen):
+    """max_error one below, at and just above each pair's cost, where the
+    register layout's max_error pruning removes most cells and whole
+    64-diagonal blocks leave the alive range; also length differences whose
+    gap cost alone exceeds max_error."""
+    pairs, _ = _layout_class_pairs(17)
+    w = workload_from_strings(pairs, max_error=[20000] * len(pairs))
+    cost = wfa_oracle(orc, w, pen=pen)
+    assert (cost >= 0).all()
+    for delta in (-1, 0, 1, pen["e"] + 1):
+        mes = [max(int(c) + delta, 0) for c in cost]
+        wt = workload_from_strings(pairs, max_error=mes)
+        print(f"[case] wfa max_error = cost + {delta} {pen}", flush=True)
+        got = wfa_gpu(cgm, dev, Uploaded(dev, wt), LAYOUTS["auto"], pen=pen)
+        exp = wfa_oracle(orc, wt, pen=pen)
+        assert_same(got, exp, f"wfa max_error = cost + {delta} {pen}")
+        assert ((got == cost) == ((delta >= 0) | (cost == 0))).all()
