@@ -1,5 +1,8 @@
 """Benchmark: banded SW GCUPS on the batched workload (200,000 pairs,
-100-300 bp, band 16-64 per pair, scores 2/-4/-4/-2) on one MI355X.
+100-300 bp, band 16-64 per pair, scores 2/-4/-4/-2) on one MI355X, plus the
+other workloads of the path as extras (plain SW on 1,000 x 150 bp, WFA on
+20,000 x 1-10 kb, the mixed batch, serving, ingest, CIGAR, NW), each with
+its roofline fraction and a CPU baseline where it has one.
 
 A step is one pass of the alignment path over the whole batch, with the
 packed sequences already resident in HBM.  Prints ONE JSON line:
@@ -21,6 +24,7 @@ import argparse
 import importlib.util
 import json
 import os
+import re
 import sys
 import time
 
@@ -32,11 +36,52 @@ PKG_DIR = os.path.join(ROOT, "computational-genomics-mojo_amd")
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # int32 lane-ops/s, MI355X spec
 HBM_PEAK_GBS = 8000.0
 SW_OPS_PER_CELL = 11
-# HBM bytes per launch of the persistent kernel on this workload, from the
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (128439.5 KiB x 2.0 + 38114.8 KiB
-# x 1.0, the scales measured on a 256 MiB device copy).
-TRAFFIC_BYTES = int((128439.5 * 2.0 + 38114.8) * 1024)
+WFA_OPS_PER_CELL = 8
+# HBM bytes per launch of the persistent kernel on this workload, measured
+# by rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the
+# counter reads on a 256 MiB device copy) and read from the summary that
+# holds those passes and their commands (its "traffic_bytes:" line).
+TRAFFIC_SOURCE = "profiles/sw_persistent_200k_100to300bp_rocprof.md"
+
+
+def traffic_from_profile(path=TRAFFIC_SOURCE):
+    try:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), path)) as f:
+            m = re.search(r"traffic_bytes:\s*(\d+)", f.read())
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
+
+
 SW = dict(match=2, mismatch=-4, gap_open=-4, gap_extend=-2)
+WFA_PEN = dict(x=4, o=6, e=2)
+# the host's CPU share for one GPU: at most this many worker threads
+CPU_THREADS_CAP = 16
+
+
+def host_info():
+    """nproc, the threads this process may use, the CPU model."""
+    model = ""
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                     if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "allowed_cpus": allowed, "cpu_model": model,
+            "threads_used": max(1, min(CPU_THREADS_CAP, allowed))}
+
+
+def load_oracle():
+    """The CPU oracle (the checker; here the timed CPU baseline), its tuned
+    build where the host supports it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    flags = oracle.select_fast_build()
+    return oracle, flags
 
 
 def load_package():
@@ -82,7 +127,7 @@ class SwRunner:
                 d.download(self.out.ref_end, self.n))
 
 
-def wfa_extra(m, dev, gen, n_pairs=20_000):
+def wfa_extra(m, dev, gen, n_pairs=20_000, cpu=True):
     """Gap-affine WFA on 20,000 pairs of 1-10 kb (x=4, o=6, e=2): pairs/s of
     the mixed persistent layout on an autotuned plan, of the automatic
     launcher (the WFA persistent workgroup-per-pair kernel) and of the plain
@@ -106,7 +151,13 @@ def wfa_extra(m, dev, gen, n_pairs=20_000):
     res["persistent"] = {"pairs_per_s": n * 3 / wall, "ms_per_step": wall / 3 * 1e3,
                          "tuning": tuning, "autotune_start_ms": start_ms,
                          "autotune_best_ms": best_ms}
-    log(f"[bench] wfa persistent (autotuned): {res['persistent']}")
+    # the WFA kernels' own span, from the plan's events
+    wfa_ms = []
+    for _ in range(3):
+        plan.align(dev, bufs, n, up["kind"], up["band"], up["max_error"], cfg, pout)
+        wfa_ms.append(plan.wfa_kernel_ms())
+    kms_wfa = float(np.median(wfa_ms))
+    log(f"[bench] wfa persistent (autotuned): {res['persistent']}, WFA kernels {kms_wfa:.3f} ms")
     plan.close()
 
     def launch(layout):
@@ -125,10 +176,79 @@ def wfa_extra(m, dev, gen, n_pairs=20_000):
     res["speedup_auto_vs_plain"] = res["auto"]["pairs_per_s"] / res["plain"]["pairs_per_s"]
     res["mean_cost"] = float(costs["plain"].mean())
     res["pairs"] = w.n_pairs
+    # roofline of the WFA kernels: 8 int ops per algorithmic wavefront cell
+    cells = gen.wfa_wavefront_cells(w.q_len, w.r_len, costs["plain"], w.max_error)
+    total = int(cells.sum())
+    achieved = WFA_OPS_PER_CELL * total / (kms_wfa * 1e-3) / 1e12
+    in_bytes = int(w.q_packed.nbytes + w.r_packed.nbytes + 5 * 4 * n)
+    res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
+                       "unit": "TOP/s (int32)", "frac": achieved / VALU_PEAK_TOPS,
+                       "traffic": None, "kernel": "WFA class kernels (plan events)",
+                       "kernel_ms": kms_wfa, "cells": total, "ops_per_cell": WFA_OPS_PER_CELL,
+                       "hbm_algorithmic_bytes": in_bytes + 4 * n,
+                       "hbm_frac": (in_bytes + 4 * n) / (kms_wfa * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"[bench] wfa roofline: {res['roofline']}")
+    if cpu:
+        res["cpu_baseline"] = wfa_cpu_baseline(w, costs["plain"])
     return res
 
 
-def mixed_extra(m, dev, gen, n_pairs=100_000):
+def wfa_cpu_baseline(w, gpu_costs, n_multi=6000, n_single=400):
+    """or_wfa_affine (the C restatement of align_cpu.mojo:75-226) on this
+    host: a random sample of the workload on the host's threads, and a smaller
+    one on 1 thread; pairs/s of each, checked against the GPU costs."""
+    oracle, flags = load_oracle()
+    info = host_info()
+    rng = np.random.default_rng(5)
+    res = {"unit": "pairs/s", "kind": "port", "oracle_build": flags}
+    for name, k, thr in (("multi", n_multi, info["threads_used"]), ("single", n_single, 1)):
+        idx = np.sort(rng.choice(w.n_pairs, min(k, w.n_pairs), replace=False))
+        sub = w.subset(idx)
+        t = time.perf_counter()
+        c = oracle.batch(1, sub.q_packed, sub.r_packed, sub.q_off, sub.r_off, sub.q_len,
+                         sub.r_len, max_error=sub.max_error, threads=thr, **WFA_PEN)[0]
+        sec = time.perf_counter() - t
+        res[name] = {"pairs_per_s": len(idx) / sec, "threads": thr, "seconds": sec,
+                     "sample": f"{len(idx)} random pairs of the workload",
+                     "bit_exact_vs_gpu": bool(np.array_equal(c, gpu_costs[idx]))}
+    res["value"] = res["multi"]["pairs_per_s"]
+    res["cores"] = res["multi"]["threads"]
+    res["sample"] = res["multi"]["sample"]
+    log(f"[bench] wfa cpu: {res}")
+    return res
+
+
+def sw_150bp_extra(m, dev, gen, cpu, steps=20):
+    """Plain banded SW (the CPU reference's own case): 1,000 pairs of 150 bp,
+    band 32, 2/-4/-4/-2: GCUPS of the one-workgroup-per-pair kernel and of
+    the automatic layout, and the oracle on 1 thread over all pairs."""
+    w = gen.generate(gen.SW_150BP, 1000)
+    run = SwRunner(m, dev, w)
+    cells = int(gen.valid_band_cells(w.q_len, w.r_len, w.band).sum())
+    res = {"pairs": w.n_pairs, "valid_cells": cells}
+    got = {}
+    for name, layout in (("plain", 1), ("auto", 0)):
+        wall, kms = time_steps(m, dev, lambda: run.launch(layout), steps, 3)
+        got[name] = run.results()
+        res[name] = {"gcups": cells * steps / wall / 1e9, "ms_per_step": wall / steps * 1e3,
+                     "pairs_per_s": w.n_pairs * steps / wall}
+    res["identical"] = all(bool(np.array_equal(a, b)) for a, b in zip(got["plain"], got["auto"]))
+    if cpu:
+        oracle, flags = load_oracle()
+        t = time.perf_counter()
+        c = oracle.batch(0, w.q_packed, w.r_packed, w.q_off, w.r_off, w.q_len, w.r_len,
+                         band=w.band, threads=1, **SW)
+        sec = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": cells / sec / 1e9, "unit": "GCUPS", "cores": 1,
+                               "kind": "port", "oracle_build": flags, "seconds": sec,
+                               "sample": "all 1000 pairs, 1 thread",
+                               "bit_exact_vs_gpu": all(bool(np.array_equal(a, b))
+                                                       for a, b in zip(c, got["plain"]))}
+    log(f"[bench] sw 1k x 150 bp: {res}")
+    return res
+
+
+def mixed_extra(m, dev, gen, n_pairs=100_000, cpu=True):
     """Mixed batch (50% SW 100-300 bp band 32, 40% SW 1-2 kb band 128, 10%
     WFA 5-20 kb): pairs/s of the persistent task-queue kernel on an
     autotuned plan replayed from a hipGraph, of the same enqueued eagerly,
@@ -153,7 +273,32 @@ def mixed_extra(m, dev, gen, n_pairs=100_000):
     res["persistent_graph"] = {"pairs_per_s": n * 3 / wall, "ms_per_step": wall / 3 * 1e3}
     wall, kms = time_steps(m, dev, lambda: plan.align(dev, bufs, n, kind, band, me, cfg, out), 2, 0)
     res["persistent_eager"] = {"pairs_per_s": n * 2 / wall, "ms_per_step": wall / 2 * 1e3}
+    sw_ms, wfa_ms = [], []
+    for _ in range(2):
+        plan.align(dev, bufs, n, kind, band, me, cfg, out)
+        sw_ms.append(plan.kernel_ms())
+        wfa_ms.append(plan.wfa_kernel_ms())
     log(f"[bench] mixed persistent: {res['persistent_graph']} eager {res['persistent_eager']}")
+    # algorithmic work: SW valid band cells x 11 ops + WFA wavefront cells x 8
+    is_wfa = w.kind == 1
+    sw_cells = int(gen.valid_band_cells(w.q_len[~is_wfa], w.r_len[~is_wfa],
+                                        w.band[~is_wfa]).sum())
+    wfa_cells = int(gen.wfa_wavefront_cells(w.q_len[is_wfa], w.r_len[is_wfa],
+                                            graph_res[0][is_wfa], w.max_error[is_wfa]).sum())
+    step_ms = res["persistent_graph"]["ms_per_step"]
+    ops = SW_OPS_PER_CELL * sw_cells + WFA_OPS_PER_CELL * wfa_cells
+    res["roofline"] = {"bound": "valu", "achieved": ops / (step_ms * 1e-3) / 1e12,
+                       "peak": VALU_PEAK_TOPS, "unit": "TOP/s (int32)",
+                       "frac": ops / (step_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                       "over": "the whole captured step (graph replay)", "traffic": None,
+                       "sw_cells": sw_cells, "wfa_cells": wfa_cells,
+                       "sw_kernel_ms": float(np.median(sw_ms)),
+                       "wfa_kernels_ms": float(np.median(wfa_ms)),
+                       "sw_kernel_frac": SW_OPS_PER_CELL * sw_cells /
+                       (float(np.median(sw_ms)) * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                       "wfa_kernels_frac": WFA_OPS_PER_CELL * wfa_cells /
+                       (float(np.median(wfa_ms)) * 1e-3) / 1e12 / VALU_PEAK_TOPS}
+    log(f"[bench] mixed roofline: {res['roofline']}")
     wall, kms = time_steps(m, dev, lambda: m.align_mixed_plain(dev, bufs, n, kind, band, me, cfg,
                                                                out), 1, 0)
     plain_res = [dev.download(a, n) for a in (out.scores, out.query_end, out.ref_end)]
@@ -163,6 +308,23 @@ def mixed_extra(m, dev, gen, n_pairs=100_000):
         res["plain_wg_per_pair"]["pairs_per_s"]
     log(f"[bench] mixed plain: {res['plain_wg_per_pair']} speedup {res['speedup_vs_plain']:.2f}")
     plan.close()
+    if cpu:
+        oracle, flags = load_oracle()
+        thr = host_info()["threads_used"]
+        idx = np.sort(np.random.default_rng(9).choice(n, 1500, replace=False))
+        sub = w.subset(idx)
+        t = time.perf_counter()
+        c = oracle.batch(0, sub.q_packed, sub.r_packed, sub.q_off, sub.r_off, sub.q_len,
+                         sub.r_len, kind=sub.kind, band=sub.band, max_error=sub.max_error,
+                         threads=thr, **SW, **WFA_PEN)
+        sec = time.perf_counter() - t
+        res["cpu_baseline"] = {
+            "value": len(idx) / sec, "unit": "pairs/s", "cores": thr, "kind": "port",
+            "oracle_build": flags, "seconds": sec,
+            "sample": f"{len(idx)} random pairs of the mixed workload ({int(sub.kind.sum())} WFA)",
+            "bit_exact_vs_gpu": all(bool(np.array_equal(a, b[idx]))
+                                    for a, b in zip(c, graph_res))}
+        log(f"[bench] mixed cpu: {res['cpu_baseline']}")
     return res
 
 
@@ -353,7 +515,8 @@ def main():
     from cgm_amd import gen
     dev = m.GenomicsDevice(0)
     info = dev.info()
-    log(f"[bench] device {info}")
+    host = host_info()
+    log(f"[bench] device {info} host {host}")
     t = time.perf_counter()
     w = gen.generate(gen.SW_100_300BP, args.pairs)
     cells = gen.valid_band_cells(w.q_len, w.r_len, w.band)
@@ -401,8 +564,9 @@ def main():
 
     extras = {}
     if not args.no_extras:
-        extras["wfa_20k_1to10kb"] = wfa_extra(m, dev, gen)
-        extras["mixed_100k"] = mixed_extra(m, dev, gen)
+        extras["sw_1k_150bp"] = sw_150bp_extra(m, dev, gen, not args.no_cpu)
+        extras["wfa_20k_1to10kb"] = wfa_extra(m, dev, gen, cpu=not args.no_cpu)
+        extras["mixed_100k"] = mixed_extra(m, dev, gen, cpu=not args.no_cpu)
         extras["serving_1to64_pairs"] = serving_extra(m)
         extras["ingest_200k_pairs"] = ingest_extra(m, dev, gen)
         extras["wfa_cigar_20k_1to10kb"] = cigar_extra(m, dev, gen)
@@ -420,9 +584,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle  # the checker / CPU baseline
-        threads = max(1, min(16, os.cpu_count() or 1))
+        oracle, oflags = load_oracle()  # the checker, here the timed CPU baseline
+        threads = host["threads_used"]
         t = time.perf_counter()
         cres = oracle.batch(0, w.q_packed, w.r_packed, w.q_off, w.r_off, w.q_len, w.r_len,
                             band=w.band, threads=threads, **SW)
@@ -435,7 +598,7 @@ def main():
                      band=sub.band, threads=1, **SW)
         cpu1_s = time.perf_counter() - t
         cpu = {"value": total_cells / cpu_s / 1e9, "unit": "GCUPS", "cores": threads,
-               "kind": "port",
+               "kind": "port", "oracle_build": oflags, "host": host,
                "sample": f"all {w.n_pairs} pairs of the workload, {threads} threads",
                "single_thread_gcups": float(cells[:n1].sum() / cpu1_s / 1e9),
                "single_thread_sample": f"first {n1} pairs, 1 thread",
@@ -468,7 +631,8 @@ def main():
                      "unit": "TOP/s (packed int16)" if dual else "TOP/s (int32)",
                      "frac": achieved_tops / peak,
                      "frac_of_int32_peak": achieved_tops / VALU_PEAK_TOPS,
-                     "traffic": TRAFFIC_BYTES if plan is not None else None,
+                     "traffic": traffic_from_profile() if plan is not None else None,
+                     "traffic_source": TRAFFIC_SOURCE if plan is not None else None,
                      "kernel": "mix_persistent_kernel" if plan is not None else "step",
                      "kernel_ms": kmean, "step_ms": step_ms,
                      "ops_per_cell": SW_OPS_PER_CELL,
@@ -478,6 +642,7 @@ def main():
         "band_area_gcups": band_area * args.steps / wall / 1e9,
         "pairs_per_s": w.n_pairs * args.steps / wall,
         "device": info["name"],
+        "host": host,
         "extras": extras,
     }
     print(json.dumps(line), flush=True)

@@ -116,6 +116,28 @@ def valid_band_cells(q_len, r_len, band) -> np.ndarray:
     return np.where((n > 0) & (r > 0), total, 0)
 
 
+def wfa_wavefront_cells(q_len, r_len, cost, max_error, o=6, e=2) -> np.ndarray:
+    """Algorithmic WFA work per pair: the reachable wavefront cells
+    sum_{s=1}^{c} n(s), n(s) = 1 for s < o + e, else
+    min(qlen, m_s, me) + min(rlen, m_s, me) + 1 with m_s = (s - o) // e, over
+    the scores s = 1 .. c that wfa_affine_cpu (align_cpu.mojo:148-224)
+    evaluates: c = the cost, or max_error when the result is -1."""
+    q = np.asarray(q_len, np.int64)
+    r = np.asarray(r_len, np.int64)
+    me = np.asarray(max_error, np.int64)
+    c = np.asarray(cost, np.int64)
+    c = np.where(c < 0, me, c)
+    total = np.zeros(q.shape, np.int64)
+    for s in range(1, int(c.max(initial=0)) + 1):
+        if s < o + e:
+            n = np.ones_like(q)
+        else:
+            ms = (s - o) // e
+            n = np.minimum(np.minimum(q, ms), me) + np.minimum(np.minimum(r, ms), me) + 1
+        total += np.where(s <= c, n, 0)
+    return total
+
+
 def generate(workload: int, n_pairs: int | None = None, seed: int = DEFAULT_SEED) -> Workload:
     n = DEFAULT_PAIRS[workload] if n_pairs is None else int(n_pairs)
     h = ctypes.c_void_p()

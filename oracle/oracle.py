@@ -21,7 +21,28 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libcgm_oracle.so")
+_FAST_PATH = os.path.join(_HERE, "libcgm_oracle_fast.so")
 _lib = None
+
+
+def host_has_v3() -> bool:
+    """Whether this host's CPU runs x86-64-v3 code (AVX2, BMI2, FMA)."""
+    try:
+        flags = next(l for l in open("/proc/cpuinfo") if l.startswith("flags")).split()
+    except (OSError, StopIteration):
+        return False
+    return all(f in flags for f in ("avx2", "bmi2", "fma", "movbe"))
+
+
+def select_fast_build() -> str:
+    """Load the -O3 -march=x86-64-v3 build instead of the portable one when
+    the host supports it (call before the first use); returns the flags of
+    the build that will be used.  For timing the CPU baseline only."""
+    global _LIB_PATH
+    if _lib is None and host_has_v3() and os.path.exists(_FAST_PATH):
+        _LIB_PATH = _FAST_PATH
+        return "-O3 -march=x86-64-v3"
+    return "-O3 -march=x86-64-v2"
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32
