@@ -169,12 +169,11 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
     const int nq = 2 * qw + 4, nr = 2 * rw + 4;
     uint32_t *qs = (uint32_t *)lds;
     uint32_t *rs = qs + nq;
-    // edge buffers: ER[buf][gb + 1] = max(M_o, I_e) of block gb's lane 63,
-    // EL[buf][gb + 1] = max(M_o, D_e) of its lane 0; entries 0 and nblk + 1
-    // are the outer edges and stay invalid
-    int *ER = (int *)(rs + nr);
-    int *EL = ER + 2 * (NBLK + 2);
-    int *done = EL + 2 * (NBLK + 2);
+    // edge buffers, two by step parity: E[buf][gb + 1] = {max(M_o, I_e) of
+    // block gb's lane 63, max(M_o, D_e) of its lane 0}; entries 0 and
+    // nblk + 1 are the outer edges and stay invalid
+    int2 *E = (int2 *)(rs + nr);
+    int *done = (int *)(E + 2 * (NBLK + 2));
     for (int i = tid; i < nq + nr; i += NW * 64) {
         const bool isq = i < nq;
         const int j = isq ? i : i - nq;
@@ -185,7 +184,7 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
         }
         qs[i] = v;
     }
-    for (int i = tid; i < 4 * (NBLK + 2); i += NW * 64) ER[i] = kNeg;  // ER and EL
+    for (int i = tid; i < 2 * (NBLK + 2); i += NW * 64) E[i] = make_int2(kNeg, kNeg);
     if (tid == 0) done[0] = done[1] = -1;
     __syncthreads();
 
@@ -194,34 +193,28 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
     int kbase, kmax;
     reg_span(qlen, rlen, me, p.o, p.e, g, kbase, kmax);
     const int nblk = (kmax - kbase + 64) / 64;
+    // the block and lane of the target diagonal
+    const int tb = (target - kbase) >> 6, tl = (target - kbase) & 63;
 
     int Mr[NB][NM], Ir[NB][NI], Dr[NB][NI];
-    // quiet[b]: scores block b has been wholly outside the alive range, up to
-    // NM + 1; for the first NM of them it still runs, every lane held
-    // invalid, so that its slots and published edges hold the invalid value,
-    // and from then on it skips
-    int quiet[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
 #pragma unroll
         for (int j = 0; j < NM; ++j) Mr[b][j] = kNeg;
 #pragma unroll
         for (int j = 0; j < NI; ++j) Ir[b][j] = Dr[b][j] = kNeg;
-        quiet[b] = NM + 1;
     }
     // score 0: M[0, 0] = the matching prefix, in the lane owning diagonal 0
-    {
-        const int g0 = -kbase / 64, l0 = -kbase % 64;
+    const int g0 = -kbase >> 6, l0 = -kbase & 63;
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
-            if (b * NW + w == g0 && lane == l0) Mr[b][0] = extend_lsb(qs, qlen, rs, rlen, 0, 0);
-    }
+    for (int b = 0; b < NB; ++b)
+        if (b * NW + w == g0 && lane == l0) Mr[b][0] = extend_lsb(qs, qlen, rs, rlen, 0, 0);
     // score 0 alone reaches the target only on diagonal 0
     if (target == 0) {
         int m0 = kNeg;
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-            if (b * NW + w == -kbase / 64 && lane == -kbase % 64) m0 = Mr[b][0];
+            if (b * NW + w == g0 && lane == l0) m0 = Mr[b][0];
         if (__ballot(m0 >= qlen)) done[0] = 0;
         __syncthreads();
         if (done[0] == 0) return 0;
@@ -231,27 +224,52 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
         constexpr int mo1 = pos_mod(1 - DO, NM), ie1 = pos_mod(1 - DE, NI);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            const int gb = b * NW + w, kb0 = kbase + 64 * gb;
-            if (gb < nblk && kb0 <= 0 && kb0 + 63 >= 0) {
-                quiet[b] = 0;
-                if (lane == 63) ER[(NBLK + 2) + gb + 1] = max(Mr[b][mo1], Ir[b][ie1]);
-                if (lane == 0) EL[(NBLK + 2) + gb + 1] = max(Mr[b][mo1], Dr[b][ie1]);
+            if (b * NW + w == g0) {
+                const int ei = __builtin_amdgcn_readlane(max(Mr[b][mo1], Ir[b][ie1]), 63);
+                const int ed = max(Mr[b][mo1], Dr[b][ie1]);
+                if (lane == 0) E[(NBLK + 2) + g0 + 1] = make_int2(ei, ed);
             }
         }
     }
     __syncthreads();
-    int Rs = 0, Ps = me / p.e, seen = -1;
+    // with o + e = DO g and e = DE g exactly, every per-score quantity is a
+    // function of si = s / g: R(s) = (si - DO + DE) / DE from si = DO on,
+    // P(s) = (mg - si) / DE with mg = me / g, and s <= me iff si <= mg
+    const int mg = me / g;
+    // blocks are gb = b NW + w: the first b of this wavefront at or after
+    // block index x is (x - w + NW - 1) >> LNW, the last at or before it
+    // (x - w) >> LNW (arithmetic shifts: floor and ceiling of x / NW)
+    constexpr int LNW = NW == 1 ? 0 : NW == 2 ? 1 : NW == 4 ? 2 : NW == 8 ? 3 : 4;
+    static_assert((1 << LNW) == NW, "NW: 1, 2, 4, 8 or 16");
+    // this wavefront's b of the target diagonal's block, or -1
+    const int tbb = ((tb - w) & (NW - 1)) == 0 && tb >= 0 ? (tb - w) >> LNW : -1;
+    int seen = -1;
     for (int base = 0;; base += P) {
 #pragma unroll
         for (int ph = 1; ph <= P; ++ph) {
             const int si = base + ph;
+            if (si > mg) return seen;
             const int s = si * g;
-            if (s > me) return seen;
-            while ((Rs + 1) * p.e + p.o <= s) ++Rs;
-            while (Ps * p.e > me - s) --Ps;
+            const int Rs = si >= DO ? (si - DO + DE) / DE : 0;
+            const int Ps = (mg - si) / DE;
+            const int Pl = (mg - max(si - NM, 0)) / DE;  // P(s - NM g), P(0) before it
             // the alive diagonals of this score
             const int lo_s = max(max(-rlen, -Rs), target - Ps);
             const int hi_s = min(min(qlen, Rs), target + Ps);
+            // the blocks that run: every block that may hold a cell alive at
+            // one of the scores s - NM g .. s (so a block that leaves the
+            // alive range writes the invalid value for NM more scores and
+            // publishes edges from its older slots meanwhile): this
+            // wavefront's b in [b_lo, b_lo + nrun)
+            const int run_lo = max(max(-rlen, -Rs), target - Pl);
+            const int run_hi = min(min(qlen, Rs), target + Pl);
+            const int b_lo = (((run_lo - kbase) >> 6) - w + NW - 1) >> LNW;
+            const int b_hi = ((min((run_hi - kbase) >> 6, nblk - 1)) - w) >> LNW;
+            const unsigned nrun = (unsigned)max(b_hi - b_lo + 1, 0);
+            // the blocks wholly inside [lo_s, hi_s]: b in [f_lo, f_lo + nfull)
+            const int f_lo = ((((lo_s - kbase + 63) >> 6)) - w + NW - 1) >> LNW;
+            const int f_hi = (((hi_s - kbase - 63) >> 6) - w) >> LNW;
+            const unsigned nfull = (unsigned)max(f_hi - f_lo + 1, 0);
             const int ms = ph % NM, is = ph % NI;
             const int mx = pos_mod(ph - DX, NM);
             const int mo = pos_mod(ph - DO, NM), ie = pos_mod(ph - DE, NI);
@@ -261,13 +279,9 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int gb = b * NW + w, kb0 = kbase + 64 * gb;
-                // a block that left the alive range less than NM scores ago
-                // runs with every lane held invalid
-                const bool alive = kb0 <= hi_s && kb0 + 63 >= lo_s;
-                quiet[b] = alive ? 0 : min(quiet[b] + 1, NM + 1);
-                if (gb < nblk && quiet[b] <= NM) {
-                    const int ti_in = ER[cur + gb];      // block gb - 1's lane 63
-                    const int td_in = EL[cur + gb + 2];  // block gb + 1's lane 0
+                if ((unsigned)(b - b_lo) < nrun) {
+                    // block gb - 1's lane 63 and block gb + 1's lane 0
+                    const int ti_in = E[cur + gb].x, td_in = E[cur + gb + 2].y;
                     // each lane's max(M_o, I_e) and max(M_o, D_e), shifted
                     // one lane up and one lane down
                     const int ti = max(Mr[b][mo], Ir[b][ie]), td = max(Mr[b][mo], Dr[b][ie]);
@@ -275,7 +289,7 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
                     int dv = __builtin_amdgcn_update_dpp(td_in, td, 0x130, 0xf, 0xf, false);
                     int mv = max(max(Mr[b][mx] + 1, iv), dv);
                     const int k = kb0 + lane;
-                    if (kb0 < lo_s || kb0 + 63 > hi_s) {  // a block across the range's ends
+                    if ((unsigned)(b - f_lo) >= nfull) {  // a block across the range's ends
                         if (k < lo_s || k > hi_s) iv = dv = mv = kNeg;
                     }
                     // EXTEND: the first 32 bases without a branch, the rest
@@ -287,28 +301,27 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
                     const int lim = min(qlen - row, rlen - col);
                     const int n = med3(match32(a0, a1, b0, b1), lim, 0);
                     int mr = row >= 0 ? row + n : kNeg;
-                    const bool longer = row >= 0 && n == 32 && lim > 32;
-                    if (__ballot(longer))
-                        if (longer) mr = extend_lsb(qs, qlen, rs, rlen, row + 32, col + 32);
+                    if (row >= 0 && n == 32 && lim > 32)
+                        mr = extend_lsb(qs, qlen, rs, rlen, row + 32, col + 32);
                     Ir[b][is] = iv;
                     Dr[b][is] = dv;
                     Mr[b][ms] = mr;
-                    if (kb0 <= target && target <= kb0 + 63)
-                        found |= (k == target && mr >= qlen);
+                    if (b == tbb) found = __ballot(lane == tl && mr >= qlen) != 0;
                     // this block's edge values for the next step
-                    if (lane == 63) ER[nxt + gb + 1] = max(Mr[b][mo2], Ir[b][ie2]);
-                    if (lane == 0) EL[nxt + gb + 1] = max(Mr[b][mo2], Dr[b][ie2]);
+                    const int ei = __builtin_amdgcn_readlane(max(Mr[b][mo2], Ir[b][ie2]), 63);
+                    const int ed = max(Mr[b][mo2], Dr[b][ie2]);
+                    if (lane == 0) E[nxt + gb + 1] = make_int2(ei, ed);
                 }
             }
             if (NW == 1) {
                 // one wavefront: LDS accesses of a wavefront are ordered, no
                 // barrier; the pair ends as soon as its target lane reaches
                 // the query end
-                if (__ballot(found)) return s;
+                if (found) return s;
             } else {
                 // one barrier per step; the flag of step si is acted on after
                 // the next step's barrier (one extra step, harmless)
-                if (found) done[si & 1] = s;
+                if (found && lane == 0) done[si & 1] = s;
                 __syncthreads();
                 if (seen >= 0) return seen;
                 seen = done[si & 1];

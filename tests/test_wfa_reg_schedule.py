@@ -76,11 +76,12 @@ def span_bisect(ql, rl, x, o, e, me):
     return max(-rl, -side(-t)), min(ql, side(t))
 
 
-def wfa_reg_model(q, r, x, o, e, me, NB, NW, lanes=LANES, prune=True, flush=True):
+def wfa_reg_model(q, r, x, o, e, me, NB, NW, lanes=LANES, prune=True, lag=None):
     """Cost or -1 as the register layout computes it, or None when the
     pair's diagonals do not fit NB x NW blocks.  prune: cells whose
     remaining gap budget cannot reach the target diagonal (|k - t| >
-    (me - s) // e) are held invalid."""
+    (me - s) // e) are held invalid.  lag: scores a block that left the
+    alive range keeps running with every lane invalid (NM by default)."""
     ql, rl = len(q), len(r)
     if ql == 0 and rl == 0:
         return 0
@@ -122,11 +123,13 @@ def wfa_reg_model(q, r, x, o, e, me, NB, NW, lanes=LANES, prune=True, flush=True
     # EL = max(M_o, D_e) of lane 0, for the step they are published for
     ER = [[NEG] * nblk for _ in range(2)]
     EL = [[NEG] * nblk for _ in range(2)]
-    # quiet[gb]: consecutive steps the block has been entirely outside the
-    # alive range, counted up to NM: from NM on all its slots and published
-    # edge values are invalid and it skips the step
-    quiet = [NM] * nblk
-    limit = NM if flush else 0
+    # a block runs at score s when it may hold a cell alive at one of the
+    # scores s - NM g .. s (a superset: below max(-rlen, -R(s),
+    # t - P(s - NM g)) and above min(qlen, R(s), t + P(s - NM g)) no such
+    # cell exists), every lane outside the alive range of s held invalid; so
+    # a block that leaves the range still writes the invalid value for NM
+    # scores, and its published edges come from its older slots until then
+    lag = NM if lag is None else lag
 
     def alive_range(s, Rs):
         lo, hi = max(-rl, -Rs), min(ql, Rs)
@@ -135,9 +138,13 @@ def wfa_reg_model(q, r, x, o, e, me, NB, NW, lanes=LANES, prune=True, flush=True
             lo, hi = max(lo, target - P), min(hi, target + P)
         return lo, hi
 
-    def inside(gb, lo, hi):
-        k0 = kbase + lanes * gb
-        return k0 <= hi and k0 + lanes - 1 >= lo
+    def run_blocks(s, Rs):
+        if not prune:
+            lo, hi = alive_range(s, Rs)
+        else:
+            Pl = (me - max(s - lag * g, 0)) // e
+            lo, hi = max(-rl, -Rs, target - Pl), min(ql, Rs, target + Pl)
+        return (lo - kbase) // lanes, min((hi - kbase) // lanes, nblk - 1)
 
     def publish(gb, si_next):
         # a block's edge values for the next step, from its slots of the
@@ -146,11 +153,7 @@ def wfa_reg_model(q, r, x, o, e, me, NB, NW, lanes=LANES, prune=True, flush=True
         ER[si_next & 1][gb] = max(M[gb][lanes - 1][mo], I[gb][lanes - 1][ie])
         EL[si_next & 1][gb] = max(M[gb][0][mo], D[gb][0][ie])
 
-    lo0, hi0 = alive_range(0, 0)
-    for gb in range(nblk):   # after score 0: the blocks holding diagonal 0
-        if inside(gb, lo0, hi0):
-            quiet[gb] = 0
-            publish(gb, 1)
+    publish(b0, 1)   # after score 0: the block holding diagonal 0
     Rs = 0
     si = 0
     while (si + 1) * g <= me:
@@ -163,16 +166,10 @@ def wfa_reg_model(q, r, x, o, e, me, NB, NW, lanes=LANES, prune=True, flush=True
         found = False
         buf = si & 1
         lo_s, hi_s = alive_range(s, Rs)
+        blo, bhi = run_blocks(s, Rs)
         for gb in range(nblk):
-            if not inside(gb, lo_s, hi_s):
-                if quiet[gb] >= limit:
-                    continue
-                quiet[gb] += 1       # flush: this score's slots invalid
-                for l in range(lanes):
-                    M[gb][l][ms] = I[gb][l][is_] = D[gb][l][is_] = NEG
-                publish(gb, si + 1)
+            if gb < blo or gb > bhi:
                 continue
-            quiet[gb] = 0
             ti = [max(M[gb][l][mo], I[gb][l][ie]) for l in range(lanes)]
             td = [max(M[gb][l][mo], D[gb][l][ie]) for l in range(lanes)]
             ti_in = ER[buf][gb - 1] if gb > 0 else NEG
@@ -182,9 +179,9 @@ def wfa_reg_model(q, r, x, o, e, me, NB, NW, lanes=LANES, prune=True, flush=True
                 iv = (ti[l - 1] if l > 0 else ti_in) + 1
                 dv = td[l + 1] if l < lanes - 1 else td_in
                 mv = max(M[gb][l][mx] + 1, iv, dv)
-                mr = ext(mv, k) if mv >= 0 else mv
-                if k > kmax or k < lo_s or k > hi_s:  # held invalid
-                    iv, dv, mr = min(iv, NEG), min(dv, NEG), min(mr, NEG)
+                if k < lo_s or k > hi_s:  # held invalid
+                    iv = dv = mv = NEG
+                mr = ext(mv, k) if mv >= 0 else NEG
                 I[gb][l][is_] = iv
                 D[gb][l][is_] = dv
                 M[gb][l][ms] = mr
@@ -278,7 +275,7 @@ def test_pruning_shrinks_the_span():
 
 
 @pytest.mark.parametrize("mistake", ["edge_shift", "ring_back", "ghost_zero", "late_skip",
-                                     "over_prune", "no_flush"])
+                                     "over_prune", "no_flush", "short_flush"])
 def test_planted_layout_mistakes_fail(mistake):
     """Copies of the model with one slip each must differ from the oracle."""
     rng = random.Random(9)
@@ -294,7 +291,9 @@ def test_planted_layout_mistakes_fail(mistake):
         if mistake == "ghost_zero":     # the missing neighbour of the outer blocks read as row 0
             return _model_variant(q, r, x, o, e, me, ghost=0)
         if mistake == "no_flush":       # a block leaving the alive range skips at once
-            return wfa_reg_model(q, r, x, o, e, me, 8, 4, lanes=4, flush=False)
+            return wfa_reg_model(q, r, x, o, e, me, 8, 4, lanes=4, lag=0)
+        if mistake == "short_flush":    # ... or after one more score
+            return wfa_reg_model(q, r, x, o, e, me, 8, 4, lanes=4, lag=1)
         if mistake == "over_prune":     # the gap budget counted with one extension too few
             return wfa_reg_model(q, r, x, o, e, max(me - e, 0), 8, 4, lanes=4) if me >= e else None
         return _model_variant(q, r, x, o, e, me, skip_margin=1)  # late_skip
