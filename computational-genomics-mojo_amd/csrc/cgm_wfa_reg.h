@@ -144,6 +144,18 @@ __device__ __forceinline__ uint32_t wfa_class_key(int cls, uint32_t work) {
     return ((uint32_t)(kWfaClasses - 1 - cls) << 25) | (min(work, (uint32_t)((1 << 25) - 2)) + 1);
 }
 
+// Bases pos .. pos + 15 of an LSB-first sequence as one word (base pos + i
+// at bits 2i).
+__device__ __forceinline__ uint32_t window16_lsb(const uint32_t *s, int pos) {
+    const uint32_t *w = s + (pos >> 4);
+    return __builtin_amdgcn_alignbit(w[1], w[0], 2u * (uint32_t)(pos & 15));
+}
+
+// Leading bases (0..16) on which two 16-base windows agree.
+__device__ __forceinline__ int match16(uint32_t a, uint32_t b) {
+    return (int)(min(wfadev::lowest_bit(a ^ b), 32u) >> 1);
+}
+
 // Ring sizes: NI = DE + 1 I and D slots, and M slots rounded up from
 // max(DX, DO) + 1 to a multiple of NI, so that both rings repeat with the
 // M ring's period P (a shorter unrolled score loop; the extra M slot is
@@ -232,6 +244,11 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
         }
     }
     __syncthreads();
+    // loop-invariant VGPR copies of the window clamps' bounds
+    int zero_v, qlen_v, rlen_v;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero_v));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(qlen_v) : "s"(qlen));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(rlen_v) : "s"(rlen));
     // with o + e = DO g and e = DE g exactly, every per-score quantity is a
     // function of si = s / g: R(s) = (si - DO + DE) / DE from si = DO on,
     // P(s) = (mg - si) / DE with mg = me / g, and s <= me iff si <= mg
@@ -266,10 +283,6 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
             const int b_lo = (((run_lo - kbase) >> 6) - w + NW - 1) >> LNW;
             const int b_hi = ((min((run_hi - kbase) >> 6, nblk - 1)) - w) >> LNW;
             const unsigned nrun = (unsigned)max(b_hi - b_lo + 1, 0);
-            // the blocks wholly inside [lo_s, hi_s]: b in [f_lo, f_lo + nfull)
-            const int f_lo = ((((lo_s - kbase + 63) >> 6)) - w + NW - 1) >> LNW;
-            const int f_hi = (((hi_s - kbase - 63) >> 6) - w) >> LNW;
-            const unsigned nfull = (unsigned)max(f_hi - f_lo + 1, 0);
             const int ms = ph % NM, is = ph % NI;
             const int mx = pos_mod(ph - DX, NM);
             const int mo = pos_mod(ph - DO, NM), ie = pos_mod(ph - DE, NI);
@@ -289,20 +302,20 @@ __device__ int wfa_reg_pair(const uint64_t *qg, int qlen, const uint64_t *rg, in
                     int dv = __builtin_amdgcn_update_dpp(td_in, td, 0x130, 0xf, 0xf, false);
                     int mv = max(max(Mr[b][mx] + 1, iv), dv);
                     const int k = kb0 + lane;
-                    if ((unsigned)(b - f_lo) >= nfull) {  // a block across the range's ends
+                    if (kb0 < lo_s || kb0 + 63 > hi_s) {  // a block across the range's ends
                         if (k < lo_s || k > hi_s) iv = dv = mv = kNeg;
                     }
-                    // EXTEND: the first 32 bases without a branch, the rest
-                    // for the few lanes that matched all of them
+                    // EXTEND: the first 16 bases without a branch (one
+                    // window word per side), the rest in 32-base windows for
+                    // the few lanes that matched all of them
                     const int row = mv, col = row - k;
-                    uint32_t a0, a1, b0, b1;
-                    window_lsb(qs, med3(row, 0, qlen), a0, a1);
-                    window_lsb(rs, med3(col, 0, rlen), b0, b1);
+                    const uint32_t a = window16_lsb(qs, med3(row, zero_v, qlen_v));
+                    const uint32_t c = window16_lsb(rs, med3(col, zero_v, rlen_v));
                     const int lim = min(qlen - row, rlen - col);
-                    const int n = med3(match32(a0, a1, b0, b1), lim, 0);
+                    const int n = med3(match16(a, c), lim, zero_v);
                     int mr = row >= 0 ? row + n : kNeg;
-                    if (row >= 0 && n == 32 && lim > 32)
-                        mr = extend_lsb(qs, qlen, rs, rlen, row + 32, col + 32);
+                    if (row >= 0 && n == 16 && lim > 16)
+                        mr = extend_lsb(qs, qlen, rs, rlen, row + 16, col + 16);
                     Ir[b][is] = iv;
                     Dr[b][is] = dv;
                     Mr[b][ms] = mr;
