@@ -331,23 +331,31 @@ def mixed_extra(m, dev, gen, n_pairs=100_000, cpu=True):
 def serving_extra(m, duration=3.0):
     """Serving loop on requests of 1-64 pairs (log-normal lengths,
     SW band 64 below 2 kb, WFA above): peak sustainable pairs/s in a closed
-    loop, then Poisson arrivals at 70% of that peak (p50/p99), then the
-    50,000 requests/s arrival rate of the workload definition."""
+    loop, then Poisson arrivals at 70% and at 10% of that peak (p50/p99 over
+    all requests, over SW-only requests and over requests holding a WFA
+    pair), then the 50,000 requests/s arrival rate of the workload
+    definition (an overload: admission and back-pressure)."""
     eng = m.ServingEngine(hbm_budget_bytes=16 << 30, n_slots=4, max_batch_pairs=16384,
                           max_batch_words=1 << 19, max_error_cap=8192)
     g = m.GeneratorHandle(5, 400_000, 11)
-    res = {"hbm_budget_bytes": 16 << 30, "device_bytes": eng.stats()["device_bytes"]}
-    eng.run_load(g.handle, 0.0, 0.5, max_outstanding=2048)  # warm-up
-    peak = eng.run_load(g.handle, 0.0, duration, max_outstanding=2048)
+    res = {"hbm_budget_bytes": 16 << 30, "device_bytes": eng.stats()["device_bytes"],
+           "bins": "SW pairs and WFA pairs batched apart, 4 slots and a dispatcher thread each"}
+    eng.run_load(g.handle, 0.0, 0.5, max_outstanding=4096)  # warm-up
+    peak = eng.run_load(g.handle, 0.0, duration, max_outstanding=4096)
     res["peak_closed_loop"] = peak
-    rate70 = 0.7 * peak["requests_per_s"]
-    res["at_70pct_peak"] = eng.run_load(g.handle, rate70, duration)
-    res["at_70pct_peak"]["offered_rps"] = rate70
+    for name, frac in (("at_70pct_peak", 0.7), ("at_10pct_peak", 0.1)):
+        rate = frac * peak["requests_per_s"]
+        res[name] = eng.run_load(g.handle, rate, duration)
+        res[name]["offered_rps"] = rate
     res["at_50k_rps"] = eng.run_load(g.handle, 50_000.0, duration)
-    for k in ("peak_closed_loop", "at_70pct_peak", "at_50k_rps"):
+    for k in ("peak_closed_loop", "at_70pct_peak", "at_10pct_peak", "at_50k_rps"):
         r = res[k]
         log(f"[bench] serving {k}: {r['requests_per_s']:.0f} req/s {r['pairs_per_s']:.0f} pairs/s "
-            f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms refused {r['refused']:.0f}")
+            f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms | SW-only p50 "
+            f"{r['sw_only_p50_ms']:.2f} p99 {r['sw_only_p99_ms']:.2f} | with WFA p50 "
+            f"{r['with_wfa_p50_ms']:.2f} p99 {r['with_wfa_p99_ms']:.2f} | batches "
+            f"{r['mean_sw_batch_pairs']:.0f} / {r['mean_wfa_batch_pairs']:.0f} pairs | "
+            f"refused {r['refused']:.0f}")
     g.close()
     eng.close()
     return res
@@ -477,7 +485,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run 3 device-to-device copies of this many bytes (counter scale)")
     ap.add_argument("--only-extra", choices=["wfa_20k_1to10kb", "ingest_200k_pairs", "wfa_cigar_20k_1to10kb",
-                                             "nw_200k_100to300bp"],
+                                             "nw_200k_100to300bp", "serving_1to64_pairs"],
                     help="run only this extra workload and print its result (for profiling "
                          "its kernels alone)")
     args = ap.parse_args()
@@ -485,7 +493,9 @@ def main():
         m = load_package()
         from cgm_amd import gen
         dev = m.GenomicsDevice(0)
-        if args.only_extra == "nw_200k_100to300bp":
+        if args.only_extra == "serving_1to64_pairs":
+            res = serving_extra(m)
+        elif args.only_extra == "nw_200k_100to300bp":
             w = gen.generate(gen.SW_100_300BP, args.pairs)
             cells = int(gen.valid_band_cells(w.q_len, w.r_len, w.band).sum())
             res = nw_extra(m, dev, w, cells)

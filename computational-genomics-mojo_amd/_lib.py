@@ -91,7 +91,7 @@ SIGNATURES = {
     "cgm_engine_create": [_i64, _i32, _i64, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                           _pvp],
     "cgm_engine_destroy": [_vp],
-    "cgm_engine_submit": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp,
+    "cgm_engine_submit": [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp,
                           ctypes.c_double, _pi64],
     "cgm_engine_poll": [_vp, _i64, _pi32],
     "cgm_engine_wait": [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)],

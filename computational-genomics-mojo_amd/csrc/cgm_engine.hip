@@ -1,18 +1,25 @@
 // cgm_engine.hip -- continuous-batching serving engine for SW + WFA pairs.
 //
-// Requests (a few pairs each, with their packed sequences) are copied at
-// submit time into host memory and queued.  One dispatcher thread owns the
-// GPU side: it keeps `n_slots` slots, each with a stream, device buffers,
-// pinned staging and a mixed-kernel plan (cgm_mixed.hip), all allocated at
-// creation and charged against a fixed HBM budget.  Whenever a slot is free
-// and requests are waiting, every waiting request that fits the slot forms
-// the next batch (first come, first served), so requests that arrive while
-// other batches run on the GPU start as soon as a slot frees and their
-// kernels overlap with the running ones.  Admission is bounded: a request
-// is refused with CGM_ERR_BUSY while the queued pairs would exceed the
-// engine's pending limit (back-pressure instead of unbounded queueing).
-// Every pair's result depends only on the pair, never on the batch it ran
-// in, so a request gets the same result alone and in any mix.
+// Pairs are batched by bin: bin 0 holds the SW pairs (kind 0), bin 1 the
+// WFA pairs (kind 1), and each bin's batches run the mixed-kernel plan with
+// that kind for every pair.  A request's pairs may go to both bins; it is
+// done when all of them are.
+// Each bin has its own dispatcher thread, `n_slots` slots (a stream,
+// device buffers and a mixed-kernel plan each, allocated at creation and
+// charged against a fixed HBM budget) and 2 n_slots + 1 pinned staging
+// buffers.  cgm_engine_submit writes a request's pairs straight into the
+// bin's open staging buffer (metadata and packed words, the only host
+// copy); whenever a slot of the bin is free the dispatcher takes the oldest
+// closed staging buffer, or the open one, and launches it: two host-to-
+// device copies, a kernel that spreads the metadata into the batch arrays,
+// the mixed-kernel plan, one device-to-host copy of the results and a host
+// function on the stream that hands the results to their requests and
+// frees the slot.  So SW requests never wait behind long WFA pairs, batches
+// of both bins overlap on the GPU, requests that arrive while batches run
+// are admitted as staging space and slots free up, and a request is refused
+// with CGM_ERR_BUSY when its bin has no staging space left (back-pressure
+// instead of unbounded queueing).  Every pair's result depends only on the
+// pair, never on the batch it ran in.
 
 #include <math.h>
 #include <string.h>
@@ -32,6 +39,8 @@
 
 namespace cgm {
 
+struct Engine;
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -40,204 +49,249 @@ double now_s() {
     return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
 }
 
+constexpr int kBins = 2;   // 0: SW, 1: WFA (the pair kind)
+constexpr int kMeta = 7;   // per pair: q_len, r_len, q_off, r_off, kind, band, max_error
+
 struct Request {
     int64_t id = 0;
     int32_t n = 0;
-    std::vector<uint64_t> qp, rp;
-    std::vector<int32_t> ql, rl, qo, ro, kind, band, me;
     std::vector<int32_t> score, qend, rend;
+    std::atomic<int32_t> remaining{0};
     double t_arrival = 0, t_done = 0;
-    int state = 0;  // 0 queued, 1 on the GPU, 2 done
+    int state = 0;  // 0 queued or running, 2 done
+    bool has_wfa = false;
+};
+
+struct Route {
+    Request *req;
+    int32_t idx;
+};
+
+struct Bin;
+
+// One batch being formed or run: pinned metadata (kMeta ints per pair),
+// packed words of both sides, results, and where each pair's result goes.
+struct Staging {
+    int32_t *h_meta = nullptr;
+    uint64_t *h_words = nullptr;
+    int32_t *h_out = nullptr;
+    int64_t n = 0, words = 0;
+    std::vector<Route> routes;
+    Bin *bin = nullptr;
+    int slot = -1;
+    int rc = CGM_OK;
 };
 
 struct Slot {
     hipStream_t stream = nullptr;
     MixPlan *plan = nullptr;
-    uint64_t *d_qp = nullptr, *d_rp = nullptr, *h_qp = nullptr, *h_rp = nullptr;
-    int32_t *d_arr = nullptr, *h_arr = nullptr;  // 7 arrays of cap_pairs
-    int32_t *d_out = nullptr, *h_out = nullptr;  // 3 arrays of cap_pairs
-    hipEvent_t done = nullptr;
-    std::vector<Request *> reqs;
-    bool busy = false;
+    int32_t *d_meta = nullptr, *d_arr = nullptr, *d_out = nullptr;
+    uint64_t *d_words = nullptr;
 };
+
+struct Bin {
+    int id = 0;
+    Engine *E = nullptr;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Slot> slots;
+    std::vector<int> free_slots;
+    std::vector<Staging> stages;
+    std::vector<Staging *> free_stages;
+    std::deque<Staging *> ready;
+    Staging *open = nullptr;
+    int64_t pending_pairs = 0;
+    std::atomic<int64_t> batches{0}, batch_pairs{0};
+    std::thread dispatcher;
+};
+
+// metadata of n pairs (kMeta ints each) -> kMeta arrays of stride cap
+__global__ void engine_unpack_kernel(const int32_t *meta, int64_t n, int32_t *arr, int64_t cap) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 0; k < kMeta; ++k) arr[k * cap + i] = meta[i * kMeta + k];
+}
+
+void engine_host_done(void *arg);
 
 }  // namespace
 
 struct Engine {
     int device = 0;
     int n_slots = 0;
-    int64_t cap_pairs = 0, cap_words = 0, max_pending_pairs = 0, device_bytes = 0;
+    int64_t cap_pairs = 0, cap_words = 0, device_bytes = 0;
     SwScoring sc{};
     WfaPen pen{};
-    std::vector<Slot> slots;
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Request *> queue;
-    int64_t queued_pairs = 0;
+    Bin bins[kBins];
+    std::mutex mu;  // requests: live map, pool, cv_done
+    std::condition_variable cv_done;
     std::unordered_map<int64_t, Request *> live;
+    std::vector<Request *> pool;
     int64_t next_id = 1;
-    bool stop = false;
-    std::thread worker;
-    std::atomic<int64_t> batches{0}, batch_pairs{0}, done_requests{0}, done_pairs{0}, refused{0};
-    int worker_error = 0;
-    std::string worker_msg;
+    std::atomic<bool> stop{false};
+    std::atomic<int64_t> done_requests{0}, done_pairs{0}, refused{0};
+    std::atomic<int> worker_error{0};
 
     ~Engine() {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            stop = true;
+        stop = true;
+        for (Bin &B : bins) {
+            {
+                std::lock_guard<std::mutex> lk(B.mu);
+            }
+            B.cv.notify_all();
+            if (B.dispatcher.joinable()) B.dispatcher.join();
         }
-        cv_work.notify_all();
-        if (worker.joinable()) worker.join();
         (void)hipSetDevice(device);
-        for (Slot &s : slots) {
-            if (s.stream) (void)hipStreamSynchronize(s.stream);
-            if (s.plan) mix_plan_destroy(s.plan);
-            void *dev_ptrs[] = {s.d_qp, s.d_rp, s.d_arr, s.d_out};
-            for (void *p : dev_ptrs)
-                if (p) (void)hipFree(p);
-            void *host_ptrs[] = {s.h_qp, s.h_rp, s.h_arr, s.h_out};
-            for (void *p : host_ptrs)
-                if (p) (void)hipHostFree(p);
-            if (s.done) (void)hipEventDestroy(s.done);
-            if (s.stream) (void)hipStreamDestroy(s.stream);
+        for (Bin &B : bins) {
+            for (Slot &s : B.slots) {
+                if (s.stream) (void)hipStreamSynchronize(s.stream);
+                if (s.plan) mix_plan_destroy(s.plan);
+                void *dev_ptrs[] = {s.d_meta, s.d_arr, s.d_out, s.d_words};
+                for (void *p : dev_ptrs)
+                    if (p) (void)hipFree(p);
+                if (s.stream) (void)hipStreamDestroy(s.stream);
+            }
+            for (Staging &st : B.stages) {
+                void *host_ptrs[] = {st.h_meta, st.h_words, st.h_out};
+                for (void *p : host_ptrs)
+                    if (p) (void)hipHostFree(p);
+            }
         }
         for (auto &kv : live) delete kv.second;
+        for (Request *r : pool) delete r;
     }
 
-    // Copies finished slots' results into their requests.
-    bool retire() {
-        bool any = false;
-        for (Slot &s : slots) {
-            if (!s.busy) continue;
-            hipError_t q = hipEventQuery(s.done);
-            if (q == hipErrorNotReady) continue;
-            any = true;
-            const double t = now_s();
-            int64_t off = 0;
-            const int64_t n_batch = [&] {
-                int64_t k = 0;
-                for (Request *r : s.reqs) k += r->n;
-                return k;
-            }();
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                for (Request *r : s.reqs) {
-                    r->score.assign(s.h_out + off, s.h_out + off + r->n);
-                    r->qend.assign(s.h_out + cap_pairs + off, s.h_out + cap_pairs + off + r->n);
-                    r->rend.assign(s.h_out + 2 * cap_pairs + off,
-                                   s.h_out + 2 * cap_pairs + off + r->n);
-                    if (q != hipSuccess) std::fill(r->score.begin(), r->score.end(), CGM_BAD_PAIR);
-                    r->t_done = t;
-                    r->state = 2;
-                    off += r->n;
-                }
-                done_requests += (int64_t)s.reqs.size();
-                done_pairs += n_batch;
-            }
-            s.reqs.clear();
-            s.busy = false;
-            cv_done.notify_all();
-        }
-        return any;
+    Request *take_request() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (pool.empty()) return new Request();
+        Request *r = pool.back();
+        pool.pop_back();
+        return r;
     }
 
-    // Forms and launches one batch on a free slot; false when nothing waits.
-    bool launch(Slot &s) {
-        std::vector<Request *> take;
-        int64_t np = 0, qw = 0, rw = 0;
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            while (!queue.empty()) {
-                Request *r = queue.front();
-                if (np + r->n > cap_pairs || qw + (int64_t)r->qp.size() > cap_words ||
-                    rw + (int64_t)r->rp.size() > cap_words)
-                    break;
-                queue.pop_front();
-                queued_pairs -= r->n;
-                np += r->n;
-                qw += (int64_t)r->qp.size();
-                rw += (int64_t)r->rp.size();
-                r->state = 1;
-                take.push_back(r);
-            }
-        }
-        if (take.empty()) return false;
-        // pack into pinned staging: words, then 7 arrays of cap_pairs
-        int64_t p = 0, wq = 0, wr = 0;
-        int32_t *A = s.h_arr;
-        for (Request *r : take) {
-            memcpy(s.h_qp + wq, r->qp.data(), r->qp.size() * 8);
-            memcpy(s.h_rp + wr, r->rp.data(), r->rp.size() * 8);
-            for (int32_t i = 0; i < r->n; ++i) {
-                A[0 * cap_pairs + p + i] = r->ql[i];
-                A[1 * cap_pairs + p + i] = r->rl[i];
-                A[2 * cap_pairs + p + i] = r->qo[i] + (int32_t)wq;
-                A[3 * cap_pairs + p + i] = r->ro[i] + (int32_t)wr;
-                A[4 * cap_pairs + p + i] = r->kind[i];
-                A[5 * cap_pairs + p + i] = r->band[i];
-                A[6 * cap_pairs + p + i] = r->me[i];
-            }
-            p += r->n;
-            wq += (int64_t)r->qp.size();
-            wr += (int64_t)r->rp.size();
-        }
-        hipStream_t st = s.stream;
+    // Launches staging `st` on slot `si` of bin B; on any failure the batch's
+    // pairs are reported as refused (score CGM_BAD_PAIR, ends -1).
+    void launch(Bin &B, int si, Staging *st) {
+        Slot &S = B.slots[(size_t)si];
+        hipStream_t s = S.stream;
+        const int64_t p = st->n, w = st->words, cap = cap_pairs;
+        st->slot = si;
         int rc = CGM_OK;
-        auto h2d = [&](void *d, const void *h, int64_t bytes) {
-            if (rc == CGM_OK && bytes > 0 &&
-                hipMemcpyAsync(d, h, (size_t)bytes, hipMemcpyHostToDevice, st) != hipSuccess)
-                rc = CGM_ERR_HIP;
+        auto ok = [&](hipError_t e) {
+            if (rc == CGM_OK && e != hipSuccess) rc = CGM_ERR_HIP;
         };
-        h2d(s.d_qp, s.h_qp, wq * 8);
-        h2d(s.d_rp, s.h_rp, wr * 8);
-        for (int k = 0; k < 7; ++k) h2d(s.d_arr + k * cap_pairs, s.h_arr + k * cap_pairs, p * 4);
-        int32_t *D = s.d_arr;
-        Batch b = {s.d_qp, s.d_rp, D + 0 * cap_pairs, D + 1 * cap_pairs, D + 2 * cap_pairs,
-                   D + 3 * cap_pairs, p, wq, wr};
-        SwOut out = {s.d_out, s.d_out + cap_pairs, s.d_out + 2 * cap_pairs};
-        if (rc == CGM_OK)
-            rc = mix_enqueue(s.plan, b, D + 4 * cap_pairs, 0, D + 5 * cap_pairs, 0,
-                             D + 6 * cap_pairs, 0, sc, pen, out, st);
-        for (int k = 0; k < 3 && rc == CGM_OK; ++k)
-            if (hipMemcpyAsync(s.h_out + k * cap_pairs, s.d_out + k * cap_pairs, (size_t)p * 4,
-                               hipMemcpyDeviceToHost, st) != hipSuccess)
-                rc = CGM_ERR_HIP;
-        if (rc == CGM_OK && hipEventRecord(s.done, st) != hipSuccess) rc = CGM_ERR_HIP;
-        s.reqs = take;
-        s.busy = true;
-        batches += 1;
-        batch_pairs += p;
-        if (rc != CGM_OK) {
-            // results of this batch are reported as bad pairs
-            (void)hipStreamSynchronize(st);
-            std::lock_guard<std::mutex> lk(mu);
-            worker_error = rc;
-            for (int64_t i = 0; i < p; ++i) s.h_out[i] = CGM_BAD_PAIR;
-            (void)hipEventRecord(s.done, st);
+        ok(hipMemcpyAsync(S.d_meta, st->h_meta, (size_t)(p * kMeta * 4), hipMemcpyHostToDevice, s));
+        if (w > 0)
+            ok(hipMemcpyAsync(S.d_words, st->h_words, (size_t)(w * 8), hipMemcpyHostToDevice, s));
+        if (rc == CGM_OK) {
+            hipLaunchKernelGGL(engine_unpack_kernel, dim3((unsigned)((p + 255) / 256)), dim3(256), 0,
+                               s, S.d_meta, p, S.d_arr, cap);
+            ok(hipGetLastError());
         }
-        return true;
+        const int32_t *A = S.d_arr;
+        Batch b = {S.d_words, S.d_words, A, A + cap, A + 2 * cap, A + 3 * cap, p, w, w};
+        SwOut out = {S.d_out, S.d_out + p, S.d_out + 2 * p};
+        if (rc == CGM_OK)
+            rc = mix_enqueue(S.plan, b, nullptr, B.id, A + 5 * cap, 0, A + 6 * cap, 0, sc, pen, out, s);
+        if (rc == CGM_OK)
+            ok(hipMemcpyAsync(st->h_out, S.d_out, (size_t)(3 * p * 4), hipMemcpyDeviceToHost, s));
+        if (rc == CGM_OK) ok(hipLaunchHostFunc(s, engine_host_done, st));
+        B.batches += 1;
+        B.batch_pairs += p;
+        if (rc != CGM_OK) {
+            (void)hipStreamSynchronize(s);
+            worker_error = rc;
+            st->rc = rc;
+            engine_host_done(st);
+        }
     }
 
-    void run() {
+    void dispatch(Bin &B) {
         (void)hipSetDevice(device);
         for (;;) {
-            bool did = retire();
-            for (Slot &s : slots)
-                if (!s.busy && launch(s)) did = true;
-            std::unique_lock<std::mutex> lk(mu);
-            if (stop) break;
-            bool busy_any = false;
-            for (Slot &s : slots) busy_any |= s.busy;
-            if (!did) {
-                if (busy_any || !queue.empty())
-                    cv_work.wait_for(lk, std::chrono::microseconds(20));
-                else
-                    cv_work.wait_for(lk, std::chrono::milliseconds(2));
+            Staging *st = nullptr;
+            int si = -1;
+            {
+                std::unique_lock<std::mutex> lk(B.mu);
+                B.cv.wait(lk, [&] {
+                    return stop.load() || (!B.free_slots.empty() &&
+                                           (!B.ready.empty() || B.open->n > 0));
+                });
+                if (stop.load()) return;
+                if (!B.ready.empty()) {
+                    st = B.ready.front();
+                    B.ready.pop_front();
+                } else {
+                    // a slot is free and nothing else waits, so a free staging
+                    // buffer exists (2 n_slots + 1 of them)
+                    st = B.open;
+                    B.open = B.free_stages.back();
+                    B.free_stages.pop_back();
+                }
+                B.pending_pairs -= st->n;
+                si = B.free_slots.back();
+                B.free_slots.pop_back();
             }
+            launch(B, si, st);
         }
     }
 };
+
+namespace {
+
+// Runs once a batch's results are in pinned memory (a host function on the
+// slot's stream, or directly after a failed launch): routes every pair's
+// result to its request, marks the finished requests, frees the slot and
+// the staging buffer.  Makes no HIP call.
+void engine_host_done(void *arg) {
+    Staging *st = (Staging *)arg;
+    Bin &B = *st->bin;
+    Engine &E = *B.E;
+    const int64_t p = st->n;
+    const int32_t *o = st->h_out;
+    std::vector<Request *> finished;
+    for (int64_t i = 0; i < p; ++i) {
+        const Route &rt = st->routes[(size_t)i];
+        Request *r = rt.req;
+        if (st->rc == CGM_OK) {
+            r->score[(size_t)rt.idx] = o[i];
+            r->qend[(size_t)rt.idx] = o[p + i];
+            r->rend[(size_t)rt.idx] = o[2 * p + i];
+        } else {
+            r->score[(size_t)rt.idx] = CGM_BAD_PAIR;
+            r->qend[(size_t)rt.idx] = -1;
+            r->rend[(size_t)rt.idx] = -1;
+        }
+        if (r->remaining.fetch_sub(1) == 1) finished.push_back(r);
+    }
+    if (!finished.empty()) {
+        const double t = now_s();
+        {
+            std::lock_guard<std::mutex> lk(E.mu);
+            for (Request *r : finished) {
+                r->t_done = t;
+                r->state = 2;
+            }
+        }
+        E.cv_done.notify_all();
+        E.done_requests += (int64_t)finished.size();
+    }
+    E.done_pairs += p;
+    {
+        std::lock_guard<std::mutex> lk(B.mu);
+        B.free_slots.push_back(st->slot);
+        st->n = 0;
+        st->words = 0;
+        st->rc = CGM_OK;
+        st->slot = -1;
+        st->routes.clear();
+        B.free_stages.push_back(st);
+    }
+    B.cv.notify_one();
+}
+
+}  // namespace
 
 }  // namespace cgm
 
@@ -250,7 +304,7 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
                       int32_t mismatch, int32_t gap_open, int32_t gap_extend, int32_t x, int32_t o,
                       int32_t e, void **engine) {
     if (!engine || n_slots < 1 || n_slots > 16 || max_batch_pairs < 1 || max_batch_words < 1 ||
-        max_batch_pairs > (int64_t(1) << 28) || max_batch_words > (int64_t(1) << 30))
+        max_batch_pairs > (int64_t(1) << 28) || max_batch_words > (int64_t(1) << 29))
         return set_error(CGM_ERR_INVALID, "cgm_engine_create: bad arguments");
     SwScoring sc = {match, mismatch, gap_open, gap_extend};
     WfaPen pen = {x, o, e};
@@ -265,33 +319,50 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
     E->n_slots = n_slots;
     E->cap_pairs = max_batch_pairs;
     E->cap_words = max_batch_words;
-    E->max_pending_pairs = (int64_t)n_slots * max_batch_pairs;
     E->sc = sc;
     E->pen = pen;
-    E->slots.resize((size_t)n_slots);
-    for (Slot &s : E->slots) {
-        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
-            return fail(set_error(CGM_ERR_HIP, "engine: stream/event"));
-        int rc = mix_plan_create(max_batch_pairs, max_error_cap, pen, 0, 0, &s.plan);
-        if (rc) return fail(rc);
-        E->device_bytes += mix_plan_bytes(s.plan);
-        const int64_t wb = max_batch_words * 8, ab = 7 * max_batch_pairs * 4,
-                      ob = 3 * max_batch_pairs * 4;
-        E->device_bytes += 2 * wb + ab + ob;
-        if (E->device_bytes > hbm_budget_bytes)
-            return fail(set_error(CGM_ERR_NOMEM, "engine: slots exceed the HBM budget"));
-        if (hipMalloc((void **)&s.d_qp, wb) != hipSuccess ||
-            hipMalloc((void **)&s.d_rp, wb) != hipSuccess ||
-            hipMalloc((void **)&s.d_arr, ab) != hipSuccess ||
-            hipMalloc((void **)&s.d_out, ob) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_qp, wb, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_rp, wb, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_arr, ab, hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void **)&s.h_out, ob, hipHostMallocDefault) != hipSuccess)
-            return fail(set_error(CGM_ERR_NOMEM, "engine: slot buffers"));
+    const int64_t cap = max_batch_pairs, wcap = 2 * max_batch_words;
+    for (int bi = 0; bi < kBins; ++bi) {
+        Bin &B = E->bins[bi];
+        B.id = bi;
+        B.E = E;
+        B.slots.resize((size_t)n_slots);
+        for (int k = 0; k < n_slots; ++k) {
+            Slot &s = B.slots[(size_t)k];
+            if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess)
+                return fail(set_error(CGM_ERR_HIP, "engine: stream"));
+            // SW slots carry no WFA workspace
+            int rc = mix_plan_create(max_batch_pairs, bi == 1 ? max_error_cap : 0, pen, 0, 0, &s.plan);
+            if (rc) return fail(rc);
+            E->device_bytes += mix_plan_bytes(s.plan) + wcap * 8 + (2 * kMeta + 3) * cap * 4;
+            if (E->device_bytes > hbm_budget_bytes)
+                return fail(set_error(CGM_ERR_NOMEM, "engine: slots exceed the HBM budget"));
+            if (hipMalloc((void **)&s.d_meta, (size_t)(kMeta * cap * 4)) != hipSuccess ||
+                hipMalloc((void **)&s.d_arr, (size_t)(kMeta * cap * 4)) != hipSuccess ||
+                hipMalloc((void **)&s.d_out, (size_t)(3 * cap * 4)) != hipSuccess ||
+                hipMalloc((void **)&s.d_words, (size_t)(wcap * 8)) != hipSuccess)
+                return fail(set_error(CGM_ERR_NOMEM, "engine: slot buffers"));
+            B.free_slots.push_back(n_slots - 1 - k);
+        }
+        B.stages.resize((size_t)(2 * n_slots + 1));
+        for (Staging &st : B.stages) {
+            st.bin = &B;
+            st.routes.reserve((size_t)cap);
+            if (hipHostMalloc((void **)&st.h_meta, (size_t)(kMeta * cap * 4), hipHostMallocDefault) !=
+                    hipSuccess ||
+                hipHostMalloc((void **)&st.h_words, (size_t)(wcap * 8), hipHostMallocDefault) !=
+                    hipSuccess ||
+                hipHostMalloc((void **)&st.h_out, (size_t)(3 * cap * 4), hipHostMallocDefault) !=
+                    hipSuccess)
+                return fail(set_error(CGM_ERR_NOMEM, "engine: pinned staging"));
+        }
+        B.open = &B.stages[0];
+        for (size_t k = B.stages.size(); k-- > 1;) B.free_stages.push_back(&B.stages[k]);
     }
-    E->worker = std::thread([E] { E->run(); });
+    for (Bin &B : E->bins) {
+        Bin *pb = &B;
+        B.dispatcher = std::thread([E, pb] { E->dispatch(*pb); });
+    }
     *engine = E;
     return CGM_OK;
 }
@@ -301,55 +372,96 @@ int cgm_engine_destroy(void *engine) {
     return CGM_OK;
 }
 
-int cgm_engine_submit(void *engine, const uint64_t *q_packed, const uint64_t *r_packed,
-                      const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
-                      const int32_t *r_off, int32_t n_pairs, const int32_t *kind,
-                      const int32_t *band, const int32_t *max_error, double arrival_s,
-                      int64_t *request_id) {
+int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
+                      const uint64_t *r_packed, int64_t r_words, const int32_t *q_len,
+                      const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                      int32_t n_pairs, const int32_t *kind, const int32_t *band,
+                      const int32_t *max_error, double arrival_s, int64_t *request_id) {
     Engine *E = (Engine *)engine;
     if (!E || !request_id || n_pairs < 1 || !q_len || !r_len || !q_off || !r_off || !kind ||
-        !band || !max_error)
+        !band || !max_error || q_words < 0 || r_words < 0)
         return set_error(CGM_ERR_INVALID, "cgm_engine_submit: bad arguments");
-    if (n_pairs > E->cap_pairs) return set_error(CGM_ERR_INVALID, "request larger than a batch");
-    Request *r = new Request();
-    r->n = n_pairs;
-    r->ql.assign(q_len, q_len + n_pairs);
-    r->rl.assign(r_len, r_len + n_pairs);
-    r->kind.assign(kind, kind + n_pairs);
-    r->band.assign(band, band + n_pairs);
-    r->me.assign(max_error, max_error + n_pairs);
-    r->qo.resize((size_t)n_pairs);
-    r->ro.resize((size_t)n_pairs);
-    // copy each sequence's words so the request owns compact buffers
+    // every pair's words must lie inside the caller's buffers
+    int64_t nb[kBins] = {0, 0}, wb[kBins] = {0, 0};
     for (int32_t i = 0; i < n_pairs; ++i) {
-        if (q_len[i] < 0 || r_len[i] < 0 || q_off[i] < 0 || r_off[i] < 0) {
-            delete r;
-            return set_error(CGM_ERR_INVALID, "negative length or offset");
-        }
-        const int32_t qw = (q_len[i] + 31) / 32, rw = (r_len[i] + 31) / 32;
-        r->qo[(size_t)i] = (int32_t)r->qp.size();
-        r->ro[(size_t)i] = (int32_t)r->rp.size();
-        r->qp.insert(r->qp.end(), q_packed + q_off[i], q_packed + q_off[i] + qw);
-        r->rp.insert(r->rp.end(), r_packed + r_off[i], r_packed + r_off[i] + rw);
+        if (q_len[i] < 0 || r_len[i] < 0 || q_off[i] < 0 || r_off[i] < 0)
+            return set_error(CGM_ERR_INVALID, "cgm_engine_submit: negative length or offset");
+        if (kind[i] != 0 && kind[i] != 1)
+            return set_error(CGM_ERR_INVALID, "cgm_engine_submit: kind must be 0 (SW) or 1 (WFA)");
+        const int64_t qw = ((int64_t)q_len[i] + 31) / 32, rw = ((int64_t)r_len[i] + 31) / 32;
+        if ((qw > 0 && !q_packed) || (rw > 0 && !r_packed) || q_off[i] + qw > q_words ||
+            r_off[i] + rw > r_words)
+            return set_error(CGM_ERR_INVALID, "cgm_engine_submit: sequence outside its buffer");
+        const int bi = kind[i];
+        nb[bi] += 1;
+        wb[bi] += qw + rw;
     }
-    if ((int64_t)r->qp.size() > E->cap_words || (int64_t)r->rp.size() > E->cap_words) {
-        delete r;
-        return set_error(CGM_ERR_INVALID, "request sequences exceed a batch");
-    }
+    for (int bi = 0; bi < kBins; ++bi)
+        if (nb[bi] > E->cap_pairs || wb[bi] > 2 * E->cap_words)
+            return set_error(CGM_ERR_INVALID, "cgm_engine_submit: request larger than a batch");
+    Request *r = E->take_request();
+    r->n = n_pairs;
+    r->score.assign((size_t)n_pairs, 0);
+    r->qend.assign((size_t)n_pairs, 0);
+    r->rend.assign((size_t)n_pairs, 0);
+    r->remaining = n_pairs;
+    r->state = 0;
+    r->has_wfa = nb[1] > 0;
     r->t_arrival = arrival_s > 0 ? arrival_s : now_s();
+    r->t_done = 0;
     {
         std::lock_guard<std::mutex> lk(E->mu);
-        if (E->queued_pairs + n_pairs > E->max_pending_pairs) {
-            E->refused += 1;
-            delete r;
-            return set_error(CGM_ERR_BUSY, "engine queue full");
-        }
         r->id = E->next_id++;
-        E->queue.push_back(r);
-        E->queued_pairs += n_pairs;
         E->live[r->id] = r;
     }
-    E->cv_work.notify_one();
+    // both bins' locks, in bin order, so the request is admitted whole or not at all
+    std::unique_lock<std::mutex> l0(E->bins[0].mu), l1(E->bins[1].mu);
+    bool fits_open[kBins];
+    for (int bi = 0; bi < kBins; ++bi) {
+        Bin &B = E->bins[bi];
+        fits_open[bi] = B.open->n + nb[bi] <= E->cap_pairs && B.open->words + wb[bi] <= 2 * E->cap_words;
+        if (nb[bi] > 0 && !fits_open[bi] && B.free_stages.empty()) {
+            l1.unlock();
+            l0.unlock();
+            E->refused += 1;
+            std::lock_guard<std::mutex> lk(E->mu);
+            E->live.erase(r->id);
+            E->pool.push_back(r);
+            return set_error(CGM_ERR_BUSY, "engine queue full");
+        }
+    }
+    for (int bi = 0; bi < kBins; ++bi) {
+        if (nb[bi] == 0) continue;
+        Bin &B = E->bins[bi];
+        if (!fits_open[bi]) {
+            B.ready.push_back(B.open);
+            B.open = B.free_stages.back();
+            B.free_stages.pop_back();
+        }
+        Staging *st = B.open;
+        for (int32_t i = 0; i < n_pairs; ++i) {
+            if (kind[i] != bi) continue;
+            const int64_t qw = ((int64_t)q_len[i] + 31) / 32, rw = ((int64_t)r_len[i] + 31) / 32;
+            int32_t *m = st->h_meta + st->n * kMeta;
+            m[0] = q_len[i];
+            m[1] = r_len[i];
+            m[2] = (int32_t)st->words;
+            m[3] = (int32_t)(st->words + qw);
+            m[4] = kind[i];
+            m[5] = band[i];
+            m[6] = max_error[i];
+            if (qw) memcpy(st->h_words + st->words, q_packed + q_off[i], (size_t)qw * 8);
+            if (rw) memcpy(st->h_words + st->words + qw, r_packed + r_off[i], (size_t)rw * 8);
+            st->words += qw + rw;
+            st->routes.push_back({r, i});
+            st->n += 1;
+        }
+        B.pending_pairs += nb[bi];
+    }
+    l1.unlock();
+    l0.unlock();
+    for (int bi = 0; bi < kBins; ++bi)
+        if (nb[bi] > 0) E->bins[bi].cv.notify_one();
     *request_id = r->id;
     return CGM_OK;
 }
@@ -383,20 +495,26 @@ int cgm_engine_wait(void *engine, int64_t request_id, int32_t timeout_ms, int32_
     if (r_end) memcpy(r_end, r->rend.data(), (size_t)r->n * 4);
     if (latency_ms) *latency_ms = (r->t_done - r->t_arrival) * 1e3;
     E->live.erase(it);
-    delete r;
+    E->pool.push_back(r);
     return CGM_OK;
 }
 
 int cgm_engine_stats(void *engine, int64_t *out /* 8 */) {
     Engine *E = (Engine *)engine;
     if (!E || !out) return set_error(CGM_ERR_INVALID, "cgm_engine_stats: bad arguments");
-    std::lock_guard<std::mutex> lk(E->mu);
-    out[0] = E->batches;
-    out[1] = E->batch_pairs;
+    int64_t batches = 0, batch_pairs = 0, queued = 0;
+    for (Bin &B : E->bins) {
+        std::lock_guard<std::mutex> lk(B.mu);
+        batches += B.batches;
+        batch_pairs += B.batch_pairs;
+        queued += B.pending_pairs;
+    }
+    out[0] = batches;
+    out[1] = batch_pairs;
     out[2] = E->done_requests;
     out[3] = E->done_pairs;
     out[4] = E->refused;
-    out[5] = E->queued_pairs;
+    out[5] = queued;
     out[6] = E->device_bytes;
     out[7] = E->worker_error;
     return CGM_OK;
@@ -406,7 +524,7 @@ double cgm_engine_now(void) { return now_s(); }
 
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                   int32_t max_pairs_per_request, int32_t max_outstanding, uint64_t seed,
-                  double *stats /* 16 */) {
+                  double *stats /* 24 */) {
     Engine *E = (Engine *)engine;
     if (!E || !gen || !stats || duration_s <= 0 || max_pairs_per_request < 1)
         return set_error(CGM_ERR_INVALID, "cgm_serve_run: bad arguments");
@@ -419,21 +537,28 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
     if (int rc = cgm_gen_copy(gen, qp.data(), nullptr, qo.data(), ql.data(), rp.data(), nullptr,
                               ro.data(), rl.data(), kind.data(), band.data(), me.data()))
         return rc;
+    int64_t b0[kBins], bp0[kBins];
+    for (int bi = 0; bi < kBins; ++bi) {
+        b0[bi] = E->bins[bi].batches;
+        bp0[bi] = E->bins[bi].batch_pairs;
+    }
     std::mt19937_64 rng(seed);
     std::exponential_distribution<double> gap(rate_rps > 0 ? rate_rps : 1.0);
     std::uniform_int_distribution<int> sz(1, max_pairs_per_request);
     struct Sub {
         int64_t id;
         int n;
+        bool wfa;
     };
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Sub> inflight;
     bool submit_done = false;
-    std::vector<double> lat;
+    std::vector<double> lat, lat_sw, lat_wfa;
     int64_t completed = 0, pairs_done = 0, bad = 0;
     std::atomic<int64_t> outstanding{0};
     std::thread collector([&] {
+        std::vector<int32_t> sc((size_t)max_pairs_per_request);
         for (;;) {
             Sub s;
             {
@@ -443,13 +568,13 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                 s = inflight.front();
                 inflight.pop_front();
             }
-            std::vector<int32_t> sc((size_t)s.n);
             double l = 0;
             if (cgm_engine_wait(E, s.id, -1, sc.data(), nullptr, nullptr, &l) == CGM_OK) {
                 lat.push_back(l);
+                (s.wfa ? lat_wfa : lat_sw).push_back(l);
                 completed += 1;
                 pairs_done += s.n;
-                for (int32_t v : sc) bad += v == CGM_BAD_PAIR;
+                for (int i = 0; i < s.n; ++i) bad += sc[(size_t)i] == CGM_BAD_PAIR;
             }
             outstanding -= 1;
         }
@@ -466,7 +591,7 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                 continue;
             }
         } else if (outstanding.load() >= max_outstanding) {
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            std::this_thread::sleep_for(std::chrono::microseconds(5));
             continue;
         }
         const int n = sz(rng);
@@ -474,16 +599,19 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
         const double arrival = rate_rps > 0 ? t_next : t;
         int64_t id = 0;
         const size_t c = (size_t)cursor;
-        const int rc = cgm_engine_submit(E, qp.data(), rp.data(), ql.data() + c, rl.data() + c,
-                                         qo.data() + c, ro.data() + c, n, kind.data() + c,
-                                         band.data() + c, me.data() + c, arrival, &id);
+        bool wfa = false;
+        for (int i = 0; i < n; ++i) wfa |= kind[c + (size_t)i] == 1;
+        const int rc = cgm_engine_submit(E, qp.data(), qwn, rp.data(), rwn, ql.data() + c,
+                                         rl.data() + c, qo.data() + c, ro.data() + c, n,
+                                         kind.data() + c, band.data() + c, me.data() + c, arrival,
+                                         &id);
         cursor += n;
         if (rate_rps > 0) t_next += gap(rng);
         if (rc == CGM_OK) {
             submitted += 1;
             outstanding += 1;
             std::lock_guard<std::mutex> lk(mu);
-            inflight.push_back({id, n});
+            inflight.push_back({id, n, wfa});
             cv.notify_one();
         } else if (rc == CGM_ERR_BUSY) {
             refused += 1;
@@ -500,19 +628,22 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
     cv.notify_all();
     collector.join();
     const double t_end = now_s();
-    std::sort(lat.begin(), lat.end());
-    auto pct = [&](double p) {
-        if (lat.empty()) return 0.0;
-        size_t i = (size_t)std::min<double>((double)lat.size() - 1, floor(p * (lat.size() - 1) + 0.5));
-        return lat[i];
+    auto pct = [](std::vector<double> &v, double p) {
+        if (v.empty()) return 0.0;
+        std::sort(v.begin(), v.end());
+        const size_t i = (size_t)std::min<double>((double)v.size() - 1, floor(p * (double)(v.size() - 1) + 0.5));
+        return v[i];
     };
     double mean = 0;
     for (double v : lat) mean += v;
     if (!lat.empty()) mean /= (double)lat.size();
-    int64_t st[8];
-    cgm_engine_stats(E, st);
+    int64_t nb[kBins], np[kBins];
+    for (int bi = 0; bi < kBins; ++bi) {
+        nb[bi] = E->bins[bi].batches - b0[bi];
+        np[bi] = E->bins[bi].batch_pairs - bp0[bi];
+    }
     const double elapsed = t_end - t0;
-    for (int i = 0; i < 16; ++i) stats[i] = 0;
+    for (int i = 0; i < 24; ++i) stats[i] = 0;
     stats[0] = (double)submitted;
     stats[1] = (double)completed;
     stats[2] = (double)refused;
@@ -520,15 +651,23 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
     stats[4] = elapsed;
     stats[5] = pairs_done / elapsed;
     stats[6] = completed / elapsed;
-    stats[7] = pct(0.50);
-    stats[8] = pct(0.90);
-    stats[9] = pct(0.99);
+    stats[7] = pct(lat, 0.50);
+    stats[8] = pct(lat, 0.90);
+    stats[9] = pct(lat, 0.99);
     stats[10] = lat.empty() ? 0 : lat.back();
     stats[11] = mean;
-    stats[12] = (double)st[0];
-    stats[13] = st[0] ? (double)st[1] / (double)st[0] : 0;
+    stats[12] = (double)(nb[0] + nb[1]);
+    stats[13] = nb[0] + nb[1] ? (double)(np[0] + np[1]) / (double)(nb[0] + nb[1]) : 0;
     stats[14] = (double)bad;
     stats[15] = t_sub_end - t0;
+    stats[16] = (double)lat_sw.size();
+    stats[17] = pct(lat_sw, 0.50);
+    stats[18] = pct(lat_sw, 0.99);
+    stats[19] = (double)lat_wfa.size();
+    stats[20] = pct(lat_wfa, 0.50);
+    stats[21] = pct(lat_wfa, 0.99);
+    stats[22] = nb[0] ? (double)np[0] / (double)nb[0] : 0;
+    stats[23] = nb[1] ? (double)np[1] / (double)nb[1] : 0;
     return CGM_OK;
 }
 

@@ -42,9 +42,9 @@ namespace cgm {
 
 namespace {
 
-constexpr int kMixClasses = 6;
+constexpr int kMixClasses = 7;
 constexpr int kMixSwReg = 0, kMixSwPacked = 1, kMixWfa = 2, kMixSwLds = 3, kMixSwPlain = 4,
-              kMixBad = 5;
+              kMixBad = 5, kMixSwWave = 6;
 constexpr int kMixWaves = 4;              // wavefronts per workgroup
 constexpr int kMixMaxBlocksPerCu = 2;     // 256 VGPRs: two workgroups per CU at most
 constexpr int kMixMaxLds = 160 * 1024;    // LDS of a CU
@@ -67,6 +67,7 @@ constexpr int kSmallBandCount = 32;          // register-lane pairs per band [kR
 constexpr int kSmallBandTask = 32 + 72;      // first task of each band, widest first [kRegBands+1]
 constexpr int kSmallBandPair = 32 + 2 * 72;  // first pair (in the class) of each band [kRegBands]
 constexpr int kSmallUnits = 28;              // SW kernel: long/short unit counters (64 bit)
+constexpr int kSmallWaveQueue = 30;          // SW wavefront-per-pair kernel: pair counter
 constexpr int kSmallWfaCounts = 256;         // WFA pairs per layout class [8]
 constexpr int kSmallWfaRanges = 264;         // [start, end) per WFA layout class [16]
 constexpr int kSmallWfaQueues = 280;         // WFA class kernels' counters [8]
@@ -118,6 +119,9 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
                     const bool long_pair = max(ql, rl) > tune.lane_max_len;
                     if (hmax > kMaxScoreProduct) {
                         cls = kMixBad;
+                    } else if (long_pair && be <= kWaveMaxBand && hmax < (1 << 23) &&
+                               max(ql, rl) < kWaveMaxLen) {
+                        cls = kMixSwWave;
                     } else if (long_pair && be <= kPlainMaxBand) {
                         cls = kMixSwPlain;
                     } else if (reg_scoring && be <= tune.reg_max_band && hmax < (1 << 23)) {
@@ -130,8 +134,10 @@ __global__ void mix_class_kernel(Batch b, const int32_t *kind, int32_t kind_scal
                     } else if (be <= kPlainMaxBand) {
                         cls = kMixSwPlain;
                     }
-                    payload = ((uint32_t)min(be, 2047) << 17) |
-                              (uint32_t)min(max(ql, rl), (1 << 17) - 1);
+                    payload = cls == kMixSwWave
+                                  ? (uint32_t)min(ql + rl, (1 << 28) - 1)  // longest first
+                                  : ((uint32_t)min(be, 2047) << 17) |
+                                        (uint32_t)min(max(ql, rl), (1 << 17) - 1);
                     if (cls == kMixSwReg) atomicAdd(&hist[be], 1);
                 }
             } else if (k == 1) {
@@ -587,9 +593,19 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     // launched first so that the SW persistent kernel fills the CUs their
     // tails free
     const bool has_wfa = kind != nullptr || kind_scalar == 1;
-    hipStream_t aux[2] = {stream, stream};
+    // long SW pairs (beyond lane_max_len, bands <= 64): one wavefront per
+    // pair on a third forked stream, also launched before the persistent
+    // kernel, since their serial chains are the batch's longest
+    const bool has_wave = tune.lane_max_len < kWaveMaxLen && (kind != nullptr || kind_scalar == 0);
+    hipStream_t aux[3] = {stream, stream, stream};
+    if (has_wfa || has_wave)
+        if (int rc = fork_streams(stream, 3, aux)) return rc;
+    if (has_wave)
+        if (int rc = launch_sw_wave_range(b, band, band_scalar, sc, out, P->order,
+                                          ranges + 2 * kMixSwWave, P->small + kSmallWaveQueue,
+                                          P->cus * 4, aux[2]))
+            return rc;
     if (has_wfa) {
-        if (int rc = fork_streams(stream, 2, aux)) return rc;
         CGM_HIP(hipEventRecord(P->ev_wfa_start, aux[0]));
         if (int rc = launch_wfa_classes(b, me, me_scalar, P->me_cap, pen, true, out.score, P->order,
                                         P->small + kSmallWfaRanges, P->small + kSmallWfaQueues,
@@ -624,7 +640,7 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     if (int rc = launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
                                        ranges + 2 * kMixSwPlain, P->cus * 2, stream))
         return rc;
-    return has_wfa ? join_streams(stream, 2, aux) : CGM_OK;
+    return has_wfa || has_wave ? join_streams(stream, 3, aux) : CGM_OK;
 }
 
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,

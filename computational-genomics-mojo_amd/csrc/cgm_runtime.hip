@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -163,45 +164,45 @@ int device_max_i32(const int32_t *a, int64_t n, hipStream_t stream, int32_t *res
     return CGM_OK;
 }
 
+// Forked streams per (device, caller stream): each caller stream gets its
+// own auxiliary streams and events, created on its first fork and kept
+// until the process ends, so work forked from different streams (several
+// plans, several serving slots) never queues behind each other.
 namespace {
 struct AuxStreams {
     hipStream_t s[kMaxAuxStreams] = {};
-    hipEvent_t fork[kMaxAuxStreams] = {}, join[kMaxAuxStreams] = {};
-    bool ready = false;
+    hipEvent_t fork = nullptr, join[kMaxAuxStreams] = {};
 };
 constexpr int kMaxDevices = 64;
-AuxStreams g_aux[kMaxDevices];
 std::mutex g_aux_mu;
-thread_local hipEvent_t t_fork_event[kMaxDevices] = {};
+std::map<std::pair<int, hipStream_t>, AuxStreams *> g_aux;
 }  // namespace
 
-static int aux_for_device(AuxStreams **out) {
+static int aux_for_stream(hipStream_t main, AuxStreams **out) {
     int dev = 0;
     CGM_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return set_error(CGM_ERR_INVALID, "device index too large");
     std::lock_guard<std::mutex> lk(g_aux_mu);
-    AuxStreams &a = g_aux[dev];
-    if (!a.ready) {
+    AuxStreams *&a = g_aux[{dev, main}];
+    if (!a) {
+        AuxStreams *n = new AuxStreams();
+        CGM_HIP(hipEventCreateWithFlags(&n->fork, hipEventDisableTiming));
         for (int i = 0; i < kMaxAuxStreams; ++i) {
-            CGM_HIP(hipStreamCreateWithFlags(&a.s[i], hipStreamNonBlocking));
-            CGM_HIP(hipEventCreateWithFlags(&a.join[i], hipEventDisableTiming));
+            CGM_HIP(hipStreamCreateWithFlags(&n->s[i], hipStreamNonBlocking));
+            CGM_HIP(hipEventCreateWithFlags(&n->join[i], hipEventDisableTiming));
         }
-        a.ready = true;
+        a = n;
     }
-    *out = &a;
+    *out = a;
     return CGM_OK;
 }
 
 int fork_streams(hipStream_t main, int n, hipStream_t *aux) {
     AuxStreams *a = nullptr;
-    if (int rc = aux_for_device(&a)) return rc;
-    int dev = 0;
-    CGM_HIP(hipGetDevice(&dev));
-    hipEvent_t &ev = t_fork_event[dev];
-    if (!ev) CGM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    CGM_HIP(hipEventRecord(ev, main));
+    if (int rc = aux_for_stream(main, &a)) return rc;
+    CGM_HIP(hipEventRecord(a->fork, main));
     for (int i = 0; i < n && i < kMaxAuxStreams; ++i) {
-        CGM_HIP(hipStreamWaitEvent(a->s[i], ev, 0));
+        CGM_HIP(hipStreamWaitEvent(a->s[i], a->fork, 0));
         aux[i] = a->s[i];
     }
     return CGM_OK;
@@ -209,7 +210,7 @@ int fork_streams(hipStream_t main, int n, hipStream_t *aux) {
 
 int join_streams(hipStream_t main, int n, hipStream_t *aux) {
     AuxStreams *a = nullptr;
-    if (int rc = aux_for_device(&a)) return rc;
+    if (int rc = aux_for_stream(main, &a)) return rc;
     for (int i = 0; i < n && i < kMaxAuxStreams; ++i) {
         CGM_HIP(hipEventRecord(a->join[i], aux[i]));
         CGM_HIP(hipStreamWaitEvent(main, a->join[i], 0));
@@ -414,6 +415,8 @@ int cgm_sw_banded_launch(const uint64_t *q_packed, const uint64_t *r_packed, con
             return launch_sw_auto(b, band, band_scalar, sc, out, (hipStream_t)stream);
         case CGM_LAYOUT_LANE:
             return launch_sw_buckets(b, band, band_scalar, sc, out, (hipStream_t)stream);
+        case CGM_LAYOUT_WAVE:
+            return launch_sw_wave(b, band, band_scalar, sc, out, (hipStream_t)stream);
         default:
             return set_error(CGM_ERR_INVALID, "unknown SW layout");
     }

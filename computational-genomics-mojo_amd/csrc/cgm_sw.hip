@@ -374,4 +374,24 @@ int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
     });
 }
 
+int launch_sw_wave(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                   SwOut out, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    if (int rc = check_sw_scoring(sc)) return rc;
+    if (b.n_pairs == 0) return CGM_OK;
+    const WfaPen pen = {4, 6, 2};  // unused: the batch holds no WFA pair
+    return with_cached_plan(stream, b.n_pairs, 0, pen, [&](MixPlan *P) {
+        // every non-empty pair counts as long for this call (the plan's
+        // tuning is restored before the lock is released)
+        const MixTuning saved = mix_plan_tuning(P);
+        MixTuning t = saved;
+        t.lane_max_len = 0;
+        mix_plan_set_tuning(P, t);
+        const int rc =
+            mix_enqueue(P, b, nullptr, 0, band, band_scalar, nullptr, 0, sc, pen, out, stream);
+        mix_plan_set_tuning(P, saved);
+        return rc;
+    });
+}
+
 }  // namespace cgm

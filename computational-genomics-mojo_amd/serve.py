@@ -10,12 +10,15 @@ from . import _lib
 
 STAT_NAMES = ["submitted", "completed", "refused", "pairs_completed", "elapsed_s", "pairs_per_s",
               "requests_per_s", "p50_ms", "p90_ms", "p99_ms", "max_ms", "mean_ms", "batches",
-              "mean_batch_pairs", "bad_results", "submit_phase_s"]
+              "mean_batch_pairs", "bad_results", "submit_phase_s", "sw_only_requests",
+              "sw_only_p50_ms", "sw_only_p99_ms", "with_wfa_requests", "with_wfa_p50_ms",
+              "with_wfa_p99_ms", "mean_sw_batch_pairs", "mean_wfa_batch_pairs"]
 
 
 class ServingEngine:
-    """n_slots preallocated batch slots within hbm_budget_bytes; one
-    dispatcher thread forms a batch whenever a slot is free."""
+    """Two bins (SW pairs, WFA pairs), each with n_slots preallocated batch
+    slots within hbm_budget_bytes and its own dispatcher thread, which
+    launches the bin's next batch whenever one of its slots is free."""
 
     def __init__(self, hbm_budget_bytes: int = 8 << 30, n_slots: int = 4,
                  max_batch_pairs: int = 16384, max_batch_words: int = 1 << 19,
@@ -36,8 +39,8 @@ class ServingEngine:
         qp = np.ascontiguousarray(q_packed, np.uint64)
         rp = np.ascontiguousarray(r_packed, np.uint64)
         rid = ctypes.c_int64()
-        _lib.call("cgm_engine_submit", self.handle, _lib.ptr(qp), _lib.ptr(rp),
-                  *[_lib.ptr(a) for a in arrs[:4]], int(arrs[0].size),
+        _lib.call("cgm_engine_submit", self.handle, _lib.ptr(qp), int(qp.size), _lib.ptr(rp),
+                  int(rp.size), *[_lib.ptr(a) for a in arrs[:4]], int(arrs[0].size),
                   *[_lib.ptr(a) for a in arrs[4:]], float(arrival_s), ctypes.byref(rid))
         return rid.value
 
@@ -69,7 +72,7 @@ class ServingEngine:
                  seed: int = 1) -> dict:
         """Native load generator (cgm_serve_run): Poisson arrivals at
         rate_rps (<= 0: closed loop with max_outstanding requests)."""
-        st = np.zeros(16, np.float64)
+        st = np.zeros(len(STAT_NAMES), np.float64)
         _lib.call("cgm_serve_run", self.handle, gen_handle, float(rate_rps), float(duration_s),
                   max_pairs_per_request, max_outstanding, seed, _lib.ptr(st))
         return dict(zip(STAT_NAMES, st.tolist()))

@@ -58,7 +58,9 @@ extern "C" {
                                CGM_LAYOUT_WAVE; longest pairs first */
 #define CGM_LAYOUT_PLAIN 1  /* one workgroup per pair (baseline) */
 #define CGM_LAYOUT_LANE 2   /* SW: one pair per lane, one kernel per band bucket */
-#define CGM_LAYOUT_WAVE 3   /* WFA: team layouts only: one workgroup per pair, wavefronts in
+#define CGM_LAYOUT_WAVE 3   /* SW: one wavefront per pair sweeping the band's anti-diagonals
+                               (effective bands <= 64; wider bands as CGM_LAYOUT_AUTO);
+                               WFA: team layouts only: one workgroup per pair, wavefronts in
                                LDS (or tiled over an HBM slab), longest pairs first */
 
 /* ------------------------------------------------------------ errors, info */
@@ -352,28 +354,37 @@ int cgm_plan_autotune(void *plan, const uint64_t *q_packed, const uint64_t *r_pa
 
 /* ------------------------------------------------------ serving engine
  * Continuous batching for a stream of alignment requests (no reference
- * counterpart; the reference aligns one pair per call).  The engine
- * preallocates n_slots batch slots (device buffers, pinned staging, one
- * mixed-kernel plan each) and fails with CGM_ERR_NOMEM if they exceed
- * hbm_budget_bytes.  A dispatcher thread launches a new batch whenever a
- * slot is free, taking every queued request that fits, so requests that
- * arrive while others run are admitted as slots free up and their kernels
- * overlap on the GPU.  cgm_engine_submit copies the request (pairs given as
- * in the batch layout: kind/band/max_error per pair) and returns at once;
- * it refuses with CGM_ERR_BUSY when the queued pairs would exceed
- * n_slots * max_batch_pairs.  arrival_s (cgm_engine_now() clock, <= 0 =
- * now) is the time latency is measured from.  Results do not depend on
- * which requests share a batch. */
+ * counterpart; the reference aligns one pair per call).  Pairs are batched
+ * by bin: SW pairs (kind 0) and WFA pairs (kind 1), so SW requests never wait behind
+ * long WFA pairs.  Each bin preallocates n_slots slots (a stream, device
+ * buffers and one mixed-kernel plan each) and 2 n_slots + 1 pinned staging
+ * buffers of max_batch_pairs pairs and 2 max_batch_words packed words;
+ * creation fails with CGM_ERR_NOMEM if the slots exceed hbm_budget_bytes.
+ * Each bin has its own dispatcher thread, which launches the next staging
+ * buffer whenever one of the bin's slots is free; a host function on the
+ * slot's stream returns the results, so requests that arrive while others
+ * run are admitted as slots free up and batches of both bins overlap on
+ * the GPU.  max_batch_words <= 2^29, max_batch_pairs <= 2^28.
+ *
+ * cgm_engine_submit copies the request's pairs (kind/band/max_error per
+ * pair, sequences at word offsets inside q_packed[0, q_words) and
+ * r_packed[0, r_words)) into the open staging buffers and returns at once;
+ * it refuses with CGM_ERR_INVALID a pair whose words lie outside those
+ * buffers or whose kind is not 0 or 1, and with CGM_ERR_BUSY a request for which a bin has no staging
+ * space left (back-pressure); a request is admitted whole or not at all.
+ * arrival_s (cgm_engine_now() clock, <= 0 = now) is the time latency is
+ * measured from.  Results do not depend on which requests share a batch;
+ * a batch whose launch fails reports CGM_BAD_PAIR scores and -1 ends. */
 int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_batch_pairs,
                       int64_t max_batch_words, int32_t max_error_cap, int32_t match,
                       int32_t mismatch, int32_t gap_open, int32_t gap_extend, int32_t x, int32_t o,
                       int32_t e, void **engine);
 int cgm_engine_destroy(void *engine);
-int cgm_engine_submit(void *engine, const uint64_t *q_packed, const uint64_t *r_packed,
-                      const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
-                      const int32_t *r_off, int32_t n_pairs, const int32_t *kind,
-                      const int32_t *band, const int32_t *max_error, double arrival_s,
-                      int64_t *request_id);
+int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
+                      const uint64_t *r_packed, int64_t r_words, const int32_t *q_len,
+                      const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                      int32_t n_pairs, const int32_t *kind, const int32_t *band,
+                      const int32_t *max_error, double arrival_s, int64_t *request_id);
 /* done = 1 once the request's results are ready. */
 int cgm_engine_poll(void *engine, int64_t request_id, int32_t *done);
 /* Blocks (timeout_ms < 0: forever) until the request is done, copies its
@@ -387,9 +398,11 @@ double cgm_engine_now(void);
 /* Load generator: requests of U{1..max_pairs_per_request} consecutive pairs
  * of a generated batch (cgm_gen_create), Poisson arrivals at rate_rps for
  * duration_s (rate_rps <= 0: closed loop with max_outstanding requests in
- * flight).  stats[16]: submitted, completed, refused, pairs completed,
+ * flight).  stats[24]: submitted, completed, refused, pairs completed,
  * elapsed s, pairs/s, requests/s, p50, p90, p99, max, mean latency (ms),
- * batches, mean pairs per batch, bad results, submit phase s. */
+ * batches, mean pairs per batch, bad results, submit phase s, then
+ * SW-only requests: count, p50, p99 (ms); requests with a WFA pair: count,
+ * p50, p99 (ms); mean pairs per SW batch, per WFA batch. */
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                   int32_t max_pairs_per_request, int32_t max_outstanding, uint64_t seed,
                   double *stats);

@@ -114,6 +114,8 @@ TUNINGS = [
     {"lane_max_len": 1024, "tile_steps": 16, "lds_bytes": 24 * 1024},
     {"sw_dual": 0},
     {"sw_dual": 1, "blocks_per_cu": 1, "lds_bytes": 96 * 1024},
+    {"lane_max_len": 1},     # every SW pair of band <= 64 on one wavefront per pair
+    {"lane_max_len": 200},
 ]
 
 

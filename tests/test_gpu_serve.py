@@ -99,6 +99,25 @@ def test_admission_back_pressure(cgm, dev, serving_pairs):
     eng.close()
 
 
+def test_submit_checks_buffers(cgm, dev, serving_pairs):
+    """A pair whose words lie outside the caller's buffers is refused before
+    anything is copied; a valid request still runs afterwards."""
+    w = serving_pairs
+    eng = cgm.ServingEngine(hbm_budget_bytes=4 << 30, n_slots=1, max_batch_pairs=256,
+                            max_batch_words=1 << 15, max_error_cap=int(w.max_error.max()))
+    idx = np.arange(0, 8)
+    qp = w.q_packed[: int(w.q_off[idx].max())]   # too short for the last pair
+    with pytest.raises(cgm._lib.CgmError) as ei:
+        eng.submit(qp, w.r_packed, w.q_len[idx], w.r_len[idx], w.q_off[idx], w.r_off[idx],
+                   w.kind[idx], w.band[idx], w.max_error[idx])
+    assert ei.value.code == cgm._lib.CGM_ERR_INVALID
+    rid = eng.submit_pairs(w, idx)
+    s = eng.wait(rid, idx.size)[0]
+    assert (s != cgm._lib.CGM_BAD_PAIR).all()
+    assert eng.stats()["done_requests"] == 1
+    eng.close()
+
+
 def test_engine_respects_hbm_budget(cgm, dev):
     with pytest.raises(cgm._lib.CgmError) as ei:
         cgm.ServingEngine(hbm_budget_bytes=1 << 20, n_slots=2, max_batch_pairs=4096,
@@ -117,5 +136,9 @@ def test_native_load_generator(cgm, dev):
     print(f"[case] open loop 2000 rps {opened}", flush=True)
     assert opened["completed"] > 0 and opened["bad_results"] == 0
     assert 0 < opened["p50_ms"] <= opened["p99_ms"]
+    # per-bin latency: SW-only requests and requests holding a WFA pair
+    assert opened["sw_only_requests"] + opened["with_wfa_requests"] == opened["completed"]
+    assert opened["sw_only_requests"] > 0 and 0 < opened["sw_only_p50_ms"] <= opened["sw_only_p99_ms"]
+    assert opened["mean_sw_batch_pairs"] > 0
     g.close()
     eng.close()

@@ -11,7 +11,7 @@ from gpu_helpers import (CONFIG_SW, DEFAULT_SW, Uploaded, assert_same, sw_gpu, s
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = {"plain": 1, "lane": 2, "auto": 0}
+LAYOUTS = {"plain": 1, "lane": 2, "auto": 0, "wave": 3}
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "alignment_vectors.json")
 
 
@@ -72,7 +72,7 @@ def test_sw_200k_full_size(cgm, dev, orc, gen_mod):
     up = Uploaded(dev, w)
     print("[case] sw full 200000 pairs x 100-300 bp", flush=True)
     ref = sw_gpu(cgm, dev, up, LAYOUTS["plain"])
-    for name in ("lane", "auto"):
+    for name in ("lane", "auto", "wave"):
         assert_same(sw_gpu(cgm, dev, up, LAYOUTS[name]), ref, f"sw 200k full {name} vs plain")
     idx = np.random.default_rng(0).choice(w.n_pairs, 2000, replace=False)
     sub = w.subset(idx)
@@ -215,3 +215,39 @@ def test_sw_dual_lanes_int16_boundary(cgm, dev, orc):
     scores, q_end = got[0], got[1]
     assert scores[0] == 32764 and q_end[0] == 16381 and scores[1] == 32766
     plan.close()
+
+
+def test_sw_wave_long_pairs(cgm, dev, orc):
+    """Reads of 300 bp to 6 kb with planted substitutions, indels and
+    flanks, at bands 0-64 (64: the extra diagonal) and bands beyond the
+    layout's limit (placed as the automatic layout places them)."""
+    import random
+    rng = random.Random(31)
+    pairs, bands = [], []
+    for i in range(240):
+        L = rng.randint(300, 6000)
+        a = "".join(rng.choice("ACGT") for _ in range(L))
+        b = []
+        for c in a:
+            u = rng.random()
+            if u < 0.02:
+                b.append(rng.choice("ACGT"))
+            elif u < 0.025:
+                continue
+            elif u < 0.03:
+                b.append(c + rng.choice("ACGT"))
+            else:
+                b.append(c)
+        b = "".join(b)
+        if i % 5 == 1:
+            b = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 80))) + b
+        if i % 7 == 2:
+            b = b[: len(b) * 3 // 4]
+        pairs.append((a, b) if i % 2 else (b, a))
+        bands.append(rng.choice([0, 1, 2, 31, 32, 33, 63, 64, 64, 70, rng.randint(0, 64)]))
+    w = workload_from_strings(pairs, band=bands)
+    up = Uploaded(dev, w)
+    for sc in (CONFIG_SW, DEFAULT_SW):
+        print(f"[case] sw wave long pairs scoring={sc}", flush=True)
+        assert_same(sw_gpu(cgm, dev, up, LAYOUTS["wave"], scoring=sc), sw_oracle(orc, w, sc),
+                    f"sw wave long pairs {sc}")
