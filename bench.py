@@ -335,11 +335,12 @@ def serving_extra(m, duration=3.0):
     all requests, over SW-only requests and over requests holding a WFA
     pair), then the 50,000 requests/s arrival rate of the workload
     definition (an overload: admission and back-pressure)."""
-    eng = m.ServingEngine(hbm_budget_bytes=16 << 30, n_slots=4, max_batch_pairs=16384,
+    eng = m.ServingEngine(hbm_budget_bytes=16 << 30, n_slots=2, max_batch_pairs=16384,
                           max_batch_words=1 << 19, max_error_cap=8192)
     g = m.GeneratorHandle(5, 400_000, 11)
     res = {"hbm_budget_bytes": 16 << 30, "device_bytes": eng.stats()["device_bytes"],
-           "bins": "SW pairs and WFA pairs batched apart, 4 slots and a dispatcher thread each"}
+           "bins": "SW pairs and WFA pairs batched apart, 2 slots (streams) and a dispatcher "
+                   "thread each: one hardware queue per slot"}
     eng.run_load(g.handle, 0.0, 0.5, max_outstanding=4096)  # warm-up
     peak = eng.run_load(g.handle, 0.0, duration, max_outstanding=4096)
     res["peak_closed_loop"] = peak

@@ -51,6 +51,7 @@ double now_s() {
 
 constexpr int kBins = 2;   // 0: SW, 1: WFA (the pair kind)
 constexpr int kMeta = 7;   // per pair: q_len, r_len, q_off, r_off, kind, band, max_error
+constexpr int kEngineLaneMaxLen = 400;  // SW reads longer than this: one wavefront per pair
 
 struct Request {
     int64_t id = 0;
@@ -334,6 +335,17 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
             // SW slots carry no WFA workspace
             int rc = mix_plan_create(max_batch_pairs, bi == 1 ? max_error_cap : 0, pen, 0, 0, &s.plan);
             if (rc) return fail(rc);
+            // every kernel on the slot's own stream: the process's streams
+            // share GPU_MAX_HW_QUEUES hardware queues (4 by default), and a
+            // short SW batch must not queue behind a long WFA one
+            mix_plan_set_fork(s.plan, false);
+            if (bi == 0) {
+                // a batch lasts as long as its longest pair: reads beyond
+                // kEngineLaneMaxLen take one wavefront each instead of a lane
+                MixTuning t = mix_plan_tuning(s.plan);
+                t.lane_max_len = kEngineLaneMaxLen;
+                mix_plan_set_tuning(s.plan, t);
+            }
             E->device_bytes += mix_plan_bytes(s.plan) + wcap * 8 + (2 * kMeta + 3) * cap * 4;
             if (E->device_bytes > hbm_budget_bytes)
                 return fail(set_error(CGM_ERR_NOMEM, "engine: slots exceed the HBM budget"));

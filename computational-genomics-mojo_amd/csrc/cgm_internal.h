@@ -152,6 +152,8 @@ MixTuning mix_default_tuning();
 void mix_set_default_tuning(const MixTuning &t);
 MixTuning mix_plan_tuning(const MixPlan *P);
 void mix_plan_set_tuning(MixPlan *P, const MixTuning &t);
+// false: every kernel of an enqueue on the caller's stream (no forked streams)
+void mix_plan_set_fork(MixPlan *P, bool fork);
 // Measures candidate tunings on one batch (coordinate descent over the
 // fields, `reps` timed enqueues each, best kept), leaves the best in the
 // plan and returns it with its time and the starting tuning's time.

@@ -337,6 +337,7 @@ struct MixPlan {
     hipGraphExec_t exec = nullptr;
     int64_t bytes = 0;  // device memory held by the plan
     bool had_wfa = false;  // the last enqueue launched the WFA kernels
+    bool fork = true;      // WFA and long-SW kernels on streams forked from the caller's
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // around the SW persistent kernel
     hipEvent_t ev_wfa_start = nullptr, ev_wfa_stop = nullptr;  // around the WFA kernels
     hipEvent_t ev_wfa_mid = nullptr;  // end of the second WFA stream (no timing)
@@ -410,6 +411,7 @@ void mix_set_default_tuning(const MixTuning &t) {
 MixTuning mix_plan_tuning(const MixPlan *P) { return P->tune; }
 
 void mix_plan_set_tuning(MixPlan *P, const MixTuning &t) { P->tune = t; }
+void mix_plan_set_fork(MixPlan *P, bool fork) { P->fork = fork; }
 
 // waves_per_cu / lds_bytes > 0 override the default tuning's shape.
 int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per_cu,
@@ -598,7 +600,8 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     // kernel, since their serial chains are the batch's longest
     const bool has_wave = tune.lane_max_len < kWaveMaxLen && (kind != nullptr || kind_scalar == 0);
     hipStream_t aux[3] = {stream, stream, stream};
-    if (has_wfa || has_wave)
+    const bool forked = (has_wfa || has_wave) && P->fork;
+    if (forked)
         if (int rc = fork_streams(stream, 3, aux)) return rc;
     if (has_wave)
         if (int rc = launch_sw_wave_range(b, band, band_scalar, sc, out, P->order,
@@ -640,7 +643,7 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     if (int rc = launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
                                        ranges + 2 * kMixSwPlain, P->cus * 2, stream))
         return rc;
-    return has_wfa || has_wave ? join_streams(stream, 3, aux) : CGM_OK;
+    return forked ? join_streams(stream, 3, aux) : CGM_OK;
 }
 
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
