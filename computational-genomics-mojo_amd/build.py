@@ -21,7 +21,7 @@ ARCH = "gfx950"
 HIP_SOURCES = ["cgm_runtime.hip", "cgm_sw.hip", "cgm_sw_lane.hip", "cgm_wfa.hip", "cgm_sort.hip",
                "cgm_mixed.hip", "cgm_engine.hip", "cgm_encode.hip",
                "cgm_cigar.hip", "cgm_nw.hip", "cgm_sw_start.hip",
-               "cgm_noalloc.hip", "cgm_sw_wave.hip"]
+               "cgm_noalloc.hip"]
 CXX_SOURCES = ["cgm_gen.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]

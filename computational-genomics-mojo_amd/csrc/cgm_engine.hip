@@ -102,6 +102,7 @@ struct Bin {
     std::deque<Staging *> ready;
     Staging *open = nullptr;
     int64_t pending_pairs = 0;
+    MixTuning tune{};  // applied to the slot's plan at every launch
     std::atomic<int64_t> batches{0}, batch_pairs{0};
     std::thread dispatcher;
 };
@@ -173,8 +174,9 @@ struct Engine {
 
     // Launches staging `st` on slot `si` of bin B; on any failure the batch's
     // pairs are reported as refused (score CGM_BAD_PAIR, ends -1).
-    void launch(Bin &B, int si, Staging *st) {
+    void launch(Bin &B, int si, Staging *st, const MixTuning &tune) {
         Slot &S = B.slots[(size_t)si];
+        mix_plan_set_tuning(S.plan, tune);
         hipStream_t s = S.stream;
         const int64_t p = st->n, w = st->words, cap = cap_pairs;
         st->slot = si;
@@ -213,6 +215,7 @@ struct Engine {
         for (;;) {
             Staging *st = nullptr;
             int si = -1;
+            MixTuning tune;
             {
                 std::unique_lock<std::mutex> lk(B.mu);
                 B.cv.wait(lk, [&] {
@@ -233,8 +236,9 @@ struct Engine {
                 B.pending_pairs -= st->n;
                 si = B.free_slots.back();
                 B.free_slots.pop_back();
+                tune = B.tune;
             }
-            launch(B, si, st);
+            launch(B, si, st, tune);
         }
     }
 };
@@ -339,12 +343,11 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
             // share GPU_MAX_HW_QUEUES hardware queues (4 by default), and a
             // short SW batch must not queue behind a long WFA one
             mix_plan_set_fork(s.plan, false);
-            if (bi == 0) {
-                // a batch lasts as long as its longest pair: reads beyond
+            if (k == 0) {
+                B.tune = mix_plan_tuning(s.plan);
+                // a batch lasts as long as its longest pair: SW reads beyond
                 // kEngineLaneMaxLen take one wavefront each instead of a lane
-                MixTuning t = mix_plan_tuning(s.plan);
-                t.lane_max_len = kEngineLaneMaxLen;
-                mix_plan_set_tuning(s.plan, t);
+                if (bi == 0) B.tune.lane_max_len = kEngineLaneMaxLen;
             }
             E->device_bytes += mix_plan_bytes(s.plan) + wcap * 8 + (2 * kMeta + 3) * cap * 4;
             if (E->device_bytes > hbm_budget_bytes)
@@ -470,10 +473,15 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
         }
         B.pending_pairs += nb[bi];
     }
+    // the dispatcher only acts when a slot is free; when none is, the slot's
+    // completion wakes it, and a wake-up here would cost a futex round trip
+    // per request for nothing
+    bool wake[kBins];
+    for (int bi = 0; bi < kBins; ++bi) wake[bi] = nb[bi] > 0 && !E->bins[bi].free_slots.empty();
     l1.unlock();
     l0.unlock();
     for (int bi = 0; bi < kBins; ++bi)
-        if (nb[bi] > 0) E->bins[bi].cv.notify_one();
+        if (wake[bi]) E->bins[bi].cv.notify_one();
     *request_id = r->id;
     return CGM_OK;
 }
@@ -508,6 +516,28 @@ int cgm_engine_wait(void *engine, int64_t request_id, int32_t timeout_ms, int32_
     if (latency_ms) *latency_ms = (r->t_done - r->t_arrival) * 1e3;
     E->live.erase(it);
     E->pool.push_back(r);
+    return CGM_OK;
+}
+
+int cgm_engine_get_tuning(void *engine, int32_t bin, int32_t *params) {
+    Engine *E = (Engine *)engine;
+    if (!E || !params || bin < 0 || bin >= kBins)
+        return set_error(CGM_ERR_INVALID, "cgm_engine_get_tuning: bad arguments");
+    Bin &B = E->bins[bin];
+    std::lock_guard<std::mutex> lk(B.mu);
+    tuning_to_array(B.tune, params);
+    return CGM_OK;
+}
+
+int cgm_engine_set_tuning(void *engine, int32_t bin, const int32_t *params) {
+    Engine *E = (Engine *)engine;
+    if (!E || !params || bin < 0 || bin >= kBins)
+        return set_error(CGM_ERR_INVALID, "cgm_engine_set_tuning: bad arguments");
+    MixTuning t;
+    if (int rc = tuning_from_array(params, &t)) return rc;
+    Bin &B = E->bins[bin];
+    std::lock_guard<std::mutex> lk(B.mu);
+    B.tune = t;
     return CGM_OK;
 }
 

@@ -248,14 +248,11 @@ int launch_sw_lds_lane(const Batch &b, const int32_t *band, int32_t band_scalar,
 int launch_sw_plain_range(const Batch &b, const int32_t *band, int32_t band_scalar,
                           SwScoring sc, SwOut out, const int32_t *order, const int32_t *range,
                           int grid, hipStream_t stream);
-// One wavefront per pair (cgm_sw_wave.hip): pairs order[range[0] ..
-// range[1]) with effective band <= kWaveMaxBand, both lengths below
-// kWaveMaxLen and match * min(qlen, rlen) < 2^23; *queue must be 0.
+// One wavefront per pair (cgm_sw_wave_dev.h, run inside the SW persistent
+// kernel): pairs with effective band <= kWaveMaxBand, both lengths below
+// kWaveMaxLen and match * min(qlen, rlen) < 2^23.
 constexpr int kWaveMaxBand = 64;
 constexpr int kWaveMaxLen = 1 << 20;
-int launch_sw_wave_range(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
-                         SwOut out, const int32_t *order, const int32_t *range, int32_t *queue,
-                         int grid, hipStream_t stream);
 int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                    SwOut out, hipStream_t stream);
 // The automatic layout with every SW pair of effective band <= 64 on the

@@ -7,11 +7,12 @@
 // Pipeline (all device work; WFA pairs on a forked stream):
 //  1. classify each pair (kind, effective band, lengths, max_error) into
 //     0 SW register lanes (band <= reg_max_band <= 64), 1 SW packed register
-//     lanes (band 128), 2 WFA, 3 SW LDS lanes, 4 SW one workgroup per pair
-//     (also SW pairs longer than lane_max_len), 5 bad; WFA pairs also get
-//     the WFA launcher's layout class (register layouts by width, team
-//     layouts beyond); bad pairs get their outputs written here, WFA pairs
-//     their ends (-1);
+//     lanes (band 128), 2 WFA, 3 SW LDS lanes, 4 SW one workgroup per pair,
+//     5 bad, 6 SW one wavefront per pair (pairs longer than lane_max_len
+//     with bands <= 64; longer ones with wider bands go to class 4); WFA
+//     pairs also get the WFA launcher's layout class (register layouts by
+//     width, team layouts beyond); bad pairs get their outputs written here,
+//     WFA pairs their ends (-1);
 //  2. radix-sort pair indices by (class, band, length) or (class, WFA
 //     layout class, WFA work);
 //  3. the WFA pairs run the WFA launcher's class kernels over their ranges
@@ -20,9 +21,10 @@
 //  4. build SW tasks: 256 consecutive SW pairs of one register class per
 //     task, each with an estimated cost, and radix-sort them largest first;
 //  5. the SW persistent kernel: CUs x blocks_per_cu four-wavefront
-//     workgroups whose wavefronts take 64-pair units from a long and a short
-//     queue; it is launched after the WFA kernels so that its workgroups
-//     fill the CUs the WFA tail leaves idle;
+//     workgroups whose wavefronts first take the class-6 pairs one
+//     wavefront each, longest first, then 64-pair units from a long and a
+//     short queue; it is launched after the WFA kernels so that its
+//     workgroups fill the CUs the WFA tail leaves idle;
 //  6. classes 3 and 4 (unusual scorings, bands or lengths) run the LDS-lane
 //     and one-workgroup-per-pair kernels over their ranges;
 //  7. the caller's stream joins the forked one.
@@ -36,6 +38,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "cgm_sw_dev.h"
+#include "cgm_sw_wave_dev.h"
 #include "cgm_wfa_reg.h"
 
 namespace cgm {
@@ -275,6 +278,34 @@ mix_persistent_kernel(Batch b, const int32_t *band_arr, int32_t band_scalar, SwS
     const int lane = threadIdx.x & 63;
     const uint32_t units = (uint32_t)total * kMixWaves;
     unsigned long long *ends = (unsigned long long *)(small + kSmallUnits);
+    // SW pairs in the wavefront-per-pair layout (reads longer than the
+    // tuning's lane_max_len, sorted longest first): each is one serial chain
+    // of 2 (qlen + rlen) steps, so every wavefront takes them before any
+    // 64-pair unit, the longest first
+    {
+        const int32_t w0 = small[kSmallRanges + 2 * kMixSwWave];
+        const int32_t w1 = small[kSmallRanges + 2 * kMixSwWave + 1];
+        if (w1 > w0) {
+            for (;;) {
+                int item = 0;
+                if (lane == 0) item = atomicAdd(small + kSmallWaveQueue, 1);
+                item = __builtin_amdgcn_readfirstlane(item);
+                if (item >= w1 - w0) break;
+                const int64_t pair = order[w0 + item];
+                const uint64_t *q, *r;
+                int ql, rl;
+                load_pair(b, pair, q, r, ql, rl);  // checked by the classifier
+                const int bd = band_arr ? band_arr[pair] : band_scalar;
+                int score, qe, re;
+                swwave::sw_wave_pair(q, ql, r, rl, effective_band(bd, ql, rl), sc, score, qe, re);
+                if (lane == 0) {
+                    out.score[pair] = score;
+                    if (out.q_end) out.q_end[pair] = qe;
+                    if (out.r_end) out.r_end[pair] = re;
+                }
+            }
+        }
+    }
     int slot = 0;
     if (lane == 0) {
         const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_ID
@@ -595,19 +626,10 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     // launched first so that the SW persistent kernel fills the CUs their
     // tails free
     const bool has_wfa = kind != nullptr || kind_scalar == 1;
-    // long SW pairs (beyond lane_max_len, bands <= 64): one wavefront per
-    // pair on a third forked stream, also launched before the persistent
-    // kernel, since their serial chains are the batch's longest
-    const bool has_wave = tune.lane_max_len < kWaveMaxLen && (kind != nullptr || kind_scalar == 0);
-    hipStream_t aux[3] = {stream, stream, stream};
-    const bool forked = (has_wfa || has_wave) && P->fork;
+    hipStream_t aux[2] = {stream, stream};
+    const bool forked = has_wfa && P->fork;
     if (forked)
-        if (int rc = fork_streams(stream, 3, aux)) return rc;
-    if (has_wave)
-        if (int rc = launch_sw_wave_range(b, band, band_scalar, sc, out, P->order,
-                                          ranges + 2 * kMixSwWave, P->small + kSmallWaveQueue,
-                                          P->cus * 4, aux[2]))
-            return rc;
+        if (int rc = fork_streams(stream, 2, aux)) return rc;
     if (has_wfa) {
         CGM_HIP(hipEventRecord(P->ev_wfa_start, aux[0]));
         if (int rc = launch_wfa_classes(b, me, me_scalar, P->me_cap, pen, true, out.score, P->order,
@@ -643,7 +665,7 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     if (int rc = launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
                                        ranges + 2 * kMixSwPlain, P->cus * 2, stream))
         return rc;
-    return forked ? join_streams(stream, 3, aux) : CGM_OK;
+    return forked ? join_streams(stream, 2, aux) : CGM_OK;
 }
 
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,

@@ -67,6 +67,24 @@ class ServingEngine:
                 "queued_pairs", "device_bytes", "worker_error"]
         return dict(zip(keys, out.tolist()))
 
+    def get_tuning(self, bin: int) -> dict:
+        """Tuning of bin 0 (SW pairs) or 1 (WFA pairs), CGM_TUNE_* fields."""
+        a = np.zeros(len(_lib.TUNE_FIELDS), np.int32)
+        _lib.call("cgm_engine_get_tuning", self.handle, bin, _lib.ptr(a))
+        return dict(zip(_lib.TUNE_FIELDS, (int(v) for v in a)))
+
+    def set_tuning(self, bin: int, **fields) -> dict:
+        """Changes the given fields of a bin's tuning (from its next launch)
+        and returns the new tuning."""
+        t = self.get_tuning(bin)
+        for k, v in fields.items():
+            if k not in t:
+                raise KeyError(k)
+            t[k] = int(v)
+        a = np.array([t[k] for k in _lib.TUNE_FIELDS], np.int32)
+        _lib.call("cgm_engine_set_tuning", self.handle, bin, _lib.ptr(a))
+        return t
+
     def run_load(self, gen_handle, rate_rps: float, duration_s: float,
                  max_pairs_per_request: int = 64, max_outstanding: int = 512,
                  seed: int = 1) -> dict:

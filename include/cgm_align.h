@@ -391,6 +391,14 @@ int cgm_engine_poll(void *engine, int64_t request_id, int32_t *done);
  * n_pairs results (any pointer may be NULL) and releases the request. */
 int cgm_engine_wait(void *engine, int64_t request_id, int32_t timeout_ms, int32_t *score,
                     int32_t *q_end, int32_t *r_end, double *latency_ms);
+/* Tuning of one bin's batches (bin 0: SW pairs, 1: WFA pairs) as a
+ * CGM_TUNE_N array (fields as for cgm_plan_get_tuning); a new value applies
+ * from the bin's next launch.  Bin 0 starts from the process default with
+ * LANE_MAX_LEN = 400: SW reads longer than that run one wavefront per pair
+ * inside the persistent kernel instead of one lane, which shortens a small
+ * batch's longest serial chain. */
+int cgm_engine_get_tuning(void *engine, int32_t bin, int32_t *params);
+int cgm_engine_set_tuning(void *engine, int32_t bin, const int32_t *params);
 /* out[8]: batches, batched pairs, finished requests, finished pairs,
  * refused requests, queued pairs, device bytes held, last worker error. */
 int cgm_engine_stats(void *engine, int64_t *out);

@@ -57,6 +57,29 @@ def test_engine_results_equal_oracle(cgm, dev, orc, serving_pairs):
     eng.close()
 
 
+@pytest.mark.parametrize("lane_max_len", [1, 150, 1 << 30])
+def test_engine_bin_tuning_keeps_results(cgm, dev, orc, serving_pairs, lane_max_len):
+    """Bin 0's lane/wavefront switch-over moves SW reads between the
+    one-lane and the one-wavefront layouts; results stay the oracle's."""
+    w = serving_pairs
+    eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=2, max_batch_pairs=2048,
+                            max_batch_words=1 << 17, max_error_cap=int(w.max_error.max()))
+    assert eng.get_tuning(0)["lane_max_len"] == 400
+    t = eng.set_tuning(0, lane_max_len=lane_max_len)
+    assert eng.get_tuning(0) == t and t["lane_max_len"] == lane_max_len
+    with pytest.raises(cgm._lib.CgmError):
+        eng.set_tuning(0, sw_dual=5)
+    reqs = requests_of(w.n_pairs, np.random.default_rng(1))
+    ids = [eng.submit_pairs(w, idx) for idx in reqs]
+    got = [np.zeros(w.n_pairs, np.int32) for _ in range(3)]
+    for idx, rid in zip(reqs, ids):
+        s, q, r, _ = eng.wait(rid, idx.size)
+        got[0][idx], got[1][idx], got[2][idx] = s, q, r
+    print(f"[case] engine lane_max_len {lane_max_len}: {len(reqs)} requests", flush=True)
+    assert_same(tuple(got), oracle(orc, w), f"engine lane_max_len {lane_max_len} vs oracle")
+    eng.close()
+
+
 def test_request_alone_equals_in_mix(cgm, dev, serving_pairs):
     w = serving_pairs
     eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=4, max_batch_pairs=4096,

@@ -1,6 +1,6 @@
 """CPU model of the wavefront-per-pair SW layout, checked against the oracle.
 
-The layout (cgm_sw_wave.hip in the package's csrc/) gives one 64-lane
+The layout (cgm_sw_wave_dev.h in the package's csrc/) gives one 64-lane
 wavefront to one pair of effective band B <= 64 and sweeps the band's
 anti-diagonals a = qi + ri.  Band diagonal d = ri - qi (|d| <= B) has a cell
 on anti-diagonal a when d = a (mod 2), so lane l owns the two diagonals
@@ -14,7 +14,7 @@ neighbour's (one lane down for de's left, one lane up for do's up).  The
 band's outer edges read the reference's values (left of d = -B: H 0 and no
 E; above d = B: H 0; with penalties <= 0 that missing E term never decides
 a cell).  With B = 64 the diagonal d = 64 has no lane; its cell
-is computed beside lane 63 (the kernel does it in scalar registers).
+is computed beside lane 63 (in a third register set of every lane).
 Bases reach the lanes by the same shifts: on every step of a parity the
 query bases move one lane up and the reference bases one lane down, with
 one new base entering at each end.  Cells outside the matrix hold H = 0 and
@@ -198,3 +198,84 @@ def test_planted_wave_mistakes_fail(mutant):
         if got is not None and got != O.sw_banded(q, r, *sc, band=band):
             bad += 1
     assert bad > 0, mutant
+
+
+def wave_feed_model(q, r, band, lanes=64):
+    """The base feeds of the wavefront layout, restated register by register:
+    yields, per step (anti-diagonals a and a + 1), the query base entering
+    lane 0 on the odd and on the even step, the reference base entering lane
+    lanes - 1, and (band = lanes) the d = band cell's query base (lane
+    lanes - 1's even-step base before its shift) and reference base (lane
+    lanes - 1 of the reference feed after it moved)."""
+    L = lanes
+    ql, rl = len(q), len(r)
+    B = min(band, max(ql, rl))
+    codes = {"A": 0, "C": 1, "G": 2, "T": 3}
+
+    def b0(s, p):
+        return codes[s[p]] if 0 <= p < len(s) else 0
+
+    a_first = (B & 1) - 1
+    a_end = (ql - 1) + min(rl - 1, ql - 1 + B)
+    c1_0, c2_0 = (a_first + B - 1) // 2, (a_first - B + 1) // 2
+    pq = [c1_0 + 1 + l for l in range(L)]
+    pr = [c2_0 + (L - 1) + (L - 1) - l for l in range(L)]
+    fq = [b0(q, p) for p in pq]
+    fr = [b0(r, p) for p in pr]
+    nq = [p + L for p in pq]  # positions whose words are in flight
+    nr = [p + L for p in pr]
+    qlast = [b0(q, c1_0)] * L
+    qbe = [b0(q, (a_first - 1 + B) // 2 - l) for l in range(L)]
+    fill = 0
+    a = a_first
+    while a <= a_end:
+        q_odd = qlast[0]
+        q_even = fq[0]
+        r_new = fr[L - 1]
+        qx = qbe[L - 1]
+        qbe = [q_even] + qbe[:-1]
+        qlast = fq[:]
+        fq = fq[1:] + [0]       # lane l <- lane l + 1
+        fr = [0] + fr[:-1]      # lane l <- lane l - 1
+        fill += 1
+        if fill == L:
+            fill = 0
+            pq = [p + L for p in pq]
+            pr = [p + L for p in pr]
+            assert pq == nq and pr == nr
+            fq = [b0(q, p) for p in pq]
+            fr = [b0(r, p) for p in pr]
+            nq = [p + L for p in pq]
+            nr = [p + L for p in pr]
+        yield a, q_odd, q_even, r_new, qx, fr[L - 1]
+        a += 2
+
+
+def test_wave_feeds_deliver_the_direct_bases():
+    """The feed registers hand each step exactly the bases the layout reads
+    directly (query (a + B - 1) / 2 on the odd step, (a + 1 + B) / 2 on the
+    even one, reference (a - B + 1) / 2 + 63 on both), across refills, and
+    the d = 64 cell gets query (a + 1 - 64) / 2 and reference (a + 1 + 64) / 2."""
+    rng = random.Random(14)
+    codes = {"A": 0, "C": 1, "G": 2, "T": 3}
+    refilled = 0
+    for _ in range(60):
+        q = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 400)))
+        r = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 400)))
+        band = rng.choice([0, 1, 17, 63, 64, rng.randint(0, 64)])
+        B = min(band, max(len(q), len(r)))
+
+        def b0(s, p):
+            return codes[s[p]] if 0 <= p < len(s) else 0
+
+        steps = 0
+        for a, q_odd, q_even, r_new, qx, rx in wave_feed_model(q, r, band):
+            steps += 1
+            assert q_odd == b0(q, (a + B - 1) // 2), (q, r, band, a)
+            assert q_even == b0(q, (a + 1 + B) // 2), (q, r, band, a)
+            assert r_new == b0(r, (a - B + 1) // 2 + 63), (q, r, band, a)
+            if B == 64:
+                assert qx == b0(q, (a + 1 - 64) // 2), (q, r, a)
+                assert rx == b0(r, (a + 1 + 64) // 2), (q, r, a)
+        refilled += steps > 128
+    assert refilled > 10  # cases that crossed at least two refills
