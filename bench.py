@@ -335,14 +335,17 @@ def serving_extra(m, duration=3.0):
     all requests, over SW-only requests and over requests holding a WFA
     pair), then the 50,000 requests/s arrival rate of the workload
     definition (an overload: admission and back-pressure)."""
-    eng = m.ServingEngine(hbm_budget_bytes=16 << 30, n_slots=2, max_batch_pairs=16384,
+    eng = m.ServingEngine(hbm_budget_bytes=16 << 30, n_slots=2, max_batch_pairs=65536,
                           max_batch_words=1 << 19, max_error_cap=8192)
     g = m.GeneratorHandle(5, 400_000, 11)
     res = {"hbm_budget_bytes": 16 << 30, "device_bytes": eng.stats()["device_bytes"],
            "bins": "SW pairs and WFA pairs batched apart, 2 slots (streams) and a dispatcher "
-                   "thread each: one hardware queue per slot"}
-    eng.run_load(g.handle, 0.0, 0.5, max_outstanding=4096)  # warm-up
-    peak = eng.run_load(g.handle, 0.0, duration, max_outstanding=4096)
+                   "thread each: one hardware queue per slot",
+           "sw_bin_tuning": eng.get_tuning(0),
+           "client": "one native client thread; closed loop with up to 16,384 requests "
+                     "in flight for the peak, Poisson arrivals otherwise"}
+    eng.run_load(g.handle, 0.0, 0.5, max_outstanding=16384)  # warm-up
+    peak = eng.run_load(g.handle, 0.0, duration, max_outstanding=16384)
     res["peak_closed_loop"] = peak
     for name, frac in (("at_70pct_peak", 0.7), ("at_10pct_peak", 0.1)):
         rate = frac * peak["requests_per_s"]

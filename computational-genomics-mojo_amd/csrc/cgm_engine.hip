@@ -51,7 +51,7 @@ double now_s() {
 
 constexpr int kBins = 2;   // 0: SW, 1: WFA (the pair kind)
 constexpr int kMeta = 7;   // per pair: q_len, r_len, q_off, r_off, kind, band, max_error
-constexpr int kEngineLaneMaxLen = 400;  // SW reads longer than this: one wavefront per pair
+constexpr int kEngineLaneMaxLen = 200;  // SW reads longer than this: one wavefront per pair
 
 struct Request {
     int64_t id = 0;
@@ -78,6 +78,7 @@ struct Staging {
     int32_t *h_out = nullptr;
     int64_t n = 0, words = 0;
     std::vector<Route> routes;
+    int writers = 0;  // submits copying into their reserved part (under the bin's lock)
     Bin *bin = nullptr;
     int slot = -1;
     int rc = CGM_OK;
@@ -218,9 +219,13 @@ struct Engine {
             MixTuning tune;
             {
                 std::unique_lock<std::mutex> lk(B.mu);
+                // the oldest closed staging buffer, else the open one; either
+                // only once every submit writing into it has finished
                 B.cv.wait(lk, [&] {
-                    return stop.load() || (!B.free_slots.empty() &&
-                                           (!B.ready.empty() || B.open->n > 0));
+                    if (stop.load()) return true;
+                    if (B.free_slots.empty()) return false;
+                    const Staging *c = !B.ready.empty() ? B.ready.front() : B.open;
+                    return c->n > 0 && c->writers == 0;
                 });
                 if (stop.load()) return;
                 if (!B.ready.empty()) {
@@ -322,6 +327,7 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
     };
     if (hipGetDevice(&E->device) != hipSuccess) return fail(set_error(CGM_ERR_NODEV, "no device"));
     E->n_slots = n_slots;
+    E->live.reserve(1 << 16);
     E->cap_pairs = max_batch_pairs;
     E->cap_words = max_batch_words;
     E->sc = sc;
@@ -445,6 +451,10 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
             return set_error(CGM_ERR_BUSY, "engine queue full");
         }
     }
+    // reserve the request's part of each bin's open staging buffer under the
+    // locks, copy into it after releasing them
+    Staging *stg[kBins] = {nullptr, nullptr};
+    int64_t n0[kBins] = {0, 0}, w0[kBins] = {0, 0};
     for (int bi = 0; bi < kBins; ++bi) {
         if (nb[bi] == 0) continue;
         Bin &B = E->bins[bi];
@@ -454,34 +464,54 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
             B.free_stages.pop_back();
         }
         Staging *st = B.open;
+        stg[bi] = st;
+        n0[bi] = st->n;
+        w0[bi] = st->words;
+        st->n += nb[bi];
+        st->words += wb[bi];
+        st->routes.resize((size_t)st->n);  // within the capacity reserved at creation
+        st->writers += 1;
+        B.pending_pairs += nb[bi];
+    }
+    l1.unlock();
+    l0.unlock();
+    for (int bi = 0; bi < kBins; ++bi) {
+        Staging *st = stg[bi];
+        if (!st) continue;
+        int64_t n = n0[bi], words = w0[bi];
         for (int32_t i = 0; i < n_pairs; ++i) {
             if (kind[i] != bi) continue;
             const int64_t qw = ((int64_t)q_len[i] + 31) / 32, rw = ((int64_t)r_len[i] + 31) / 32;
-            int32_t *m = st->h_meta + st->n * kMeta;
+            int32_t *m = st->h_meta + n * kMeta;
             m[0] = q_len[i];
             m[1] = r_len[i];
-            m[2] = (int32_t)st->words;
-            m[3] = (int32_t)(st->words + qw);
+            m[2] = (int32_t)words;
+            m[3] = (int32_t)(words + qw);
             m[4] = kind[i];
             m[5] = band[i];
             m[6] = max_error[i];
-            if (qw) memcpy(st->h_words + st->words, q_packed + q_off[i], (size_t)qw * 8);
-            if (rw) memcpy(st->h_words + st->words + qw, r_packed + r_off[i], (size_t)rw * 8);
-            st->words += qw + rw;
-            st->routes.push_back({r, i});
-            st->n += 1;
+            if (qw) memcpy(st->h_words + words, q_packed + q_off[i], (size_t)qw * 8);
+            if (rw) memcpy(st->h_words + words + qw, r_packed + r_off[i], (size_t)rw * 8);
+            words += qw + rw;
+            st->routes[(size_t)n] = {r, i};
+            n += 1;
         }
-        B.pending_pairs += nb[bi];
     }
-    // the dispatcher only acts when a slot is free; when none is, the slot's
-    // completion wakes it, and a wake-up here would cost a futex round trip
-    // per request for nothing
-    bool wake[kBins];
-    for (int bi = 0; bi < kBins; ++bi) wake[bi] = nb[bi] > 0 && !E->bins[bi].free_slots.empty();
-    l1.unlock();
-    l0.unlock();
-    for (int bi = 0; bi < kBins; ++bi)
-        if (wake[bi]) E->bins[bi].cv.notify_one();
+    // the dispatcher only acts when a slot is free and the buffer has no
+    // writer left; when no slot is free, the slot's completion wakes it, and
+    // a wake-up here would cost a futex round trip per request for nothing
+    for (int bi = 0; bi < kBins; ++bi) {
+        Staging *st = stg[bi];
+        if (!st) continue;
+        Bin &B = E->bins[bi];
+        bool wake;
+        {
+            std::lock_guard<std::mutex> lk(B.mu);
+            st->writers -= 1;
+            wake = st->writers == 0 && !B.free_slots.empty();
+        }
+        if (wake) B.cv.notify_one();
+    }
     *request_id = r->id;
     return CGM_OK;
 }
@@ -565,14 +595,16 @@ int cgm_engine_stats(void *engine, int64_t *out /* 8 */) {
 double cgm_engine_now(void) { return now_s(); }
 
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
-                  int32_t max_pairs_per_request, int32_t max_outstanding, uint64_t seed,
-                  double *stats /* 24 */) {
+                  int32_t max_pairs_per_request, int32_t max_outstanding, int32_t clients,
+                  uint64_t seed, double *stats /* 24 */) {
     Engine *E = (Engine *)engine;
-    if (!E || !gen || !stats || duration_s <= 0 || max_pairs_per_request < 1)
+    if (!E || !gen || !stats || duration_s <= 0 || max_pairs_per_request < 1 || clients < 1 ||
+        clients > 64)
         return set_error(CGM_ERR_INVALID, "cgm_serve_run: bad arguments");
     int64_t n_gen = 0, qwn = 0, rwn = 0;
     if (int rc = cgm_gen_sizes(gen, &n_gen, &qwn, &rwn)) return rc;
-    if (n_gen < max_pairs_per_request) return set_error(CGM_ERR_INVALID, "generator too small");
+    if (n_gen < (int64_t)max_pairs_per_request * clients)
+        return set_error(CGM_ERR_INVALID, "generator too small");
     std::vector<uint64_t> qp((size_t)qwn), rp((size_t)rwn);
     std::vector<int32_t> qo((size_t)n_gen), ql((size_t)n_gen), ro((size_t)n_gen),
         rl((size_t)n_gen), kind((size_t)n_gen), band((size_t)n_gen), me((size_t)n_gen);
@@ -584,92 +616,128 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
         b0[bi] = E->bins[bi].batches;
         bp0[bi] = E->bins[bi].batch_pairs;
     }
-    std::mt19937_64 rng(seed);
-    std::exponential_distribution<double> gap(rate_rps > 0 ? rate_rps : 1.0);
-    std::uniform_int_distribution<int> sz(1, max_pairs_per_request);
     struct Sub {
         int64_t id;
         int n;
         bool wfa;
     };
-    std::mutex mu;
-    std::condition_variable cv;
-    std::deque<Sub> inflight;
-    bool submit_done = false;
-    std::vector<double> lat, lat_sw, lat_wfa;
-    int64_t completed = 0, pairs_done = 0, bad = 0;
+    // One client: a submitting thread (Poisson arrivals at rate_rps / clients,
+    // or a closed loop) over its own slice of the generated pairs, and a
+    // collecting thread that waits for its requests in submission order.
+    struct Client {
+        std::mutex mu;
+        std::condition_variable cv;
+        std::deque<Sub> inflight;
+        bool submit_done = false;
+        std::vector<double> lat, lat_sw, lat_wfa;
+        int64_t completed = 0, pairs_done = 0, bad = 0, submitted = 0, refused = 0;
+        int error = CGM_OK;
+        double t_sub_end = 0;
+    };
+    std::vector<Client> cl((size_t)clients);
     std::atomic<int64_t> outstanding{0};
-    std::thread collector([&] {
+    const int64_t slice = n_gen / clients;
+    const double t0 = now_s();
+    auto collect = [&](Client &c) {
         std::vector<int32_t> sc((size_t)max_pairs_per_request);
         for (;;) {
-            Sub s;
+            Sub sub;
             {
-                std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return !inflight.empty() || submit_done; });
-                if (inflight.empty()) return;
-                s = inflight.front();
-                inflight.pop_front();
+                std::unique_lock<std::mutex> lk(c.mu);
+                c.cv.wait(lk, [&] { return !c.inflight.empty() || c.submit_done; });
+                if (c.inflight.empty()) return;
+                sub = c.inflight.front();
+                c.inflight.pop_front();
             }
             double l = 0;
-            if (cgm_engine_wait(E, s.id, -1, sc.data(), nullptr, nullptr, &l) == CGM_OK) {
-                lat.push_back(l);
-                (s.wfa ? lat_wfa : lat_sw).push_back(l);
-                completed += 1;
-                pairs_done += s.n;
-                for (int i = 0; i < s.n; ++i) bad += sc[(size_t)i] == CGM_BAD_PAIR;
+            if (cgm_engine_wait(E, sub.id, -1, sc.data(), nullptr, nullptr, &l) == CGM_OK) {
+                c.lat.push_back(l);
+                (sub.wfa ? c.lat_wfa : c.lat_sw).push_back(l);
+                c.completed += 1;
+                c.pairs_done += sub.n;
+                for (int i = 0; i < sub.n; ++i) c.bad += sc[(size_t)i] == CGM_BAD_PAIR;
             }
             outstanding -= 1;
         }
-    });
-    const double t0 = now_s();
-    double t_next = t0;
-    int64_t cursor = 0, submitted = 0, refused = 0;
-    while (true) {
-        const double t = now_s();
-        if (t - t0 >= duration_s) break;
-        if (rate_rps > 0) {
-            if (t < t_next) {
-                if (t_next - t > 200e-6) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    };
+    auto submit = [&](int ci) {
+        Client &c = cl[(size_t)ci];
+        std::mt19937_64 rng(seed + 0x9E3779B97F4A7C15ull * (uint64_t)(ci + 1));
+        std::exponential_distribution<double> gap(rate_rps > 0 ? rate_rps / clients : 1.0);
+        std::uniform_int_distribution<int> sz(1, max_pairs_per_request);
+        const int64_t lo = slice * ci, hi = lo + slice;
+        int64_t cursor = lo;
+        double t_next = t0;
+        for (;;) {
+            const double t = now_s();
+            if (t - t0 >= duration_s) break;
+            if (rate_rps > 0) {
+                if (t < t_next) {
+                    if (t_next - t > 200e-6) std::this_thread::sleep_for(std::chrono::microseconds(50));
+                    continue;
+                }
+            } else if (outstanding.load() >= max_outstanding) {
+                std::this_thread::sleep_for(std::chrono::microseconds(5));
                 continue;
             }
-        } else if (outstanding.load() >= max_outstanding) {
-            std::this_thread::sleep_for(std::chrono::microseconds(5));
-            continue;
+            const int n = sz(rng);
+            if (cursor + n > hi) cursor = lo;
+            const double arrival = rate_rps > 0 ? t_next : t;
+            int64_t id = 0;
+            const size_t k = (size_t)cursor;
+            bool wfa = false;
+            for (int i = 0; i < n; ++i) wfa |= kind[k + (size_t)i] == 1;
+            const int rc = cgm_engine_submit(E, qp.data(), qwn, rp.data(), rwn, ql.data() + k,
+                                             rl.data() + k, qo.data() + k, ro.data() + k, n,
+                                             kind.data() + k, band.data() + k, me.data() + k,
+                                             arrival, &id);
+            cursor += n;
+            if (rate_rps > 0) t_next += gap(rng);
+            if (rc == CGM_OK) {
+                c.submitted += 1;
+                outstanding += 1;
+                std::lock_guard<std::mutex> lk(c.mu);
+                c.inflight.push_back({id, n, wfa});
+                c.cv.notify_one();
+            } else if (rc == CGM_ERR_BUSY) {
+                c.refused += 1;
+                if (rate_rps <= 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
+            } else {
+                c.error = rc;
+                break;
+            }
         }
-        const int n = sz(rng);
-        if (cursor + n > n_gen) cursor = 0;
-        const double arrival = rate_rps > 0 ? t_next : t;
-        int64_t id = 0;
-        const size_t c = (size_t)cursor;
-        bool wfa = false;
-        for (int i = 0; i < n; ++i) wfa |= kind[c + (size_t)i] == 1;
-        const int rc = cgm_engine_submit(E, qp.data(), qwn, rp.data(), rwn, ql.data() + c,
-                                         rl.data() + c, qo.data() + c, ro.data() + c, n,
-                                         kind.data() + c, band.data() + c, me.data() + c, arrival,
-                                         &id);
-        cursor += n;
-        if (rate_rps > 0) t_next += gap(rng);
-        if (rc == CGM_OK) {
-            submitted += 1;
-            outstanding += 1;
-            std::lock_guard<std::mutex> lk(mu);
-            inflight.push_back({id, n, wfa});
-            cv.notify_one();
-        } else if (rc == CGM_ERR_BUSY) {
-            refused += 1;
-            if (rate_rps <= 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
-        } else {
-            break;
+        c.t_sub_end = now_s();
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            c.submit_done = true;
         }
+        c.cv.notify_all();
+    };
+    std::vector<std::thread> th;
+    for (int ci = 0; ci < clients; ++ci) {
+        th.emplace_back(collect, std::ref(cl[(size_t)ci]));
+        th.emplace_back(submit, ci);
     }
-    const double t_sub_end = now_s();
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        submit_done = true;
-    }
-    cv.notify_all();
-    collector.join();
+    for (std::thread &t : th) t.join();
     const double t_end = now_s();
+    std::vector<double> lat, lat_sw, lat_wfa;
+    int64_t submitted = 0, completed = 0, refused = 0, pairs_done = 0, bad = 0;
+    double t_sub_end = t0;
+    int error = CGM_OK;
+    for (Client &c : cl) {
+        lat.insert(lat.end(), c.lat.begin(), c.lat.end());
+        lat_sw.insert(lat_sw.end(), c.lat_sw.begin(), c.lat_sw.end());
+        lat_wfa.insert(lat_wfa.end(), c.lat_wfa.begin(), c.lat_wfa.end());
+        submitted += c.submitted;
+        completed += c.completed;
+        refused += c.refused;
+        pairs_done += c.pairs_done;
+        bad += c.bad;
+        t_sub_end = std::max(t_sub_end, c.t_sub_end);
+        if (c.error != CGM_OK) error = c.error;
+    }
+    if (error != CGM_OK) return error;
     auto pct = [](std::vector<double> &v, double p) {
         if (v.empty()) return 0.0;
         std::sort(v.begin(), v.end());

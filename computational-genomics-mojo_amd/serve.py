@@ -87,12 +87,13 @@ class ServingEngine:
 
     def run_load(self, gen_handle, rate_rps: float, duration_s: float,
                  max_pairs_per_request: int = 64, max_outstanding: int = 512,
-                 seed: int = 1) -> dict:
-        """Native load generator (cgm_serve_run): Poisson arrivals at
-        rate_rps (<= 0: closed loop with max_outstanding requests)."""
+                 seed: int = 1, clients: int = 1) -> dict:
+        """Native load generator (cgm_serve_run): `clients` client threads
+        with Poisson arrivals at rate_rps in total (<= 0: closed loop with
+        max_outstanding requests over all clients)."""
         st = np.zeros(len(STAT_NAMES), np.float64)
         _lib.call("cgm_serve_run", self.handle, gen_handle, float(rate_rps), float(duration_s),
-                  max_pairs_per_request, max_outstanding, seed, _lib.ptr(st))
+                  max_pairs_per_request, max_outstanding, clients, seed, _lib.ptr(st))
         return dict(zip(STAT_NAMES, st.tolist()))
 
     def close(self) -> None:

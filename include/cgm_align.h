@@ -394,7 +394,7 @@ int cgm_engine_wait(void *engine, int64_t request_id, int32_t timeout_ms, int32_
 /* Tuning of one bin's batches (bin 0: SW pairs, 1: WFA pairs) as a
  * CGM_TUNE_N array (fields as for cgm_plan_get_tuning); a new value applies
  * from the bin's next launch.  Bin 0 starts from the process default with
- * LANE_MAX_LEN = 400: SW reads longer than that run one wavefront per pair
+ * LANE_MAX_LEN = 200: SW reads longer than that run one wavefront per pair
  * inside the persistent kernel instead of one lane, which shortens a small
  * batch's longest serial chain. */
 int cgm_engine_get_tuning(void *engine, int32_t bin, int32_t *params);
@@ -403,17 +403,20 @@ int cgm_engine_set_tuning(void *engine, int32_t bin, const int32_t *params);
  * refused requests, queued pairs, device bytes held, last worker error. */
 int cgm_engine_stats(void *engine, int64_t *out);
 double cgm_engine_now(void);
-/* Load generator: requests of U{1..max_pairs_per_request} consecutive pairs
- * of a generated batch (cgm_gen_create), Poisson arrivals at rate_rps for
- * duration_s (rate_rps <= 0: closed loop with max_outstanding requests in
- * flight).  stats[24]: submitted, completed, refused, pairs completed,
- * elapsed s, pairs/s, requests/s, p50, p90, p99, max, mean latency (ms),
- * batches, mean pairs per batch, bad results, submit phase s, then
- * SW-only requests: count, p50, p99 (ms); requests with a WFA pair: count,
- * p50, p99 (ms); mean pairs per SW batch, per WFA batch. */
+/* Load generator: `clients` client threads (1..64), each submitting
+ * requests of U{1..max_pairs_per_request} consecutive pairs from its own
+ * slice of a generated batch (cgm_gen_create), with Poisson arrivals at
+ * rate_rps / clients each for duration_s (rate_rps <= 0: a closed loop with
+ * at most max_outstanding requests in flight over all clients), and each
+ * collecting its results in submission order.  stats[24]: submitted,
+ * completed, refused, pairs completed, elapsed s, pairs/s, requests/s, p50,
+ * p90, p99, max, mean latency (ms), batches, mean pairs per batch, bad
+ * results, submit phase s, then SW-only requests: count, p50, p99 (ms);
+ * requests with a WFA pair: count, p50, p99 (ms); mean pairs per SW batch,
+ * per WFA batch. */
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
-                  int32_t max_pairs_per_request, int32_t max_outstanding, uint64_t seed,
-                  double *stats);
+                  int32_t max_pairs_per_request, int32_t max_outstanding, int32_t clients,
+                  uint64_t seed, double *stats);
 
 /* -------------------------------------------- host-pointer conveniences
  * Same as the launch entry points but every pointer is a host pointer; the

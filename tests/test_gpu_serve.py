@@ -64,7 +64,7 @@ def test_engine_bin_tuning_keeps_results(cgm, dev, orc, serving_pairs, lane_max_
     w = serving_pairs
     eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=2, max_batch_pairs=2048,
                             max_batch_words=1 << 17, max_error_cap=int(w.max_error.max()))
-    assert eng.get_tuning(0)["lane_max_len"] == 400
+    assert eng.get_tuning(0)["lane_max_len"] == 200
     t = eng.set_tuning(0, lane_max_len=lane_max_len)
     assert eng.get_tuning(0) == t and t["lane_max_len"] == lane_max_len
     with pytest.raises(cgm._lib.CgmError):
