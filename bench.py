@@ -489,7 +489,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run 3 device-to-device copies of this many bytes (counter scale)")
     ap.add_argument("--only-extra", choices=["wfa_20k_1to10kb", "ingest_200k_pairs", "wfa_cigar_20k_1to10kb",
-                                             "nw_200k_100to300bp", "serving_1to64_pairs"],
+                                             "nw_200k_100to300bp", "serving_1to64_pairs",
+                                             "mixed_100k", "sw_1k_150bp"],
                     help="run only this extra workload and print its result (for profiling "
                          "its kernels alone)")
     args = ap.parse_args()
@@ -503,6 +504,10 @@ def main():
             w = gen.generate(gen.SW_100_300BP, args.pairs)
             cells = int(gen.valid_band_cells(w.q_len, w.r_len, w.band).sum())
             res = nw_extra(m, dev, w, cells)
+        elif args.only_extra == "mixed_100k":
+            res = mixed_extra(m, dev, gen, cpu=not args.no_cpu)
+        elif args.only_extra == "sw_1k_150bp":
+            res = sw_150bp_extra(m, dev, gen, not args.no_cpu)
         else:
             fn = {"wfa_20k_1to10kb": wfa_extra, "ingest_200k_pairs": ingest_extra,
                   "wfa_cigar_20k_1to10kb": cigar_extra}[args.only_extra]
