@@ -42,6 +42,7 @@ WFA_OPS_PER_CELL = 8
 # counter reads on a 256 MiB device copy) and read from the summary that
 # holds those passes and their commands (its "traffic_bytes:" line).
 TRAFFIC_SOURCE = "profiles/sw_persistent_200k_100to300bp_rocprof.md"
+WFA_TRAFFIC_SOURCE = "profiles/wfa_reg_20k_1to10kb_rocprof.md"
 
 
 def traffic_from_profile(path=TRAFFIC_SOURCE):
@@ -183,7 +184,9 @@ def wfa_extra(m, dev, gen, n_pairs=20_000, cpu=True):
     in_bytes = int(w.q_packed.nbytes + w.r_packed.nbytes + 5 * 4 * n)
     res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS,
                        "unit": "TOP/s (int32)", "frac": achieved / VALU_PEAK_TOPS,
-                       "traffic": None, "kernel": "WFA class kernels (plan events)",
+                       "traffic": traffic_from_profile(WFA_TRAFFIC_SOURCE),
+                       "traffic_source": WFA_TRAFFIC_SOURCE,
+                       "kernel": "WFA class kernels (plan events)",
                        "kernel_ms": kms_wfa, "cells": total, "ops_per_cell": WFA_OPS_PER_CELL,
                        "hbm_algorithmic_bytes": in_bytes + 4 * n,
                        "hbm_frac": (in_bytes + 4 * n) / (kms_wfa * 1e-3) / 1e9 / HBM_PEAK_GBS}
