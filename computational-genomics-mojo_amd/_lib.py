@@ -96,6 +96,10 @@ SIGNATURES = {
     "cgm_engine_poll": [_vp, _i64, _pi32],
     "cgm_engine_wait": [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)],
     "cgm_engine_stats": [_vp, _pi64],
+    "cgm_sw_banded_batch": [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i32,
+                            _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp],
+    "cgm_wfa_affine_batch": [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i32,
+                             _i32, _i32, _i32, _vp, _vp],
     "cgm_engine_get_tuning": [_vp, _i32, _vp],
     "cgm_engine_set_tuning": [_vp, _i32, _vp],
     "cgm_engine_now": [],

@@ -447,6 +447,36 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed, co
     }
 }
 
+// SequenceBatch-shaped forms: each side's SoA arrays together, NMask words
+// accepted and never read (the reference's decode ignores NMask,
+// sequence.mojo:96-101, and align_cpu.mojo never consults it).
+int cgm_sw_banded_batch(const uint64_t *q_packed, const uint64_t *q_nmask, const int32_t *q_off,
+                        const int32_t *q_len, int64_t q_words, const uint64_t *r_packed,
+                        const uint64_t *r_nmask, const int32_t *r_off, const int32_t *r_len,
+                        int64_t r_words, int64_t n_pairs, const int32_t *band,
+                        int32_t band_scalar, int32_t match, int32_t mismatch, int32_t gap_open,
+                        int32_t gap_extend, int32_t *out_score, int32_t *out_q_end,
+                        int32_t *out_r_end, void *stream) {
+    (void)q_nmask;
+    (void)r_nmask;
+    return cgm_sw_banded_launch(q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words,
+                                r_words, band, band_scalar, match, mismatch, gap_open, gap_extend,
+                                out_score, out_q_end, out_r_end, CGM_LAYOUT_AUTO, stream);
+}
+
+int cgm_wfa_affine_batch(const uint64_t *q_packed, const uint64_t *q_nmask, const int32_t *q_off,
+                         const int32_t *q_len, int64_t q_words, const uint64_t *r_packed,
+                         const uint64_t *r_nmask, const int32_t *r_off, const int32_t *r_len,
+                         int64_t r_words, int64_t n_pairs, const int32_t *max_error,
+                         int32_t max_error_scalar, int32_t x, int32_t o, int32_t e,
+                         int32_t *out_cost, void *stream) {
+    (void)q_nmask;
+    (void)r_nmask;
+    return cgm_wfa_affine_launch(q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words,
+                                 r_words, max_error, max_error_scalar, x, o, e, out_cost,
+                                 CGM_LAYOUT_AUTO, stream);
+}
+
 int cgm_plan_create(int64_t max_pairs, int32_t max_error_cap, int32_t x, int32_t o, int32_t e,
                     int32_t waves_per_cu, int32_t lds_bytes, void **plan) {
     if (!plan) return set_error(CGM_ERR_INVALID, "null plan pointer");

@@ -214,6 +214,31 @@ int cgm_wfa_affine_launch(const uint64_t *q_packed, const uint64_t *r_packed,
                           int32_t x, int32_t o, int32_t e,
                           int32_t *out_cost, int32_t layout, void *stream);
 
+/* ----------------------------------------- SequenceBatch-shaped batch forms
+ * cgm_sw_banded_launch / cgm_wfa_affine_launch with layout CGM_LAYOUT_AUTO,
+ * taking each side as the SequenceBatch structure of arrays it is held in
+ * (genomics/core/sequence.mojo:23-36: packed words, NMask words, word
+ * offsets, lengths, and the buffer's word count), as the launchers of
+ * genomics/gpu/align_gpu.mojo:460-535 receive them after
+ * GenomicsDevice.upload_* (genomics/gpu/device.mojo:60-108).  q_nmask /
+ * r_nmask may be NULL; they are accepted and never read: an N (or any
+ * non-ACGT base) is scored as the A it is packed as, exactly as the
+ * reference scores it (its decode ignores NMask, sequence.mojo:96-101).
+ * Results, limits and errors as for the launch entry points. */
+int cgm_sw_banded_batch(const uint64_t *q_packed, const uint64_t *q_nmask, const int32_t *q_off,
+                        const int32_t *q_len, int64_t q_words, const uint64_t *r_packed,
+                        const uint64_t *r_nmask, const int32_t *r_off, const int32_t *r_len,
+                        int64_t r_words, int64_t n_pairs, const int32_t *band,
+                        int32_t band_scalar, int32_t match, int32_t mismatch, int32_t gap_open,
+                        int32_t gap_extend, int32_t *out_score, int32_t *out_q_end,
+                        int32_t *out_r_end, void *stream);
+int cgm_wfa_affine_batch(const uint64_t *q_packed, const uint64_t *q_nmask, const int32_t *q_off,
+                         const int32_t *q_len, int64_t q_words, const uint64_t *r_packed,
+                         const uint64_t *r_nmask, const int32_t *r_off, const int32_t *r_len,
+                         int64_t r_words, int64_t n_pairs, const int32_t *max_error,
+                         int32_t max_error_scalar, int32_t x, int32_t o, int32_t e,
+                         int32_t *out_cost, void *stream);
+
 /* ------------------------------------------- gap-affine WFA with traceback
  * Replaces wfa_affine_cigar_cpu (genomics/cpu/align_cpu.mojo:380-623) for a
  * batch: out_cost as cgm_wfa_affine_launch, plus the CIGAR that function's

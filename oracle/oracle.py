@@ -41,8 +41,8 @@ def select_fast_build() -> str:
     global _LIB_PATH
     if _lib is None and host_has_v3() and os.path.exists(_FAST_PATH):
         _LIB_PATH = _FAST_PATH
-        return "-O3 -march=x86-64-v3"
-    return "-O3 -march=x86-64-v2"
+    # the flags of the build in use (a second call sees the first call's choice)
+    return "-O3 -march=x86-64-v3" if _LIB_PATH == _FAST_PATH else "-O3 -march=x86-64-v2"
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32

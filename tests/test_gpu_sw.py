@@ -251,3 +251,48 @@ def test_sw_wave_long_pairs(cgm, dev, orc):
         print(f"[case] sw wave long pairs scoring={sc}", flush=True)
         assert_same(sw_gpu(cgm, dev, up, LAYOUTS["wave"], scoring=sc), sw_oracle(orc, w, sc),
                     f"sw wave long pairs {sc}")
+
+
+def test_sequence_batch_forms_ignore_nmask(cgm, dev, orc, gen_mod):
+    """cgm_sw_banded_batch / cgm_wfa_affine_batch take each side's SoA with
+    its NMask words; the masks are never read, so the real masks, inverted
+    masks and NULL give the oracle's results (N scored as the A it is
+    packed as, as the reference scores it)."""
+    from gpu_helpers import WFA_PEN, wfa_oracle
+    w = gen_mod.generate(gen_mod.SW_150BP, 2000, seed=131)
+    assert w.q_nmask.any()  # the workload holds N positions
+    w3 = gen_mod.generate(gen_mod.WFA_1_10KB, 64, seed=132)
+    exp_sw = sw_oracle(orc, w)
+    exp_wfa = wfa_oracle(orc, w3)
+    for variant in ("masks", "inverted", "null"):
+        for wl, kind in ((w, "sw"), (w3, "wfa")):
+            up = Uploaded(dev, wl)
+            if variant == "null":
+                qn = rn = None
+            else:
+                qm, rm = wl.q_nmask, wl.r_nmask
+                if variant == "inverted":
+                    qm, rm = ~qm, ~rm
+                qn, rn = dev.upload(qm), dev.upload(rm)
+            qnp = qn.dptr if qn is not None else None
+            rnp = rn.dptr if rn is not None else None
+            if kind == "sw":
+                out = [cgm.DeviceArray(wl.n_pairs, np.int32) for _ in range(3)]
+                cgm._lib.call("cgm_sw_banded_batch", up.q.dptr, qnp, up.qo.dptr, up.ql.dptr,
+                              up.q_words, up.r.dptr, rnp, up.ro.dptr, up.rl.dptr, up.r_words,
+                              wl.n_pairs, up.band.dptr, 0, CONFIG_SW["match"],
+                              CONFIG_SW["mismatch"], CONFIG_SW["gap_open"],
+                              CONFIG_SW["gap_extend"], out[0].dptr, out[1].dptr, out[2].dptr,
+                              dev.stream)
+                dev.synchronize()
+                got = tuple(dev.download(o, wl.n_pairs) for o in out)
+                assert_same(got, exp_sw, f"cgm_sw_banded_batch nmask={variant}")
+            else:
+                out = cgm.DeviceArray(wl.n_pairs, np.int32)
+                cgm._lib.call("cgm_wfa_affine_batch", up.q.dptr, qnp, up.qo.dptr, up.ql.dptr,
+                              up.q_words, up.r.dptr, rnp, up.ro.dptr, up.rl.dptr, up.r_words,
+                              wl.n_pairs, up.me.dptr, 0, WFA_PEN["x"], WFA_PEN["o"],
+                              WFA_PEN["e"], out.dptr, dev.stream)
+                dev.synchronize()
+                assert_same(dev.download(out, wl.n_pairs), exp_wfa,
+                            f"cgm_wfa_affine_batch nmask={variant}")
