@@ -362,12 +362,19 @@ int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, 
 
 // Automatic layout: the persistent task-queue kernel of the mixed path with
 // every pair an SW pair, on a plan cached per (device, stream) and grown
-// when a larger batch arrives.
+// when a larger batch arrives.  A batch of at most kWaveBatchPerCu pairs per
+// CU (two rounds of the kernel's 8 wavefronts per CU) runs one wavefront per
+// pair (bands <= 64): one pair per lane would leave most SIMDs idle and
+// every pair's whole band area on one lane's serial chain.
+constexpr int64_t kWaveBatchPerCu = 16;
 int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                    SwOut out, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
     if (int rc = check_sw_scoring(sc)) return rc;
     if (b.n_pairs == 0) return CGM_OK;
+    int cus = 0;
+    if (int rc = device_cu_count(&cus)) return rc;
+    if (b.n_pairs <= kWaveBatchPerCu * cus) return launch_sw_wave(b, band, band_scalar, sc, out, stream);
     const WfaPen pen = {4, 6, 2};  // unused: the batch holds no WFA pair
     return with_cached_plan(stream, b.n_pairs, 0, pen, [&](MixPlan *P) {
         return mix_enqueue(P, b, nullptr, 0, band, band_scalar, nullptr, 0, sc, pen, out, stream);

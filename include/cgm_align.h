@@ -52,6 +52,7 @@ extern "C" {
 /* Kernel layouts for the launch entry points. */
 #define CGM_LAYOUT_AUTO 0   /* SW: persistent task-queue kernel, work-sorted, one pair per
                                lane, on a plan the library keeps per (device, stream);
+                               batches of at most 16 pairs per CU as CGM_LAYOUT_WAVE;
                                WFA: register layout (wavefronts in VGPRs, one diagonal per
                                lane and block) where the penalties are x : o+e : e =
                                2 : 4 : 1 and the pair's diagonals fit, else as
