@@ -88,6 +88,12 @@ SIGNATURES = {
     "cgm_plan_autotune": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _vp,
                           _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp,
                           _vp, _i32, _vp, _pf, _pf, _vp],
+    "cgm_plan_autotune_cached": [_vp, ctypes.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
+                                 _i64, _vp, _i32, _vp, _i32, _vp, _i32, _i32, _i32, _i32, _i32,
+                                 _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _pf, _pf, _vp, _vp],
+    "cgm_plan_profile_key": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _vp,
+                             _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, ctypes.c_char_p,
+                             _i32, _vp],
     "cgm_engine_create": [_i64, _i32, _i64, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                           _pvp],
     "cgm_engine_destroy": [_vp],

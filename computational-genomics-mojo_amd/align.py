@@ -17,6 +17,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import os
+
 import numpy as np
 
 from . import _lib
@@ -429,6 +431,36 @@ class MixedPlan:
                   _lib.ptr(a), ctypes.byref(best), ctypes.byref(start), device.stream)
         return dict(zip(_lib.TUNE_FIELDS, (int(v) for v in a))), float(best.value), \
             float(start.value)
+
+    def autotune_cached(self, device: GenomicsDevice, cache_path: str, bufs, n: int, kind, band,
+                        max_error, cfg: AlignConfig, out: SwResult, reps: int = 2):
+        """cgm_plan_autotune_cached: the tuning kept in `cache_path` for this
+        batch's length-bin profile, or a measured one that is appended there.
+        Returns (tuning dict, best ms, starting ms or -1, from_cache)."""
+        import ctypes
+        a = np.zeros(len(_lib.TUNE_FIELDS), np.int32)
+        best, start = ctypes.c_float(), ctypes.c_float()
+        hit = ctypes.c_int32()
+        args = _mixed_args(bufs, n, kind, band, max_error, cfg, out, self.pen)
+        _lib.call("cgm_plan_autotune_cached", self.handle, os.fsencode(cache_path), *args,
+                  int(reps), _lib.ptr(a), ctypes.byref(best), ctypes.byref(start),
+                  ctypes.byref(hit), device.stream)
+        return (dict(zip(_lib.TUNE_FIELDS, (int(v) for v in a))), float(best.value),
+                float(start.value), bool(hit.value))
+
+    def profile_key(self, device: GenomicsDevice, bufs, n: int, kind, band,
+                    cfg: AlignConfig) -> str:
+        """The batch's length-bin profile key (cgm_plan_profile_key)."""
+        import ctypes
+        q, r, ql, rl, qo, ro = bufs
+        ptr = lambda v: v.dptr if v is not None else None  # noqa: E731
+        buf = ctypes.create_string_buffer(4096)
+        x, o, e = self.pen
+        _lib.call("cgm_plan_profile_key", self.handle, q.dptr, r.dptr, ql.dptr, rl.dptr, qo.dptr,
+                  ro.dptr, n, q.count, r.count, ptr(kind), 0, ptr(band), cfg.band_width,
+                  cfg.match_score, cfg.mismatch, cfg.gap_open, cfg.gap_extend, x, o, e, buf,
+                  len(buf), device.stream)
+        return buf.value.decode()
 
     def close(self) -> None:
         if self.handle:

@@ -161,6 +161,23 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
                  const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                  SwScoring sc, WfaPen pen, SwOut out, int reps, hipStream_t stream,
                  MixTuning *best, float *best_ms, float *start_ms);
+// The batch's length-bin profile as a text key: device, scoring, the log2
+// bucket of the pair count and, per (kind, log2 length bin, SW band bin),
+// the share of the pairs in 32nds (nonzero cells only).  Reads counts back
+// (synchronises `stream`).
+int mix_profile_key(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                    const int32_t *band, int32_t band_scalar, SwScoring sc, WfaPen pen,
+                    hipStream_t stream, std::string *key);
+// mix_autotune with the choice persisted per profile key in a text file
+// (one line per key: key, tuning, best ms; the last line of a key wins): a
+// key found there sets the plan's tuning without measuring; otherwise the
+// autotuner runs and its choice is appended.  Either way the outputs hold the
+// batch's results.
+int mix_autotune_cached(MixPlan *P, const char *cache_path, const Batch &b, const int32_t *kind,
+                        int32_t kind_scalar, const int32_t *band, int32_t band_scalar,
+                        const int32_t *me, int32_t me_scalar, SwScoring sc, WfaPen pen, SwOut out,
+                        int reps, hipStream_t stream, MixTuning *best, float *best_ms,
+                        float *start_ms, bool *from_cache);
 // A plan owned by the library for the drop-in launchers, one per (device,
 // stream), recreated larger when a batch exceeds it (the old plan is freed
 // after the stream drains).  `use` runs with the plan locked, so calls on

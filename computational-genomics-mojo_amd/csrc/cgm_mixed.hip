@@ -30,10 +30,13 @@
 //  7. the caller's stream joins the forked one.
 // The class limits and the kernel's shape are the plan's MixTuning.
 
+#include <stdio.h>
+
 #include <algorithm>
 #include <functional>
 #include <map>
 #include <mutex>
+#include <string>
 
 #include <hipcub/hipcub.hpp>
 
@@ -256,6 +259,40 @@ __global__ void mix_tasks_kernel(const int32_t *small, const uint32_t *sorted_ke
 
 __global__ void mix_zero_kernel(int32_t *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// Length-bin profile of a batch: pairs per (kind, floor(log2(max(qlen,
+// rlen))), SW band bin), counted in LDS per workgroup.  Band bins of the
+// effective band: 0-16, 17-64, 65-128, above (WFA pairs: bin 0); bad pairs
+// are not counted.
+constexpr int kProfileLenBins = 24, kProfileBandBins = 4;
+constexpr int kProfileCells = 2 * kProfileLenBins * kProfileBandBins;
+__global__ void mix_profile_kernel(Batch b, const int32_t *kind, int32_t kind_scalar,
+                                   const int32_t *band, int32_t band_scalar, int32_t *hist) {
+    __shared__ int32_t h[kProfileCells];
+    for (int j = threadIdx.x; j < kProfileCells; j += blockDim.x) h[j] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < b.n_pairs; i += stride) {
+        const uint64_t *q, *r;
+        int ql = 0, rl = 0;
+        if (!load_pair(b, i, q, r, ql, rl)) continue;
+        const int k = kind ? kind[i] : kind_scalar;
+        if (k != 0 && k != 1) continue;
+        const int len = max(max(ql, rl), 1);
+        const int lb = min(31 - __clz(len), kProfileLenBins - 1);
+        int bb = 0;
+        if (k == 0) {
+            const int bd = band ? band[i] : band_scalar;
+            if (bd < 0) continue;
+            const int be = effective_band(bd, ql, rl);
+            bb = be <= 16 ? 0 : be <= 64 ? 1 : be <= 128 ? 2 : 3;
+        }
+        atomicAdd(&h[(k * kProfileLenBins + lb) * kProfileBandBins + bb], 1);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kProfileCells; j += blockDim.x)
+        if (h[j]) atomicAdd(&hist[j], h[j]);
 }
 
 // The SW persistent kernel.  Every wavefront takes 64-pair units of the
@@ -807,6 +844,117 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
     if (rc) return rc;
     *best = cur;
     if (best_ms) *best_ms = cur_ms;
+    return CGM_OK;
+}
+
+int mix_profile_key(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
+                    const int32_t *band, int32_t band_scalar, SwScoring sc, WfaPen pen,
+                    hipStream_t stream, std::string *key) {
+    if (int rc = check_batch(b)) return rc;
+    int32_t hist[kProfileCells];
+    hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(256), 0, stream, P->small, kProfileCells);
+    if (b.n_pairs > 0) {
+        const unsigned grid = (unsigned)std::min<int64_t>((b.n_pairs + 255) / 256, P->cus);
+        hipLaunchKernelGGL(mix_profile_kernel, dim3(grid), dim3(256), 0, stream, b, kind,
+                           kind_scalar, band, band_scalar, P->small);
+    }
+    CGM_HIP(hipGetLastError());
+    CGM_HIP(hipMemcpyAsync(hist, P->small, sizeof(hist), hipMemcpyDeviceToHost, stream));
+    CGM_HIP(hipStreamSynchronize(stream));
+    char dev[160] = {0};
+    if (int rc = cgm_device_info(P->device, dev, (int)sizeof(dev), nullptr, nullptr, nullptr,
+                                 nullptr))
+        return rc;
+    int64_t total = 0;
+    for (int32_t v : hist) total += v;
+    int nbin = 0;
+    while (nbin < 62 && (int64_t(1) << (nbin + 1)) <= b.n_pairs) ++nbin;
+    char buf[256];
+    snprintf(buf, sizeof(buf), "dev=%s;sw=%d,%d,%d,%d;wfa=%d,%d,%d;n=2^%d", dev, sc.match,
+             sc.mismatch, sc.gap_open, sc.gap_extend, pen.x, pen.o, pen.e, nbin);
+    std::string k = buf;
+    for (int c = 0; c < kProfileCells; ++c) {
+        if (!hist[c]) continue;
+        const int share = (int)std::max<int64_t>(1, (hist[c] * 32 + total / 2) / std::max<int64_t>(total, 1));
+        const int kd = c / (kProfileLenBins * kProfileBandBins);
+        const int lb = (c / kProfileBandBins) % kProfileLenBins, bb = c % kProfileBandBins;
+        snprintf(buf, sizeof(buf), ";%s%d.%d=%d", kd ? "wfa" : "sw", lb, bb, share);
+        k += buf;
+    }
+    for (char &ch : k)
+        if (ch == '\t' || ch == '\n') ch = ' ';
+    *key = k;
+    return CGM_OK;
+}
+
+namespace {
+std::mutex g_tune_cache_mu;
+}  // namespace
+
+int mix_autotune_cached(MixPlan *P, const char *cache_path, const Batch &b, const int32_t *kind,
+                        int32_t kind_scalar, const int32_t *band, int32_t band_scalar,
+                        const int32_t *me, int32_t me_scalar, SwScoring sc, WfaPen pen, SwOut out,
+                        int reps, hipStream_t stream, MixTuning *best, float *best_ms,
+                        float *start_ms, bool *from_cache) {
+    if (!cache_path || !*cache_path) return set_error(CGM_ERR_INVALID, "autotune: no cache path");
+    std::string key;
+    if (int rc = mix_profile_key(P, b, kind, kind_scalar, band, band_scalar, sc, pen, stream, &key))
+        return rc;
+    std::lock_guard<std::mutex> lk(g_tune_cache_mu);
+    bool found = false;
+    MixTuning t{};
+    float t_ms = 0;
+    if (FILE *f = fopen(cache_path, "r")) {
+        std::string line;
+        char chunk[1024];
+        while (fgets(chunk, sizeof(chunk), f)) {
+            line += chunk;
+            if (line.empty() || line.back() != '\n') continue;  // long line: keep reading
+            line.pop_back();
+            const size_t tab = line.find('\t');
+            if (tab != std::string::npos && line.compare(0, tab, key) == 0 && tab == key.size()) {
+                int32_t a[CGM_TUNE_N];
+                float ms = 0;
+                if (sscanf(line.c_str() + tab + 1, "%d,%d,%d,%d,%d,%d,%d,%d\t%f", &a[0], &a[1],
+                           &a[2], &a[3], &a[4], &a[5], &a[6], &a[7], &ms) == 9) {
+                    MixTuning v;
+                    if (tuning_from_array(a, &v) == CGM_OK) {
+                        t = v;
+                        t_ms = ms;
+                        found = true;
+                    }
+                }
+            }
+            line.clear();
+        }
+        fclose(f);
+    }
+    *from_cache = found;
+    if (found) {
+        P->tune = t;
+        // the outputs hold the batch's results, as after a measured autotune
+        if (int rc = mix_enqueue(P, b, kind, kind_scalar, band, band_scalar, me, me_scalar, sc, pen,
+                                 out, stream))
+            return rc;
+        CGM_HIP(hipStreamSynchronize(stream));
+        *best = t;
+        if (best_ms) *best_ms = t_ms;
+        if (start_ms) *start_ms = -1.0f;
+        return CGM_OK;
+    }
+    float bm = 0;
+    if (int rc = mix_autotune(P, b, kind, kind_scalar, band, band_scalar, me, me_scalar, sc, pen,
+                              out, reps, stream, best, &bm, start_ms))
+        return rc;
+    if (best_ms) *best_ms = bm;
+    FILE *f = fopen(cache_path, "a");
+    if (!f) return set_error(CGM_ERR_INVALID, "autotune: cannot write the cache file");
+    int32_t a[CGM_TUNE_N];
+    tuning_to_array(*best, a);
+    fprintf(f, "%s\t%d,%d,%d,%d,%d,%d,%d,%d\t%.4f\n", key.c_str(), a[0], a[1], a[2], a[3], a[4],
+            a[5], a[6], a[7], bm);
+    const bool ok = fclose(f) == 0;
+    if (!ok) return set_error(CGM_ERR_INVALID, "autotune: cannot write the cache file");
     return CGM_OK;
 }
 

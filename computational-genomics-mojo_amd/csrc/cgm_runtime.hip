@@ -612,6 +612,53 @@ int cgm_plan_autotune(void *plan, const uint64_t *q_packed, const uint64_t *r_pa
     return CGM_OK;
 }
 
+int cgm_plan_autotune_cached(void *plan, const char *cache_path, const uint64_t *q_packed,
+                             const uint64_t *r_packed, const int32_t *q_len, const int32_t *r_len,
+                             const int32_t *q_off, const int32_t *r_off, int64_t n_pairs,
+                             int64_t q_words, int64_t r_words, const int32_t *kind,
+                             int32_t kind_scalar, const int32_t *band, int32_t band_scalar,
+                             const int32_t *max_error, int32_t max_error_scalar, int32_t match,
+                             int32_t mismatch, int32_t gap_open, int32_t gap_extend, int32_t x,
+                             int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                             int32_t *out_r_end, int32_t reps, int32_t *best_params,
+                             float *best_ms, float *start_ms, int32_t *from_cache, void *stream) {
+    if (!plan || !best_params || !cache_path) return set_error(CGM_ERR_INVALID, "null argument");
+    Batch b = {q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words, r_words};
+    SwScoring sc = {match, mismatch, gap_open, gap_extend};
+    SwOut out = {out_score, out_q_end, out_r_end};
+    if (n_pairs > 0 && (!out_score || !out_q_end || !out_r_end))
+        return set_error(CGM_ERR_INVALID, "null output array");
+    if (int rc = check_sw_scoring(sc)) return rc;
+    MixTuning best;
+    bool hit = false;
+    if (int rc = mix_autotune_cached((MixPlan *)plan, cache_path, b, kind, kind_scalar, band,
+                                     band_scalar, max_error, max_error_scalar, sc, WfaPen{x, o, e},
+                                     out, reps, (hipStream_t)stream, &best, best_ms, start_ms, &hit))
+        return rc;
+    tuning_to_array(best, best_params);
+    if (from_cache) *from_cache = hit ? 1 : 0;
+    return CGM_OK;
+}
+
+int cgm_plan_profile_key(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                         const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                         const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                         const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                         int32_t band_scalar, int32_t match, int32_t mismatch, int32_t gap_open,
+                         int32_t gap_extend, int32_t x, int32_t o, int32_t e, char *key,
+                         int32_t key_len, void *stream) {
+    if (!plan || !key || key_len < 1) return set_error(CGM_ERR_INVALID, "null argument");
+    Batch b = {q_packed, r_packed, q_len, r_len, q_off, r_off, n_pairs, q_words, r_words};
+    std::string k;
+    if (int rc = mix_profile_key((MixPlan *)plan, b, kind, kind_scalar, band, band_scalar,
+                                 SwScoring{match, mismatch, gap_open, gap_extend}, WfaPen{x, o, e},
+                                 (hipStream_t)stream, &k))
+        return rc;
+    if ((int64_t)k.size() + 1 > key_len) return set_error(CGM_ERR_INVALID, "key buffer too small");
+    memcpy(key, k.c_str(), k.size() + 1);
+    return CGM_OK;
+}
+
 int cgm_sw_banded_host(const uint64_t *q_packed, const uint64_t *r_packed, const int32_t *q_len,
                        const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
                        int64_t n_pairs, int64_t q_words, int64_t r_words, const int32_t *band,

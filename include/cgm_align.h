@@ -378,6 +378,37 @@ int cgm_plan_autotune(void *plan, const uint64_t *q_packed, const uint64_t *r_pa
                       int32_t *out_r_end, int32_t reps, int32_t *best_params, float *best_ms,
                       float *start_ms, void *stream);
 
+/* The tuning chosen per batch profile and kept across processes.  A batch's
+ * profile key (cgm_plan_profile_key, NUL-terminated, at most key_len bytes)
+ * names the device, the scoring, the log2 bucket of the pair count and,
+ * per (kind, log2 bin of the longer length, SW effective-band bin 0-16 /
+ * 17-64 / 65-128 / wider), the share of the pairs in 32nds.
+ * cgm_plan_autotune_cached looks the key up in the text file cache_path
+ * (one line per choice: key, tab, the CGM_TUNE_N fields, tab, best ms; the
+ * file's last line for a key wins): when found it sets the plan's tuning
+ * without measuring (*from_cache = 1, *start_ms = -1); otherwise it runs
+ * cgm_plan_autotune and appends the choice (*from_cache = 0).  Either way
+ * the outputs hold the batch's results.  Both read the profile back from
+ * the device (they synchronise `stream`); calls in one process take turns
+ * on the file. */
+int cgm_plan_profile_key(void *plan, const uint64_t *q_packed, const uint64_t *r_packed,
+                         const int32_t *q_len, const int32_t *r_len, const int32_t *q_off,
+                         const int32_t *r_off, int64_t n_pairs, int64_t q_words, int64_t r_words,
+                         const int32_t *kind, int32_t kind_scalar, const int32_t *band,
+                         int32_t band_scalar, int32_t match, int32_t mismatch, int32_t gap_open,
+                         int32_t gap_extend, int32_t x, int32_t o, int32_t e, char *key,
+                         int32_t key_len, void *stream);
+int cgm_plan_autotune_cached(void *plan, const char *cache_path, const uint64_t *q_packed,
+                             const uint64_t *r_packed, const int32_t *q_len, const int32_t *r_len,
+                             const int32_t *q_off, const int32_t *r_off, int64_t n_pairs,
+                             int64_t q_words, int64_t r_words, const int32_t *kind,
+                             int32_t kind_scalar, const int32_t *band, int32_t band_scalar,
+                             const int32_t *max_error, int32_t max_error_scalar, int32_t match,
+                             int32_t mismatch, int32_t gap_open, int32_t gap_extend, int32_t x,
+                             int32_t o, int32_t e, int32_t *out_score, int32_t *out_q_end,
+                             int32_t *out_r_end, int32_t reps, int32_t *best_params,
+                             float *best_ms, float *start_ms, int32_t *from_cache, void *stream);
+
 /* ------------------------------------------------------ serving engine
  * Continuous batching for a stream of alignment requests (no reference
  * counterpart; the reference aligns one pair per call).  Pairs are batched

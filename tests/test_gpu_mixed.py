@@ -173,3 +173,58 @@ def test_mixed_autotune_keeps_results_and_tuning(cgm, dev, orc, gen_mod):
         plan.set_tuning(reg_max_band=65)
     assert plan.tuning() == best
     plan.close()
+
+
+def test_mixed_autotune_cached_by_profile(cgm, dev, orc, gen_mod, tmp_path):
+    """cgm_plan_autotune_cached: the first call measures and appends its
+    choice for the batch's length-bin profile; a new plan on the same
+    profile takes it from the file without measuring; a batch of another
+    profile misses.  Results equal the oracle in every case."""
+    cache = str(tmp_path / "tuning.tsv")
+    w = mixed_sample(gen_mod, seed=23)
+    exp = oracle_mixed(orc, w)
+    bufs, arrs = upload_mixed(dev, w)
+    plan = cgm.MixedPlan(w.n_pairs, int(w.max_error.max()))
+    out = cgm.SwResult(w.n_pairs)
+    key = plan.profile_key(dev, bufs, w.n_pairs, arrs[0], arrs[1], sw_cfg(cgm))
+    print(f"[case] cached autotune, key {key}", flush=True)
+    import re
+    assert key.startswith("dev=") and re.search(r";sw\d+\.\d=", key) and re.search(r";wfa\d+\.0=", key)
+    best, best_ms, start_ms, hit = plan.autotune_cached(dev, cache, bufs, w.n_pairs, *arrs,
+                                                        sw_cfg(cgm), out, reps=1)
+    assert not hit and best_ms > 0 and start_ms > 0
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and lines[0].split("\t")[0] == key
+    dev.synchronize()
+    assert_same(tuple(dev.download(a, w.n_pairs) for a in (out.scores, out.query_end, out.ref_end)),
+                exp, "cached autotune, measured")
+    plan.close()
+    # a new plan on the same batch: the choice comes from the file
+    plan2 = cgm.MixedPlan(w.n_pairs, int(w.max_error.max()))
+    out2 = cgm.SwResult(w.n_pairs)
+    best2, ms2, start2, hit2 = plan2.autotune_cached(dev, cache, bufs, w.n_pairs, *arrs,
+                                                     sw_cfg(cgm), out2, reps=1)
+    assert hit2 and best2 == best and start2 == -1 and abs(ms2 - best_ms) < 1e-3
+    assert plan2.tuning() == best
+    dev.synchronize()
+    assert_same(tuple(dev.download(a, w.n_pairs) for a in (out2.scores, out2.query_end,
+                                                          out2.ref_end)), exp, "from the cache")
+    assert len(open(cache).read().splitlines()) == 1
+    plan2.close()
+    # SW pairs only: another profile, measured and appended
+    sw = gen_mod.generate(gen_mod.SW_100_300BP, 3000, seed=24)
+    up = Uploaded(dev, sw)
+    plan3 = cgm.MixedPlan(sw.n_pairs, 0)
+    out3 = cgm.SwResult(sw.n_pairs)
+    sbufs = (up.q, up.r, up.ql, up.rl, up.qo, up.ro)
+    key3 = plan3.profile_key(dev, sbufs, sw.n_pairs, None, up.band, sw_cfg(cgm))
+    assert key3 != key and re.search(r";sw\d+\.\d=", key3) and not re.search(r";wfa\d+\.", key3)
+    _, _, _, hit3 = plan3.autotune_cached(dev, cache, sbufs, sw.n_pairs, None, up.band, None,
+                                          sw_cfg(cgm), out3, reps=1)
+    assert not hit3 and len(open(cache).read().splitlines()) == 2
+    dev.synchronize()
+    from gpu_helpers import sw_oracle
+    assert_same(tuple(dev.download(a, sw.n_pairs) for a in (out3.scores, out3.query_end,
+                                                           out3.ref_end)),
+                sw_oracle(orc, sw, CONFIG_SW), "SW-only profile")
+    plan3.close()
