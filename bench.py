@@ -311,6 +311,34 @@ def mixed_extra(m, dev, gen, n_pairs=100_000, cpu=True):
         res["plain_wg_per_pair"]["pairs_per_s"]
     log(f"[bench] mixed plain: {res['plain_wg_per_pair']} speedup {res['speedup_vs_plain']:.2f}")
     plan.close()
+    # the same batch split by kind into the two drop-in launchers' automatic
+    # layouts (cgm_sw_banded_launch, cgm_wfa_affine_launch), one after the
+    # other on the stream: what the mixed persistent form adds beyond them
+    sw_idx, wfa_idx = np.nonzero(~is_wfa)[0], np.nonzero(is_wfa)[0]
+    sw_run = SwRunner(m, dev, w.subset(sw_idx))
+    ww = w.subset(wfa_idx)
+    wup = {k: dev.upload(getattr(ww, k)) for k in ("q_packed", "r_packed", "q_len", "r_len",
+                                                    "q_off", "r_off", "max_error")}
+    wout = m.DeviceArray(ww.n_pairs, np.int32)
+
+    def per_kind():
+        sw_run.launch(0)
+        m._lib.call("cgm_wfa_affine_launch", wup["q_packed"].dptr, wup["r_packed"].dptr,
+                    wup["q_len"].dptr, wup["r_len"].dptr, wup["q_off"].dptr, wup["r_off"].dptr,
+                    ww.n_pairs, wup["q_packed"].count, wup["r_packed"].count,
+                    wup["max_error"].dptr, 0, WFA_PEN["x"], WFA_PEN["o"], WFA_PEN["e"],
+                    wout.dptr, 0, dev.stream)
+
+    wall, kms = time_steps(m, dev, per_kind, 2, 1)
+    sw_got = sw_run.results()
+    wfa_got = dev.download(wout, ww.n_pairs)
+    res["per_kind_auto"] = {
+        "pairs_per_s": n * 2 / wall, "ms_per_step": wall / 2 * 1e3,
+        "identical": bool(all(np.array_equal(a, b[sw_idx]) for a, b in zip(sw_got, graph_res))
+                          and np.array_equal(wfa_got, graph_res[0][wfa_idx]))}
+    res["speedup_vs_per_kind_auto"] = res["persistent_graph"]["pairs_per_s"] / \
+        res["per_kind_auto"]["pairs_per_s"]
+    log(f"[bench] mixed per-kind automatic launchers: {res['per_kind_auto']}")
     if cpu:
         oracle, flags = load_oracle()
         thr = host_info()["threads_used"]
