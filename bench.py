@@ -383,7 +383,16 @@ def serving_extra(m, duration=3.0):
         res[name] = eng.run_load(g.handle, rate, duration)
         res[name]["offered_rps"] = rate
     res["at_50k_rps"] = eng.run_load(g.handle, 50_000.0, duration)
-    for k in ("peak_closed_loop", "at_70pct_peak", "at_10pct_peak", "at_50k_rps"):
+    # the same load with the sequences resident in HBM (uploaded once,
+    # submitted as device pointers: only metadata crosses PCIe per request)
+    res["device_resident_peak"] = eng.run_load(g.handle, 0.0, duration, max_outstanding=16384,
+                                               device_inputs=True)
+    rate = 0.7 * res["device_resident_peak"]["requests_per_s"]
+    res["device_resident_at_70pct_peak"] = eng.run_load(g.handle, rate, duration,
+                                                        device_inputs=True)
+    res["device_resident_at_70pct_peak"]["offered_rps"] = rate
+    for k in ("peak_closed_loop", "at_70pct_peak", "at_10pct_peak", "at_50k_rps",
+              "device_resident_peak", "device_resident_at_70pct_peak"):
         r = res[k]
         log(f"[bench] serving {k}: {r['requests_per_s']:.0f} req/s {r['pairs_per_s']:.0f} pairs/s "
             f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms | SW-only p50 "

@@ -99,6 +99,8 @@ SIGNATURES = {
     "cgm_engine_destroy": [_vp],
     "cgm_engine_submit": [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp,
                           ctypes.c_double, _pi64],
+    "cgm_engine_submit_device": [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
+                                 _vp, ctypes.c_double, _pi64],
     "cgm_engine_poll": [_vp, _i64, _pi32],
     "cgm_engine_wait": [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)],
     "cgm_engine_stats": [_vp, _pi64],
@@ -109,7 +111,8 @@ SIGNATURES = {
     "cgm_engine_get_tuning": [_vp, _i32, _vp],
     "cgm_engine_set_tuning": [_vp, _i32, _vp],
     "cgm_engine_now": [],
-    "cgm_serve_run": [_vp, _vp, ctypes.c_double, ctypes.c_double, _i32, _i32, _i32, _u64, _vp],
+    "cgm_serve_run": [_vp, _vp, ctypes.c_double, ctypes.c_double, _i32, _i32, _i32, _i32, _u64,
+                      _vp],
     "cgm_sw_banded_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,
                            _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32],
     "cgm_wfa_affine_host": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32,

@@ -76,7 +76,10 @@ struct Staging {
     int32_t *h_meta = nullptr;
     uint64_t *h_words = nullptr;
     int32_t *h_out = nullptr;
-    int64_t n = 0, words = 0;
+    // words of device-resident requests: (source address, destination word,
+    // word count) per sequence, gathered on the device at launch
+    int64_t *h_gather = nullptr;
+    int64_t n = 0, words = 0, ngather = 0;
     std::vector<Route> routes;
     int writers = 0;  // submits copying into their reserved part (under the bin's lock)
     Bin *bin = nullptr;
@@ -89,6 +92,7 @@ struct Slot {
     MixPlan *plan = nullptr;
     int32_t *d_meta = nullptr, *d_arr = nullptr, *d_out = nullptr;
     uint64_t *d_words = nullptr;
+    int64_t *d_gather = nullptr;
 };
 
 struct Bin {
@@ -107,6 +111,18 @@ struct Bin {
     std::atomic<int64_t> batches{0}, batch_pairs{0};
     std::thread dispatcher;
 };
+
+// Words of device-resident requests: one wavefront per descriptor
+// (source address, destination word, word count), lanes copying words.
+__global__ void engine_gather_kernel(const int64_t *desc, int64_t n, uint64_t *dst) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t d = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; d < n; d += nwaves) {
+        const uint64_t *src = (const uint64_t *)desc[3 * d];
+        const int64_t o = desc[3 * d + 1], cnt = desc[3 * d + 2];
+        for (int64_t j = lane; j < cnt; j += 64) dst[o + j] = src[j];
+    }
+}
 
 // metadata of n pairs (kMeta ints each) -> kMeta arrays of stride cap
 __global__ void engine_unpack_kernel(const int32_t *meta, int64_t n, int32_t *arr, int64_t cap) {
@@ -150,13 +166,13 @@ struct Engine {
             for (Slot &s : B.slots) {
                 if (s.stream) (void)hipStreamSynchronize(s.stream);
                 if (s.plan) mix_plan_destroy(s.plan);
-                void *dev_ptrs[] = {s.d_meta, s.d_arr, s.d_out, s.d_words};
+                void *dev_ptrs[] = {s.d_meta, s.d_arr, s.d_out, s.d_words, s.d_gather};
                 for (void *p : dev_ptrs)
                     if (p) (void)hipFree(p);
                 if (s.stream) (void)hipStreamDestroy(s.stream);
             }
             for (Staging &st : B.stages) {
-                void *host_ptrs[] = {st.h_meta, st.h_words, st.h_out};
+                void *host_ptrs[] = {st.h_meta, st.h_words, st.h_out, st.h_gather};
                 for (void *p : host_ptrs)
                     if (p) (void)hipHostFree(p);
             }
@@ -188,6 +204,18 @@ struct Engine {
         ok(hipMemcpyAsync(S.d_meta, st->h_meta, (size_t)(p * kMeta * 4), hipMemcpyHostToDevice, s));
         if (w > 0)
             ok(hipMemcpyAsync(S.d_words, st->h_words, (size_t)(w * 8), hipMemcpyHostToDevice, s));
+        if (st->ngather > 0 && rc == CGM_OK) {
+            // device-resident requests: their words over the (unused) parts
+            // of the host copy reserved for them
+            ok(hipMemcpyAsync(S.d_gather, st->h_gather, (size_t)(st->ngather * 3 * 8),
+                              hipMemcpyHostToDevice, s));
+            if (rc == CGM_OK) {
+                const int64_t blocks = std::min<int64_t>((st->ngather + 3) / 4, 2048);
+                hipLaunchKernelGGL(engine_gather_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                                   S.d_gather, st->ngather, S.d_words);
+                ok(hipGetLastError());
+            }
+        }
         if (rc == CGM_OK) {
             hipLaunchKernelGGL(engine_unpack_kernel, dim3((unsigned)((p + 255) / 256)), dim3(256), 0,
                                s, S.d_meta, p, S.d_arr, cap);
@@ -293,6 +321,7 @@ void engine_host_done(void *arg) {
         B.free_slots.push_back(st->slot);
         st->n = 0;
         st->words = 0;
+        st->ngather = 0;
         st->rc = CGM_OK;
         st->slot = -1;
         st->routes.clear();
@@ -355,13 +384,15 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
                 // kEngineLaneMaxLen take one wavefront each instead of a lane
                 if (bi == 0) B.tune.lane_max_len = kEngineLaneMaxLen;
             }
-            E->device_bytes += mix_plan_bytes(s.plan) + wcap * 8 + (2 * kMeta + 3) * cap * 4;
+            E->device_bytes += mix_plan_bytes(s.plan) + wcap * 8 + (2 * kMeta + 3) * cap * 4 +
+                               2 * cap * 3 * 8;
             if (E->device_bytes > hbm_budget_bytes)
                 return fail(set_error(CGM_ERR_NOMEM, "engine: slots exceed the HBM budget"));
             if (hipMalloc((void **)&s.d_meta, (size_t)(kMeta * cap * 4)) != hipSuccess ||
                 hipMalloc((void **)&s.d_arr, (size_t)(kMeta * cap * 4)) != hipSuccess ||
                 hipMalloc((void **)&s.d_out, (size_t)(3 * cap * 4)) != hipSuccess ||
-                hipMalloc((void **)&s.d_words, (size_t)(wcap * 8)) != hipSuccess)
+                hipMalloc((void **)&s.d_words, (size_t)(wcap * 8)) != hipSuccess ||
+                hipMalloc((void **)&s.d_gather, (size_t)(2 * cap * 3 * 8)) != hipSuccess)
                 return fail(set_error(CGM_ERR_NOMEM, "engine: slot buffers"));
             B.free_slots.push_back(n_slots - 1 - k);
         }
@@ -374,7 +405,9 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
                 hipHostMalloc((void **)&st.h_words, (size_t)(wcap * 8), hipHostMallocDefault) !=
                     hipSuccess ||
                 hipHostMalloc((void **)&st.h_out, (size_t)(3 * cap * 4), hipHostMallocDefault) !=
-                    hipSuccess)
+                    hipSuccess ||
+                hipHostMalloc((void **)&st.h_gather, (size_t)(2 * cap * 3 * 8),
+                              hipHostMallocDefault) != hipSuccess)
                 return fail(set_error(CGM_ERR_NOMEM, "engine: pinned staging"));
         }
         B.open = &B.stages[0];
@@ -393,11 +426,18 @@ int cgm_engine_destroy(void *engine) {
     return CGM_OK;
 }
 
-int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
-                      const uint64_t *r_packed, int64_t r_words, const int32_t *q_len,
-                      const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
-                      int32_t n_pairs, const int32_t *kind, const int32_t *band,
-                      const int32_t *max_error, double arrival_s, int64_t *request_id) {
+}  // extern "C"
+
+namespace {
+// cgm_engine_submit (device_words false: q_packed / r_packed are host
+// memory, copied into the pinned staging buffer) and
+// cgm_engine_submit_device (true: they are device memory, gathered on the
+// device when the batch launches; the staging buffer gets descriptors).
+int engine_submit(void *engine, bool device_words, const uint64_t *q_packed, int64_t q_words,
+                  const uint64_t *r_packed, int64_t r_words, const int32_t *q_len,
+                  const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                  int32_t n_pairs, const int32_t *kind, const int32_t *band,
+                  const int32_t *max_error, double arrival_s, int64_t *request_id) {
     Engine *E = (Engine *)engine;
     if (!E || !request_id || n_pairs < 1 || !q_len || !r_len || !q_off || !r_off || !kind ||
         !band || !max_error || q_words < 0 || r_words < 0)
@@ -454,7 +494,7 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
     // reserve the request's part of each bin's open staging buffer under the
     // locks, copy into it after releasing them
     Staging *stg[kBins] = {nullptr, nullptr};
-    int64_t n0[kBins] = {0, 0}, w0[kBins] = {0, 0};
+    int64_t n0[kBins] = {0, 0}, w0[kBins] = {0, 0}, g0[kBins] = {0, 0};
     for (int bi = 0; bi < kBins; ++bi) {
         if (nb[bi] == 0) continue;
         Bin &B = E->bins[bi];
@@ -467,8 +507,10 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
         stg[bi] = st;
         n0[bi] = st->n;
         w0[bi] = st->words;
+        g0[bi] = st->ngather;
         st->n += nb[bi];
         st->words += wb[bi];
+        if (device_words) st->ngather += 2 * nb[bi];
         st->routes.resize((size_t)st->n);  // within the capacity reserved at creation
         st->writers += 1;
         B.pending_pairs += nb[bi];
@@ -478,7 +520,7 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
     for (int bi = 0; bi < kBins; ++bi) {
         Staging *st = stg[bi];
         if (!st) continue;
-        int64_t n = n0[bi], words = w0[bi];
+        int64_t n = n0[bi], words = w0[bi], gd = g0[bi];
         for (int32_t i = 0; i < n_pairs; ++i) {
             if (kind[i] != bi) continue;
             const int64_t qw = ((int64_t)q_len[i] + 31) / 32, rw = ((int64_t)r_len[i] + 31) / 32;
@@ -490,8 +532,19 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
             m[4] = kind[i];
             m[5] = band[i];
             m[6] = max_error[i];
-            if (qw) memcpy(st->h_words + words, q_packed + q_off[i], (size_t)qw * 8);
-            if (rw) memcpy(st->h_words + words + qw, r_packed + r_off[i], (size_t)rw * 8);
+            if (device_words) {
+                int64_t *g = st->h_gather + 3 * gd;
+                g[0] = (int64_t)(uintptr_t)(q_packed + q_off[i]);
+                g[1] = words;
+                g[2] = qw;
+                g[3] = (int64_t)(uintptr_t)(r_packed + r_off[i]);
+                g[4] = words + qw;
+                g[5] = rw;
+                gd += 2;
+            } else {
+                if (qw) memcpy(st->h_words + words, q_packed + q_off[i], (size_t)qw * 8);
+                if (rw) memcpy(st->h_words + words + qw, r_packed + r_off[i], (size_t)rw * 8);
+            }
             words += qw + rw;
             st->routes[(size_t)n] = {r, i};
             n += 1;
@@ -514,6 +567,27 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
     }
     *request_id = r->id;
     return CGM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
+                      const uint64_t *r_packed, int64_t r_words, const int32_t *q_len,
+                      const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                      int32_t n_pairs, const int32_t *kind, const int32_t *band,
+                      const int32_t *max_error, double arrival_s, int64_t *request_id) {
+    return engine_submit(engine, false, q_packed, q_words, r_packed, r_words, q_len, r_len, q_off,
+                         r_off, n_pairs, kind, band, max_error, arrival_s, request_id);
+}
+
+int cgm_engine_submit_device(void *engine, const uint64_t *q_packed, int64_t q_words,
+                             const uint64_t *r_packed, int64_t r_words, const int32_t *q_len,
+                             const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                             int32_t n_pairs, const int32_t *kind, const int32_t *band,
+                             const int32_t *max_error, double arrival_s, int64_t *request_id) {
+    return engine_submit(engine, true, q_packed, q_words, r_packed, r_words, q_len, r_len, q_off,
+                         r_off, n_pairs, kind, band, max_error, arrival_s, request_id);
 }
 
 int cgm_engine_poll(void *engine, int64_t request_id, int32_t *done) {
@@ -596,7 +670,7 @@ double cgm_engine_now(void) { return now_s(); }
 
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                   int32_t max_pairs_per_request, int32_t max_outstanding, int32_t clients,
-                  uint64_t seed, double *stats /* 24 */) {
+                  int32_t device_inputs, uint64_t seed, double *stats /* 24 */) {
     Engine *E = (Engine *)engine;
     if (!E || !gen || !stats || duration_s <= 0 || max_pairs_per_request < 1 || clients < 1 ||
         clients > 64)
@@ -611,6 +685,25 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
     if (int rc = cgm_gen_copy(gen, qp.data(), nullptr, qo.data(), ql.data(), rp.data(), nullptr,
                               ro.data(), rl.data(), kind.data(), band.data(), me.data()))
         return rc;
+    // device-resident inputs: the packed words are uploaded once and every
+    // request is submitted with device pointers
+    uint64_t *qd = nullptr, *rd = nullptr;
+    struct DevFree {
+        uint64_t *&a, *&b;
+        ~DevFree() {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+        }
+    } dev_free{qd, rd};
+    if (device_inputs) {
+        CGM_HIP(hipSetDevice(E->device));
+        CGM_HIP(hipMalloc((void **)&qd, (size_t)std::max<int64_t>(qwn, 1) * 8));
+        CGM_HIP(hipMalloc((void **)&rd, (size_t)std::max<int64_t>(rwn, 1) * 8));
+        CGM_HIP(hipMemcpy(qd, qp.data(), (size_t)qwn * 8, hipMemcpyHostToDevice));
+        CGM_HIP(hipMemcpy(rd, rp.data(), (size_t)rwn * 8, hipMemcpyHostToDevice));
+    }
+    const uint64_t *q_src = device_inputs ? qd : qp.data();
+    const uint64_t *r_src = device_inputs ? rd : rp.data();
     int64_t b0[kBins], bp0[kBins];
     for (int bi = 0; bi < kBins; ++bi) {
         b0[bi] = E->bins[bi].batches;
@@ -687,10 +780,10 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
             const size_t k = (size_t)cursor;
             bool wfa = false;
             for (int i = 0; i < n; ++i) wfa |= kind[k + (size_t)i] == 1;
-            const int rc = cgm_engine_submit(E, qp.data(), qwn, rp.data(), rwn, ql.data() + k,
-                                             rl.data() + k, qo.data() + k, ro.data() + k, n,
-                                             kind.data() + k, band.data() + k, me.data() + k,
-                                             arrival, &id);
+            const int rc = engine_submit(E, device_inputs != 0, q_src, qwn, r_src, rwn,
+                                         ql.data() + k, rl.data() + k, qo.data() + k,
+                                         ro.data() + k, n, kind.data() + k, band.data() + k,
+                                         me.data() + k, arrival, &id);
             cursor += n;
             if (rate_rps > 0) t_next += gap(rng);
             if (rc == CGM_OK) {

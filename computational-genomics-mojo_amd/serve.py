@@ -44,6 +44,20 @@ class ServingEngine:
                   *[_lib.ptr(a) for a in arrs[4:]], float(arrival_s), ctypes.byref(rid))
         return rid.value
 
+    def submit_device(self, q_dev, r_dev, q_len, r_len, q_off, r_off, kind, band, max_error,
+                      arrival_s: float = 0.0) -> int:
+        """Queues one request whose packed words are already on the device
+        (q_dev / r_dev: DeviceArray of uint64, kept unchanged until the
+        request is done); the metadata arrays are host arrays."""
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (q_len, r_len, q_off, r_off, kind,
+                                                             band, max_error)]
+        rid = ctypes.c_int64()
+        _lib.call("cgm_engine_submit_device", self.handle, q_dev.dptr, int(q_dev.count),
+                  r_dev.dptr, int(r_dev.count), *[_lib.ptr(a) for a in arrs[:4]],
+                  int(arrs[0].size), *[_lib.ptr(a) for a in arrs[4:]], float(arrival_s),
+                  ctypes.byref(rid))
+        return rid.value
+
     def submit_pairs(self, w, idx, arrival_s: float = 0.0) -> int:
         """Queues pairs idx of a gen.Workload as one request."""
         idx = np.asarray(idx, np.int64)
@@ -87,13 +101,15 @@ class ServingEngine:
 
     def run_load(self, gen_handle, rate_rps: float, duration_s: float,
                  max_pairs_per_request: int = 64, max_outstanding: int = 512,
-                 seed: int = 1, clients: int = 1) -> dict:
+                 seed: int = 1, clients: int = 1, device_inputs: bool = False) -> dict:
         """Native load generator (cgm_serve_run): `clients` client threads
         with Poisson arrivals at rate_rps in total (<= 0: closed loop with
-        max_outstanding requests over all clients)."""
+        max_outstanding requests over all clients); device_inputs: the
+        sequences are uploaded once and submitted as device pointers."""
         st = np.zeros(len(STAT_NAMES), np.float64)
         _lib.call("cgm_serve_run", self.handle, gen_handle, float(rate_rps), float(duration_s),
-                  max_pairs_per_request, max_outstanding, clients, seed, _lib.ptr(st))
+                  max_pairs_per_request, max_outstanding, clients, int(bool(device_inputs)),
+                  seed, _lib.ptr(st))
         return dict(zip(STAT_NAMES, st.tolist()))
 
     def close(self) -> None:

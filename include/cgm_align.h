@@ -442,6 +442,18 @@ int cgm_engine_submit(void *engine, const uint64_t *q_packed, int64_t q_words,
                       const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
                       int32_t n_pairs, const int32_t *kind, const int32_t *band,
                       const int32_t *max_error, double arrival_s, int64_t *request_id);
+/* cgm_engine_submit for a request whose packed words are already in device
+ * memory (q_packed / r_packed device pointers, q_words / r_words their
+ * lengths, e.g. a batch encoded on the device by cgm_encode_ascii_launch);
+ * lengths, offsets, kind, band and max_error stay host arrays.  The words
+ * are gathered on the device into the batch when it launches, so only the
+ * metadata crosses PCIe; the caller keeps the device buffers unchanged until
+ * the request is done. */
+int cgm_engine_submit_device(void *engine, const uint64_t *q_packed, int64_t q_words,
+                             const uint64_t *r_packed, int64_t r_words, const int32_t *q_len,
+                             const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
+                             int32_t n_pairs, const int32_t *kind, const int32_t *band,
+                             const int32_t *max_error, double arrival_s, int64_t *request_id);
 /* done = 1 once the request's results are ready. */
 int cgm_engine_poll(void *engine, int64_t request_id, int32_t *done);
 /* Blocks (timeout_ms < 0: forever) until the request is done, copies its
@@ -465,7 +477,10 @@ double cgm_engine_now(void);
  * slice of a generated batch (cgm_gen_create), with Poisson arrivals at
  * rate_rps / clients each for duration_s (rate_rps <= 0: a closed loop with
  * at most max_outstanding requests in flight over all clients), and each
- * collecting its results in submission order.  stats[24]: submitted,
+ * collecting its results in submission order.  device_inputs = 1: the
+ * batch's packed words are uploaded once and requests go through
+ * cgm_engine_submit_device (sequences resident in HBM); 0: through
+ * cgm_engine_submit (host words copied per request).  stats[24]: submitted,
  * completed, refused, pairs completed, elapsed s, pairs/s, requests/s, p50,
  * p90, p99, max, mean latency (ms), batches, mean pairs per batch, bad
  * results, submit phase s, then SW-only requests: count, p50, p99 (ms);
@@ -473,7 +488,7 @@ double cgm_engine_now(void);
  * per WFA batch. */
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                   int32_t max_pairs_per_request, int32_t max_outstanding, int32_t clients,
-                  uint64_t seed, double *stats);
+                  int32_t device_inputs, uint64_t seed, double *stats);
 
 /* -------------------------------------------- host-pointer conveniences
  * Same as the launch entry points but every pointer is a host pointer; the

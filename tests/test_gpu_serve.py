@@ -215,3 +215,43 @@ def test_engine_concurrent_submitters(cgm, dev, orc, serving_pairs):
     assert st["completed"] > 0 and st["bad_results"] == 0
     g.close()
     eng.close()
+
+
+def test_engine_device_resident_requests(cgm, dev, orc, serving_pairs):
+    """Requests whose packed words are already in HBM (cgm_engine_submit_device:
+    gathered on the device when their batch launches) interleaved with
+    host-word requests in the same batches; every result equals the oracle,
+    and the load generator with device-resident inputs reports no bad
+    result."""
+    w = serving_pairs
+    eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=2, max_batch_pairs=2048,
+                            max_batch_words=1 << 17, max_error_cap=8192)
+    qd, rd = dev.upload(w.q_packed), dev.upload(w.r_packed)
+    reqs = requests_of(w.n_pairs, np.random.default_rng(9))
+    ids = []
+    for k, idx in enumerate(reqs):
+        if k % 2 == 0:
+            rid = eng.submit_device(qd, rd, w.q_len[idx], w.r_len[idx], w.q_off[idx],
+                                    w.r_off[idx], w.kind[idx], w.band[idx], w.max_error[idx])
+        else:
+            rid = eng.submit_pairs(w, idx)
+        ids.append((idx, rid))
+    got = [np.zeros(w.n_pairs, np.int32) for _ in range(3)]
+    for idx, rid in ids:
+        s, q, r, _ = eng.wait(rid, idx.size)
+        got[0][idx], got[1][idx], got[2][idx] = s, q, r
+    print(f"[case] device-resident and host requests: {len(reqs)}, stats {eng.stats()}", flush=True)
+    assert_same(tuple(got), oracle(orc, w), "device-resident requests vs oracle")
+    # a device pair outside the declared buffer length is refused before any copy
+    with pytest.raises(cgm._lib.CgmError) as ei:
+        idx = reqs[-1]
+        eng.submit_device(dev.upload(w.q_packed[:1]), rd, w.q_len[idx], w.r_len[idx],
+                          w.q_off[idx], w.r_off[idx], w.kind[idx], w.band[idx],
+                          w.max_error[idx])
+    assert ei.value.code == cgm._lib.CGM_ERR_INVALID
+    g = cgm.GeneratorHandle(5, 20000, 6)
+    st = eng.run_load(g.handle, 0.0, 0.3, max_outstanding=256, device_inputs=True)
+    print(f"[case] device-resident closed loop {st}", flush=True)
+    assert st["completed"] > 0 and st["bad_results"] == 0
+    g.close()
+    eng.close()
