@@ -10,7 +10,8 @@
 // buffers.  cgm_engine_submit writes a request's pairs straight into the
 // bin's open staging buffer (metadata and packed words, the only host
 // copy); whenever a slot of the bin is free the dispatcher takes the oldest
-// closed staging buffer, or the open one, and launches it: two host-to-
+// closed staging buffer, or the open one (in the SW bin only a closed one
+// once two of its batches are in flight), and launches it: two host-to-
 // device copies, a kernel that spreads the metadata into the batch arrays,
 // the mixed-kernel plan, one device-to-host copy of the results and a host
 // function on the stream that hands the results to their requests and
@@ -30,9 +31,9 @@
 #include <condition_variable>
 #include <deque>
 #include <mutex>
+#include <new>
 #include <random>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "cgm_internal.h"
@@ -51,10 +52,13 @@ double now_s() {
 
 constexpr int kBins = 2;   // 0: SW, 1: WFA (the pair kind)
 constexpr int kMeta = 7;   // per pair: q_len, r_len, q_off, r_off, kind, band, max_error
+constexpr int kEagerSwSlots = 2;  // SW batches in flight before only full buffers launch
 constexpr int kEngineLaneMaxLen = 200;  // SW reads longer than this: one wavefront per pair
 
 struct Request {
-    int64_t id = 0;
+    std::atomic<int64_t> id{0};  // 0 while the slot is free
+    int32_t slot = 0;            // index in the engine's request table
+    uint32_t gen = 0;            // uses of the slot so far (part of the id)
     int32_t n = 0;
     std::vector<int32_t> score, qend, rend;
     std::atomic<int32_t> remaining{0};
@@ -80,6 +84,7 @@ struct Staging {
     // word count) per sequence, gathered on the device at launch
     int64_t *h_gather = nullptr;
     int64_t n = 0, words = 0, ngather = 0;
+    int32_t max_band = 0, max_len = 0;  // over the batch's pairs
     std::vector<Route> routes;
     int writers = 0;  // submits copying into their reserved part (under the bin's lock)
     Bin *bin = nullptr;
@@ -108,8 +113,25 @@ struct Bin {
     Staging *open = nullptr;
     int64_t pending_pairs = 0;
     MixTuning tune{};  // applied to the slot's plan at every launch
+    int eager = 0;     // batches in flight beyond which only a closed buffer launches
     std::atomic<int64_t> batches{0}, batch_pairs{0};
     std::thread dispatcher;
+
+    // Whether the dispatcher can launch now (caller holds mu): a slot is
+    // free, and the oldest closed buffer, else the open one, holds pairs and
+    // no submit is still writing into it.  Beyond `eager` batches in flight
+    // only a closed (full) buffer is launched: under light load the SW bin
+    // keeps few batches running, each finishing sooner, and its extra slots
+    // only add concurrency once requests back up.  The WFA bin launches on
+    // every slot: its batches last as long as their longest pair, and the
+    // GPU fills only with several of them in flight.
+    bool launchable() const {
+        if (free_slots.empty()) return false;
+        const int busy = (int)(slots.size() - free_slots.size());
+        if (ready.empty() && busy >= eager) return false;
+        const Staging *c = !ready.empty() ? ready.front() : open;
+        return c->n > 0 && c->writers == 0;
+    }
 };
 
 // Words of device-resident requests: one wavefront per descriptor
@@ -143,14 +165,24 @@ struct Engine {
     SwScoring sc{};
     WfaPen pen{};
     Bin bins[kBins];
-    std::mutex mu;  // requests: live map, pool, cv_done
+    std::mutex mu;  // requests' done state, cv_done
     std::condition_variable cv_done;
-    std::unordered_map<int64_t, Request *> live;
-    std::vector<Request *> pool;
-    int64_t next_id = 1;
+    // Request table: a request's id is (use count of its slot << 32) | slot;
+    // slots live in chunks that are never moved or freed before the engine,
+    // so a lookup by id takes no lock, and submits register a request
+    // without touching `mu`.
+    static constexpr int kReqChunkBits = 12, kReqChunks = 4096;
+    std::atomic<Request *> req_chunks[kReqChunks];
+    std::mutex req_mu;  // free_req, n_req, chunk allocation
+    std::vector<int32_t> free_req;
+    int32_t n_req = 0;
     std::atomic<bool> stop{false};
     std::atomic<int64_t> done_requests{0}, done_pairs{0}, refused{0};
     std::atomic<int> worker_error{0};
+
+    Engine() {
+        for (auto &c : req_chunks) c.store(nullptr, std::memory_order_relaxed);
+    }
 
     ~Engine() {
         stop = true;
@@ -177,16 +209,52 @@ struct Engine {
                     if (p) (void)hipHostFree(p);
             }
         }
-        for (auto &kv : live) delete kv.second;
-        for (Request *r : pool) delete r;
+        for (auto &c : req_chunks) delete[] c.load();
     }
 
+    // A free request slot (its id still 0), or null when the table is full
+    // or a chunk cannot be allocated.
     Request *take_request() {
-        std::lock_guard<std::mutex> lk(mu);
-        if (pool.empty()) return new Request();
-        Request *r = pool.back();
-        pool.pop_back();
+        constexpr int kChunk = 1 << kReqChunkBits;
+        std::lock_guard<std::mutex> lk(req_mu);
+        int32_t slot;
+        if (!free_req.empty()) {
+            slot = free_req.back();
+            free_req.pop_back();
+        } else {
+            if (n_req == kReqChunks * kChunk) return nullptr;
+            slot = n_req;
+            const int c = slot >> kReqChunkBits;
+            if (!req_chunks[c].load(std::memory_order_relaxed)) {
+                Request *chunk = new (std::nothrow) Request[kChunk];
+                if (!chunk) return nullptr;
+                for (int i = 0; i < kChunk; ++i) chunk[i].slot = c * kChunk + i;
+                req_chunks[c].store(chunk, std::memory_order_release);
+            }
+            n_req += 1;
+        }
+        Request *r = &req_chunks[slot >> kReqChunkBits].load(std::memory_order_relaxed)
+                          [slot & (kChunk - 1)];
+        r->gen = (r->gen + 1) & 0x7fffffffu;
+        if (r->gen == 0) r->gen = 1;
         return r;
+    }
+
+    // The live request with this id, or null.
+    Request *find_request(int64_t id) {
+        if (id <= 0) return nullptr;
+        const int64_t slot = id & 0xffffffff;
+        if (slot >= (int64_t)kReqChunks << kReqChunkBits) return nullptr;
+        Request *chunk = req_chunks[slot >> kReqChunkBits].load(std::memory_order_acquire);
+        if (!chunk) return nullptr;
+        Request *r = &chunk[slot & ((1 << kReqChunkBits) - 1)];
+        return r->id.load(std::memory_order_acquire) == id ? r : nullptr;
+    }
+
+    // Returns a slot whose id the caller has set to 0.
+    void release_request(Request *r) {
+        std::lock_guard<std::mutex> lk(req_mu);
+        free_req.push_back(r->slot);
     }
 
     // Launches staging `st` on slot `si` of bin B; on any failure the batch's
@@ -194,6 +262,7 @@ struct Engine {
     void launch(Bin &B, int si, Staging *st, const MixTuning &tune) {
         Slot &S = B.slots[(size_t)si];
         mix_plan_set_tuning(S.plan, tune);
+        mix_plan_set_hint(S.plan, st->max_band, st->max_len);
         hipStream_t s = S.stream;
         const int64_t p = st->n, w = st->words, cap = cap_pairs;
         st->slot = si;
@@ -249,12 +318,7 @@ struct Engine {
                 std::unique_lock<std::mutex> lk(B.mu);
                 // the oldest closed staging buffer, else the open one; either
                 // only once every submit writing into it has finished
-                B.cv.wait(lk, [&] {
-                    if (stop.load()) return true;
-                    if (B.free_slots.empty()) return false;
-                    const Staging *c = !B.ready.empty() ? B.ready.front() : B.open;
-                    return c->n > 0 && c->writers == 0;
-                });
+                B.cv.wait(lk, [&] { return stop.load() || B.launchable(); });
                 if (stop.load()) return;
                 if (!B.ready.empty()) {
                     st = B.ready.front();
@@ -322,6 +386,8 @@ void engine_host_done(void *arg) {
         st->n = 0;
         st->words = 0;
         st->ngather = 0;
+        st->max_band = 0;
+        st->max_len = 0;
         st->rc = CGM_OK;
         st->slot = -1;
         st->routes.clear();
@@ -356,7 +422,6 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
     };
     if (hipGetDevice(&E->device) != hipSuccess) return fail(set_error(CGM_ERR_NODEV, "no device"));
     E->n_slots = n_slots;
-    E->live.reserve(1 << 16);
     E->cap_pairs = max_batch_pairs;
     E->cap_words = max_batch_words;
     E->sc = sc;
@@ -366,6 +431,7 @@ int cgm_engine_create(int64_t hbm_budget_bytes, int32_t n_slots, int64_t max_bat
         Bin &B = E->bins[bi];
         B.id = bi;
         B.E = E;
+        B.eager = bi == 0 ? kEagerSwSlots : n_slots;
         B.slots.resize((size_t)n_slots);
         for (int k = 0; k < n_slots; ++k) {
             Slot &s = B.slots[(size_t)k];
@@ -444,6 +510,7 @@ int engine_submit(void *engine, bool device_words, const uint64_t *q_packed, int
         return set_error(CGM_ERR_INVALID, "cgm_engine_submit: bad arguments");
     // every pair's words must lie inside the caller's buffers
     int64_t nb[kBins] = {0, 0}, wb[kBins] = {0, 0};
+    int32_t mband[kBins] = {0, 0}, mlen[kBins] = {0, 0};
     for (int32_t i = 0; i < n_pairs; ++i) {
         if (q_len[i] < 0 || r_len[i] < 0 || q_off[i] < 0 || r_off[i] < 0)
             return set_error(CGM_ERR_INVALID, "cgm_engine_submit: negative length or offset");
@@ -456,38 +523,42 @@ int engine_submit(void *engine, bool device_words, const uint64_t *q_packed, int
         const int bi = kind[i];
         nb[bi] += 1;
         wb[bi] += qw + rw;
+        mband[bi] = std::max(mband[bi], band[i]);
+        mlen[bi] = std::max(mlen[bi], std::max(q_len[i], r_len[i]));
     }
     for (int bi = 0; bi < kBins; ++bi)
         if (nb[bi] > E->cap_pairs || wb[bi] > 2 * E->cap_words)
             return set_error(CGM_ERR_INVALID, "cgm_engine_submit: request larger than a batch");
     Request *r = E->take_request();
+    if (!r) return set_error(CGM_ERR_NOMEM, "cgm_engine_submit: request table full");
     r->n = n_pairs;
-    r->score.assign((size_t)n_pairs, 0);
-    r->qend.assign((size_t)n_pairs, 0);
-    r->rend.assign((size_t)n_pairs, 0);
+    // every pair's result is written before the request is marked done
+    r->score.resize((size_t)n_pairs);
+    r->qend.resize((size_t)n_pairs);
+    r->rend.resize((size_t)n_pairs);
     r->remaining = n_pairs;
     r->state = 0;
     r->has_wfa = nb[1] > 0;
     r->t_arrival = arrival_s > 0 ? arrival_s : now_s();
     r->t_done = 0;
-    {
-        std::lock_guard<std::mutex> lk(E->mu);
-        r->id = E->next_id++;
-        E->live[r->id] = r;
-    }
-    // both bins' locks, in bin order, so the request is admitted whole or not at all
-    std::unique_lock<std::mutex> l0(E->bins[0].mu), l1(E->bins[1].mu);
-    bool fits_open[kBins];
+    const int64_t id = ((int64_t)r->gen << 32) | r->slot;
+    r->id.store(id, std::memory_order_release);
+    // the locks of the bins the request uses, in bin order, so the request
+    // is admitted whole or not at all
+    std::unique_lock<std::mutex> lks[kBins];
+    for (int bi = 0; bi < kBins; ++bi)
+        if (nb[bi] > 0) lks[bi] = std::unique_lock<std::mutex>(E->bins[bi].mu);
+    bool fits_open[kBins] = {true, true};
     for (int bi = 0; bi < kBins; ++bi) {
+        if (nb[bi] == 0) continue;
         Bin &B = E->bins[bi];
         fits_open[bi] = B.open->n + nb[bi] <= E->cap_pairs && B.open->words + wb[bi] <= 2 * E->cap_words;
-        if (nb[bi] > 0 && !fits_open[bi] && B.free_stages.empty()) {
-            l1.unlock();
-            l0.unlock();
+        if (!fits_open[bi] && B.free_stages.empty()) {
+            for (int bj = kBins - 1; bj >= 0; --bj)
+                if (lks[bj].owns_lock()) lks[bj].unlock();
             E->refused += 1;
-            std::lock_guard<std::mutex> lk(E->mu);
-            E->live.erase(r->id);
-            E->pool.push_back(r);
+            r->id.store(0, std::memory_order_release);
+            E->release_request(r);
             return set_error(CGM_ERR_BUSY, "engine queue full");
         }
     }
@@ -515,8 +586,8 @@ int engine_submit(void *engine, bool device_words, const uint64_t *q_packed, int
         st->writers += 1;
         B.pending_pairs += nb[bi];
     }
-    l1.unlock();
-    l0.unlock();
+    for (int bj = kBins - 1; bj >= 0; --bj)
+        if (lks[bj].owns_lock()) lks[bj].unlock();
     for (int bi = 0; bi < kBins; ++bi) {
         Staging *st = stg[bi];
         if (!st) continue;
@@ -550,8 +621,8 @@ int engine_submit(void *engine, bool device_words, const uint64_t *q_packed, int
             n += 1;
         }
     }
-    // the dispatcher only acts when a slot is free and the buffer has no
-    // writer left; when no slot is free, the slot's completion wakes it, and
+    // the dispatcher only acts when it can launch; otherwise a slot's
+    // completion or the last writer of the buffer it waits for wakes it, and
     // a wake-up here would cost a futex round trip per request for nothing
     for (int bi = 0; bi < kBins; ++bi) {
         Staging *st = stg[bi];
@@ -560,12 +631,14 @@ int engine_submit(void *engine, bool device_words, const uint64_t *q_packed, int
         bool wake;
         {
             std::lock_guard<std::mutex> lk(B.mu);
+            st->max_band = std::max(st->max_band, mband[bi]);
+            st->max_len = std::max(st->max_len, mlen[bi]);
             st->writers -= 1;
-            wake = st->writers == 0 && !B.free_slots.empty();
+            wake = st->writers == 0 && B.launchable();
         }
         if (wake) B.cv.notify_one();
     }
-    *request_id = r->id;
+    *request_id = id;
     return CGM_OK;
 }
 }  // namespace
@@ -594,9 +667,9 @@ int cgm_engine_poll(void *engine, int64_t request_id, int32_t *done) {
     Engine *E = (Engine *)engine;
     if (!E || !done) return set_error(CGM_ERR_INVALID, "cgm_engine_poll: bad arguments");
     std::lock_guard<std::mutex> lk(E->mu);
-    auto it = E->live.find(request_id);
-    if (it == E->live.end()) return set_error(CGM_ERR_NOTFOUND, "unknown request id");
-    *done = it->second->state == 2;
+    const Request *r = E->find_request(request_id);
+    if (!r) return set_error(CGM_ERR_NOTFOUND, "unknown request id");
+    *done = r->state == 2;
     return CGM_OK;
 }
 
@@ -605,21 +678,25 @@ int cgm_engine_wait(void *engine, int64_t request_id, int32_t timeout_ms, int32_
     Engine *E = (Engine *)engine;
     if (!E) return set_error(CGM_ERR_INVALID, "null engine");
     std::unique_lock<std::mutex> lk(E->mu);
-    auto it = E->live.find(request_id);
-    if (it == E->live.end()) return set_error(CGM_ERR_NOTFOUND, "unknown request id");
-    Request *r = it->second;
-    auto ready = [&] { return r->state == 2; };
+    Request *r = E->find_request(request_id);
+    if (!r) return set_error(CGM_ERR_NOTFOUND, "unknown request id");
+    // a concurrent wait on the same id may collect it first (the id then
+    // no longer matches the slot)
+    auto gone = [&] { return r->id.load(std::memory_order_acquire) != request_id; };
+    auto ready = [&] { return r->state == 2 || gone(); };
     if (timeout_ms < 0) {
         E->cv_done.wait(lk, ready);
     } else if (!E->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
         return set_error(CGM_ERR_BUSY, "request not finished");
     }
+    if (gone()) return set_error(CGM_ERR_NOTFOUND, "unknown request id");
     if (score) memcpy(score, r->score.data(), (size_t)r->n * 4);
     if (q_end) memcpy(q_end, r->qend.data(), (size_t)r->n * 4);
     if (r_end) memcpy(r_end, r->rend.data(), (size_t)r->n * 4);
     if (latency_ms) *latency_ms = (r->t_done - r->t_arrival) * 1e3;
-    E->live.erase(it);
-    E->pool.push_back(r);
+    r->id.store(0, std::memory_order_release);
+    lk.unlock();
+    E->release_request(r);
     return CGM_OK;
 }
 
@@ -670,7 +747,7 @@ double cgm_engine_now(void) { return now_s(); }
 
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                   int32_t max_pairs_per_request, int32_t max_outstanding, int32_t clients,
-                  int32_t device_inputs, uint64_t seed, double *stats /* 24 */) {
+                  int32_t device_inputs, uint64_t seed, double *stats /* 27 */) {
     Engine *E = (Engine *)engine;
     if (!E || !gen || !stats || duration_s <= 0 || max_pairs_per_request < 1 || clients < 1 ||
         clients > 64)
@@ -726,6 +803,9 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
         int64_t completed = 0, pairs_done = 0, bad = 0, submitted = 0, refused = 0;
         int error = CGM_OK;
         double t_sub_end = 0;
+        // submitter: time inside the submit call, time held back by
+        // max_outstanding, sum of the in-flight count seen at each submit
+        double t_submit = 0, t_throttled = 0, inflight_sum = 0;
     };
     std::vector<Client> cl((size_t)clients);
     std::atomic<int64_t> outstanding{0};
@@ -771,6 +851,7 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                 }
             } else if (outstanding.load() >= max_outstanding) {
                 std::this_thread::sleep_for(std::chrono::microseconds(5));
+                c.t_throttled += now_s() - t;
                 continue;
             }
             const int n = sz(rng);
@@ -780,10 +861,13 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
             const size_t k = (size_t)cursor;
             bool wfa = false;
             for (int i = 0; i < n; ++i) wfa |= kind[k + (size_t)i] == 1;
+            const double ts = now_s();
             const int rc = engine_submit(E, device_inputs != 0, q_src, qwn, r_src, rwn,
                                          ql.data() + k, rl.data() + k, qo.data() + k,
                                          ro.data() + k, n, kind.data() + k, band.data() + k,
                                          me.data() + k, arrival, &id);
+            c.t_submit += now_s() - ts;
+            c.inflight_sum += (double)outstanding.load();
             cursor += n;
             if (rate_rps > 0) t_next += gap(rng);
             if (rc == CGM_OK) {
@@ -816,9 +900,12 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
     const double t_end = now_s();
     std::vector<double> lat, lat_sw, lat_wfa;
     int64_t submitted = 0, completed = 0, refused = 0, pairs_done = 0, bad = 0;
-    double t_sub_end = t0;
+    double t_sub_end = t0, t_submit = 0, t_throttled = 0, inflight_sum = 0;
     int error = CGM_OK;
     for (Client &c : cl) {
+        t_submit += c.t_submit;
+        t_throttled += c.t_throttled;
+        inflight_sum += c.inflight_sum;
         lat.insert(lat.end(), c.lat.begin(), c.lat.end());
         lat_sw.insert(lat_sw.end(), c.lat_sw.begin(), c.lat_sw.end());
         lat_wfa.insert(lat_wfa.end(), c.lat_wfa.begin(), c.lat_wfa.end());
@@ -846,7 +933,7 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
         np[bi] = E->bins[bi].batch_pairs - bp0[bi];
     }
     const double elapsed = t_end - t0;
-    for (int i = 0; i < 24; ++i) stats[i] = 0;
+    for (int i = 0; i < 27; ++i) stats[i] = 0;
     stats[0] = (double)submitted;
     stats[1] = (double)completed;
     stats[2] = (double)refused;
@@ -871,6 +958,10 @@ int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
     stats[21] = pct(lat_wfa, 0.99);
     stats[22] = nb[0] ? (double)np[0] / (double)nb[0] : 0;
     stats[23] = nb[1] ? (double)np[1] / (double)nb[1] : 0;
+    const double phase = std::max(t_sub_end - t0, 1e-9) * clients;
+    stats[24] = t_submit / phase;
+    stats[25] = t_throttled / phase;
+    stats[26] = submitted + refused ? inflight_sum / (double)(submitted + refused) : 0;
     return CGM_OK;
 }
 

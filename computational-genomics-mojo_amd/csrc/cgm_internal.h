@@ -154,6 +154,9 @@ MixTuning mix_plan_tuning(const MixPlan *P);
 void mix_plan_set_tuning(MixPlan *P, const MixTuning &t);
 // false: every kernel of an enqueue on the caller's stream (no forked streams)
 void mix_plan_set_fork(MixPlan *P, bool fork);
+// Bounds the next enqueues' SW pairs keep to: every band <= max_band and
+// every read length <= max_len (-1: no bound known).
+void mix_plan_set_hint(MixPlan *P, int32_t max_band, int32_t max_len);
 // Measures candidate tunings on one batch (coordinate descent over the
 // fields, `reps` timed enqueues each, best kept), leaves the best in the
 // plan and returns it with its time and the starting tuning's time.

@@ -410,6 +410,10 @@ struct MixPlan {
     int64_t bytes = 0;  // device memory held by the plan
     bool had_wfa = false;  // the last enqueue launched the WFA kernels
     bool fork = true;      // WFA and long-SW kernels on streams forked from the caller's
+    // the caller's bounds on the next batch's SW bands and read lengths
+    // (-1: unknown); they let the enqueue leave out launches that cannot
+    // have work
+    int32_t hint_max_band = -1, hint_max_len = -1;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // around the SW persistent kernel
     hipEvent_t ev_wfa_start = nullptr, ev_wfa_stop = nullptr;  // around the WFA kernels
     hipEvent_t ev_wfa_mid = nullptr;  // end of the second WFA stream (no timing)
@@ -484,6 +488,10 @@ MixTuning mix_plan_tuning(const MixPlan *P) { return P->tune; }
 
 void mix_plan_set_tuning(MixPlan *P, const MixTuning &t) { P->tune = t; }
 void mix_plan_set_fork(MixPlan *P, bool fork) { P->fork = fork; }
+void mix_plan_set_hint(MixPlan *P, int32_t max_band, int32_t max_len) {
+    P->hint_max_band = max_band;
+    P->hint_max_len = max_len;
+}
 
 // waves_per_cu / lds_bytes > 0 override the default tuning's shape.
 int mix_plan_create(int64_t max_pairs, int32_t me_cap, WfaPen pen, int waves_per_cu,
@@ -683,6 +691,14 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
         CGM_HIP(hipEventRecord(P->ev_wfa_stop, aux[0]));
     }
     P->had_wfa = has_wfa;
+    // a batch of WFA pairs only has no SW work: none of the SW kernels is
+    // launched (each would still have to be placed on the CUs, behind the
+    // WFA kernels of other streams, before finding nothing to do)
+    if (kind == nullptr && kind_scalar == 1) {
+        CGM_HIP(hipEventRecord(P->ev_start, stream));
+        CGM_HIP(hipEventRecord(P->ev_stop, stream));
+        return forked ? join_streams(stream, 2, aux) : CGM_OK;
+    }
     // SW tasks: at most ceil(n / kMixGroup) + kRegBands + 1 (band-aligned
     // register tasks, packed tasks)
     const int64_t tasks_needed = (n + kMixGroup - 1) / kMixGroup + kRegBands + 1;
@@ -708,13 +724,24 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
                            P->small, tune.sw_dual, tune.sw_split);
     CGM_HIP(hipGetLastError());
     CGM_HIP(hipEventRecord(P->ev_stop, stream));
-    // uncommon SW classes: LDS lanes, then one workgroup per pair
-    if (int rc = launch_sw_lds_lane(b, band, band_scalar, sc, out, P->order, ranges + 2 * kMixSwLds,
-                                    P->cus * 2, stream))
-        return rc;
-    if (int rc = launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
-                                       ranges + 2 * kMixSwPlain, P->cus * 2, stream))
-        return rc;
+    // uncommon SW classes: LDS lanes, then one workgroup per pair; left out
+    // when the caller's bounds send every SW pair of the batch to the
+    // register lanes or the wavefront-per-pair phase (the class kernel's
+    // rules with the bounds in place of each pair's band and lengths)
+    const int oe = sc.gap_open + sc.gap_extend;
+    const int32_t hb = P->hint_max_band, hl = P->hint_max_len;
+    const bool only_reg_or_wave =
+        hb >= 0 && hl >= 0 && sc.mismatch <= 0 && sc.gap_extend <= 0 && oe <= 0 &&
+        hb <= tune.reg_max_band && hb <= kWaveMaxBand && hl < kWaveMaxLen &&
+        (int64_t)std::max(sc.match, 0) * hl < (1 << 23);
+    if (!only_reg_or_wave) {
+        if (int rc = launch_sw_lds_lane(b, band, band_scalar, sc, out, P->order,
+                                        ranges + 2 * kMixSwLds, P->cus * 2, stream))
+            return rc;
+        if (int rc = launch_sw_plain_range(b, band, band_scalar, sc, out, P->order,
+                                           ranges + 2 * kMixSwPlain, P->cus * 2, stream))
+            return rc;
+    }
     return forked ? join_streams(stream, 2, aux) : CGM_OK;
 }
 

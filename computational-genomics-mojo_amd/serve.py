@@ -12,7 +12,8 @@ STAT_NAMES = ["submitted", "completed", "refused", "pairs_completed", "elapsed_s
               "requests_per_s", "p50_ms", "p90_ms", "p99_ms", "max_ms", "mean_ms", "batches",
               "mean_batch_pairs", "bad_results", "submit_phase_s", "sw_only_requests",
               "sw_only_p50_ms", "sw_only_p99_ms", "with_wfa_requests", "with_wfa_p50_ms",
-              "with_wfa_p99_ms", "mean_sw_batch_pairs", "mean_wfa_batch_pairs"]
+              "with_wfa_p99_ms", "mean_sw_batch_pairs", "mean_wfa_batch_pairs",
+              "submit_busy_frac", "submit_throttled_frac", "mean_in_flight"]
 
 
 class ServingEngine:
@@ -63,6 +64,12 @@ class ServingEngine:
         idx = np.asarray(idx, np.int64)
         return self.submit(w.q_packed, w.r_packed, w.q_len[idx], w.r_len[idx], w.q_off[idx],
                            w.r_off[idx], w.kind[idx], w.band[idx], w.max_error[idx], arrival_s)
+
+    def poll(self, request_id: int) -> bool:
+        """Whether a submitted, not yet collected request has finished."""
+        done = ctypes.c_int32()
+        _lib.call("cgm_engine_poll", self.handle, request_id, ctypes.byref(done))
+        return bool(done.value)
 
     def wait(self, request_id: int, n_pairs: int, timeout_ms: int = -1):
         """(score, query_end, ref_end, latency_ms) of a finished request."""

@@ -480,12 +480,14 @@ double cgm_engine_now(void);
  * collecting its results in submission order.  device_inputs = 1: the
  * batch's packed words are uploaded once and requests go through
  * cgm_engine_submit_device (sequences resident in HBM); 0: through
- * cgm_engine_submit (host words copied per request).  stats[24]: submitted,
+ * cgm_engine_submit (host words copied per request).  stats[27]: submitted,
  * completed, refused, pairs completed, elapsed s, pairs/s, requests/s, p50,
  * p90, p99, max, mean latency (ms), batches, mean pairs per batch, bad
  * results, submit phase s, then SW-only requests: count, p50, p99 (ms);
  * requests with a WFA pair: count, p50, p99 (ms); mean pairs per SW batch,
- * per WFA batch. */
+ * per WFA batch; the submitting threads' fraction of the submit phase spent
+ * inside the submit call and held back by max_outstanding; mean requests in
+ * flight at a submit. */
 int cgm_serve_run(void *engine, void *gen, double rate_rps, double duration_s,
                   int32_t max_pairs_per_request, int32_t max_outstanding, int32_t clients,
                   int32_t device_inputs, uint64_t seed, double *stats);

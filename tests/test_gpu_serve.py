@@ -255,3 +255,26 @@ def test_engine_device_resident_requests(cgm, dev, orc, serving_pairs):
     assert st["completed"] > 0 and st["bad_results"] == 0
     g.close()
     eng.close()
+
+
+def test_engine_request_ids_after_collection(cgm, serving_pairs):
+    """A collected request's id is unknown to poll and wait; its slot is
+    reused under a new id, and ids of live requests never repeat."""
+    w = serving_pairs
+    eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=2, max_batch_pairs=2048,
+                            max_batch_words=1 << 17, max_error_cap=int(w.max_error.max()))
+    sw = np.nonzero(w.kind == 0)[0]
+    first = eng.submit_pairs(w, sw[:4])
+    s1, _, _, _ = eng.wait(first, 4)
+    print("[case] engine ids: collected id is unknown", flush=True)
+    for call in (lambda: eng.poll(first), lambda: eng.wait(first, 4)):
+        with pytest.raises(cgm._lib.CgmError) as ei:
+            call()
+        assert ei.value.code == cgm._lib.CGM_ERR_NOTFOUND
+    ids = [eng.submit_pairs(w, sw[4 * k:4 * k + 4]) for k in range(1, 40)]
+    assert first not in ids and len(set(ids)) == len(ids) and min(ids) > 0
+    got = [eng.wait(i, 4)[0] for i in ids]
+    s2, _, _, _ = eng.wait(eng.submit_pairs(w, sw[:4]), 4)
+    assert np.array_equal(s1, s2)
+    assert all(g.shape == (4,) for g in got)
+    eng.close()
