@@ -11,8 +11,9 @@ packed sequences already resident in HBM.  Prints ONE JSON line:
   roofline    integer-VALU roofline of the dominant kernel: achieved int ops
               (11 per cell) / that kernel's mean launch time (HIP events on
               the stream it runs on), against 256 CU x 64 lanes x 2.4 GHz
-              int32 ops, or twice that when the register lanes compute in
-              packed int16 (tuning sw_dual; `dtype` says which)
+              int32 ops, or twice that when the register lanes compute two
+              cells per packed 16-bit instruction (tuning sw_dual; `dtype`
+              says which)
   cpu_baseline  the CPU oracle (C restatement of align_cpu.mojo) on this
               host, all pairs, `cores` threads
 Extra objects report the plain one-workgroup-per-pair kernel on the same
@@ -664,9 +665,11 @@ def main():
                "bit_exact_vs_gpu": bool(match)}
         log(f"[bench] cpu: {cpu}")
 
-    # the register lanes of the persistent kernel compute in packed int16
-    # (two cells per instruction) when the chosen tuning enables them; every
-    # pair of this workload fits int16 (match * 300 + match <= 32767)
+    # the register lanes of the persistent kernel compute two cells per
+    # packed 16-bit instruction when the chosen tuning enables them, in IEEE
+    # half precision for every pair of this workload (all values are even
+    # integers within +-4096: match * 300 + match <= 2048 * 2), which the
+    # packed units run at the int16 rate
     dual = bool(autotune and autotune["tuning"].get("sw_dual", 0))
     peak = VALU_PEAK_TOPS * (2 if dual else 1)
     line = {
@@ -680,14 +683,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int16" if dual else "int32",
+        "dtype": "f16 (exact integers)" if dual else "int32",
         "data": "synthetic (seeded splitmix64 generator, inputs resident in HBM)",
         "config": {"workload": "batched banded SW: 200,000 pairs, targets 100-300 bp, "
                                "divergence 1-5%, 0.1% N, band 16-64 per pair, 2/-4/-4/-2",
                    "pairs": w.n_pairs, "valid_cells": total_cells, "band_area_cells": band_area,
                    "layout": args.layout, "sw_path": args.sw_path, "autotune": autotune},
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": peak,
-                     "unit": "TOP/s (packed int16)" if dual else "TOP/s (int32)",
+                     "unit": "TOP/s (packed 16-bit)" if dual else "TOP/s (int32)",
                      "frac": achieved_tops / peak,
                      "frac_of_int32_peak": achieved_tops / VALU_PEAK_TOPS,
                      "traffic": traffic_from_profile() if plan is not None else None,

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["cgm_runtime.hip", "cgm_sw.hip", "cgm_sw_lane.hip", "cgm_wfa.hip", "cgm_sort.hip",
-               "cgm_mixed.hip", "cgm_engine.hip", "cgm_encode.hip",
+               "cgm_mixed.hip", "cgm_mixed_wave.hip", "cgm_engine.hip", "cgm_encode.hip",
                "cgm_cigar.hip", "cgm_nw.hip", "cgm_sw_start.hip",
                "cgm_noalloc.hip"]
 CXX_SOURCES = ["cgm_gen.cpp"]

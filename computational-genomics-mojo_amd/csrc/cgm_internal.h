@@ -142,7 +142,7 @@ struct MixTuning {
     int32_t wave_max_width = 1 << 30;  // kept for the ABI; as tile_steps
     int32_t reg_max_band = 64;         // SW bands up to this: register lanes, wider: LDS lanes
     int32_t lane_max_len = 1 << 30;    // SW pairs longer than this: one workgroup per pair
-    int32_t sw_dual = 1;               // register SW lanes: dual-row int16 where exact (1) or int32 (0)
+    int32_t sw_dual = 1;               // register SW lanes: dual-row f16 / int16 where exact (1) or int32 (0)
     int32_t sw_split = 55;             // SW-only: percent of the 64-pair units in the long queue
 };
 constexpr int kMixTuneN = 8;
@@ -157,6 +157,14 @@ void mix_plan_set_fork(MixPlan *P, bool fork);
 // Bounds the next enqueues' SW pairs keep to: every band <= max_band and
 // every read length <= max_len (-1: no bound known).
 void mix_plan_set_hint(MixPlan *P, int32_t max_band, int32_t max_len);
+// The SW persistent kernel's instance with the wavefront-per-pair phase
+// (cgm_mixed_wave.hip), `grid` workgroups on `stream`; tasks: the plan's
+// task records.
+int launch_mix_persistent_wave(unsigned grid, hipStream_t stream, const Batch &b,
+                               const int32_t *band, int32_t band_scalar, SwScoring sc, SwOut out,
+                               const int32_t *order, const void *tasks, const int32_t *task_order,
+                               const int32_t *n_tasks, int32_t *small, int32_t sw_dual,
+                               int32_t sw_split);
 // Measures candidate tunings on one batch (coordinate descent over the
 // fields, `reps` timed enqueues each, best kept), leaves the best in the
 // plan and returns it with its time and the starting tuning's time.

@@ -330,20 +330,6 @@ __device__ __forceinline__ bool dual_ok(const SwScoring &sc, int qlen, int rlen)
 }
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
-__device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
-__device__ __forceinline__ s16x2 pk_max(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
-// Running row maximum, one v_pk_max_i16 per cell kept in sequence (for the
-// same reason as max3_chain) by an empty asm on its result.  Unlike an asm
-// instruction this leaves the hazard recognizer free to schedule around it:
-// on gfx950 a packed instruction that reads the result of the packed
-// instruction just before it needs a wait state, and an asm instruction is
-// always assumed to.
-__device__ __forceinline__ s16x2 pk_max_seq(s16x2 a, s16x2 b) {
-    uint32_t d = as_u32(__builtin_elementwise_max(a, b));
-    asm("" : "+v"(d));
-    return as_s16x2(d);
-}
 // Sign-extended bit (30 - 2 * (u % 16)) of a match-mask half (0 / -1).  The
 // bit offset is taken from `base30` (30, unknown to the compiler), so the
 // extraction stays one v_bfe_i32 instead of being turned into a compare and
@@ -355,26 +341,95 @@ __device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t a, uint32_t 
     return (a & m) | (b & ~m);
 }
 
-template <int B>
+// Dual-row cell arithmetic.  Two 16-bit encodings of the same integer
+// recurrence: int16 (v_pk_add_u16 / v_pk_max_i16), and IEEE half precision
+// (v_pk_add_f16 / v_pk_maximum3_f16, three-input maximum), which is exact
+// while every value is a multiple of t in [-2048 t, 2048 t], t the largest
+// power of two (at most 8) dividing all four scores: such values have at
+// most 11 significant bits, so every sum and maximum is the integer one.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+template <bool F16>
+struct DualArith;
+template <>
+struct DualArith<false> {
+    using V = s16x2;
+    static constexpr uint32_t kNegE = 0xC000u;  // -16384
+    __device__ static V add(V a, V b) { return a + b; }
+    __device__ static V mx(V a, V b) { return __builtin_elementwise_max(a, b); }
+    __device__ static uint32_t enc(int x) { return (uint32_t)x & 0xffffu; }
+    __device__ static int dec(uint32_t bits) { return (int)(short)(bits & 0xffffu); }
+};
+template <>
+struct DualArith<true> {
+    using V = f16x2;
+    static constexpr uint32_t kNegE = 0xF400u;  // -16384
+    __device__ static V add(V a, V b) { return a + b; }
+    __device__ static V mx(V a, V b) { return __builtin_elementwise_maximum(a, b); }
+    __device__ static uint32_t enc(int x) {
+        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x);
+    }
+    __device__ static int dec(uint32_t bits) {
+        return (int)__builtin_bit_cast(_Float16, (uint16_t)(bits & 0xffffu));
+    }
+};
+template <class V>
+__device__ __forceinline__ V dual_bits(uint32_t v) { return __builtin_bit_cast(V, v); }
+template <class V>
+__device__ __forceinline__ uint32_t dual_u32(V v) { return __builtin_bit_cast(uint32_t, v); }
+// Running row maximum kept in sequence (for the same reason as max3_chain)
+// by an empty asm on its result.  Unlike an asm instruction this leaves the
+// hazard recognizer free to schedule around it: on gfx950 a packed
+// instruction that reads the result of the packed instruction just before it
+// needs a wait state, and an asm instruction is always assumed to.
+template <class V>
+__device__ __forceinline__ V dual_seq(V v) {
+    uint32_t d = dual_u32(v);
+    asm("" : "+v"(d));
+    return dual_bits<V>(d);
+}
+// t of the half-precision encoding: largest power of two <= 8 dividing all scores
+__device__ __forceinline__ int64_t f16_unit(const SwScoring &sc) {
+    const uint64_t m = (uint64_t)llabs(sc.match) | (uint64_t)llabs(sc.mismatch) |
+                       (uint64_t)llabs(sc.gap_open) | (uint64_t)llabs(sc.gap_extend);
+    const uint64_t low = m ? (m & (0ull - m)) : 8ull;
+    return (int64_t)(low < 8 ? low : 8);
+}
+// every constant and every value bound of the recurrence within 2048 t:
+// SUBM = match - oe, SUBX = mismatch - oe, oe, oe + gap_extend, mismatch
+__device__ __forceinline__ bool f16_scoring_ok(const SwScoring &sc) {
+    const int64_t lim = 2048 * f16_unit(sc);
+    const int64_t oe = (int64_t)sc.gap_open + sc.gap_extend;
+    return sc.match <= lim && sc.match - oe <= lim && sc.mismatch - oe <= lim &&
+           sc.mismatch - oe >= -lim && sc.mismatch >= -lim && oe >= -lim &&
+           oe + sc.gap_extend >= -lim;
+}
+// H_max + match within 2048 t (H_max = match * min(qlen, rlen))
+__device__ __forceinline__ bool f16_ok(const SwScoring &sc, int qlen, int rlen) {
+    return (int64_t)max(sc.match, 0) * min(qlen, rlen) + sc.match <= 2048 * f16_unit(sc);
+}
+
+template <int B, bool F16>
 __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, const uint64_t *r,
                                                    int rlen, int rows, const SwScoring &sc) {
+    using T = DualArith<F16>;
+    using V = typename T::V;
     constexpr int S = B + 1;
     constexpr int NCH = (S + 31) / 32;
     const int oe = sc.gap_open + sc.gap_extend;
-    const uint32_t OE = ((uint32_t)oe & 0xffffu) * 0x10001u;
-    const uint32_t EXT = ((uint32_t)sc.gap_extend & 0xffffu) * 0x10001u;
-    const uint32_t SUBM = ((uint32_t)(sc.match - oe) & 0xffffu) * 0x10001u;
-    const uint32_t SUBX = ((uint32_t)(sc.mismatch - oe) & 0xffffu) * 0x10001u;
-    constexpr uint32_t kNegE16 = 0xC000u;  // -16384
+    const uint32_t OE = T::enc(oe) * 0x10001u;
+    const uint32_t EXT = T::enc(sc.gap_extend) * 0x10001u;
+    const uint32_t SUBM = T::enc(sc.match - oe) * 0x10001u;
+    const uint32_t SUBX = T::enc(sc.mismatch - oe) * 0x10001u;
+    constexpr uint32_t kNegE16 = T::kNegE;  // -16384
     const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
     const int rmin = wave_min_ref(qlen, rlen);
     int base30;
     asm volatile("s_mov_b32 %0, 30" : "=s"(base30));
-    s16x2 P[S], SV[S];
+    V P[S], SV[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) {
-        P[u] = as_s16x2(OE);
-        SV[u] = as_s16x2(OE);
+        P[u] = dual_bits<V>(OE);
+        SV[u] = dual_bits<V>(OE);
     }
     int best_lo = 0, row_lo = 0, best_hi = 0, row_hi = 0;
     uint64_t qwin = 0;
@@ -400,9 +455,9 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
         // value for the left half; cell B of row t - 1 for the right half
         // left neighbours of the two first cells: H = 0 and the E boundary
         // value for the left half; cell B of row t - 1 for the right half
-        s16x2 Gl = as_s16x2((as_u32(P[S - 1]) << 16) | (OE & 0xffffu));
-        s16x2 El = as_s16x2((el_carry << 16) | kNegE16);
-        s16x2 rowmax = as_s16x2(0u);
+        V Gl = dual_bits<V>((dual_u32(P[S - 1]) << 16) | (OE & 0xffffu));
+        V El = dual_bits<V>((el_carry << 16) | kNegE16);
+        V rowmax = dual_bits<V>(0u), hprev = dual_bits<V>(0u);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const uint64_t ml =
@@ -421,29 +476,50 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
                     // upper neighbour: register u + 1 of the previous step;
                     // for the last cell, the right half's first cell of this
                     // step (low) and H = 0 past the frame (high)
-                    const s16x2 Gu = (u + 1 < S) ? P[u + 1]
-                                                 : as_s16x2((OE & 0xffff0000u) | p0_hi);
-                    const s16x2 dterm = P[u] + as_s16x2(sel);
-                    const s16x2 e = pk_max(El + as_s16x2(EXT), Gu);
-                    s16x2 h = pk_max(pk_max(pk_max(dterm, as_s16x2(0u)), e), Gl);
-                    if (u == S - 1) h = as_s16x2(as_u32(h) & 0xffffu);
-                    rowmax = pk_max_seq(rowmax, h);
-                    const s16x2 pn = h + as_s16x2(OE);
+                    const V Gu = (u + 1 < S) ? P[u + 1]
+                                             : dual_bits<V>((OE & 0xffff0000u) | p0_hi);
+                    const V zero = dual_bits<V>(0u);
+                    const V dterm = T::add(P[u], dual_bits<V>(sel));
+                    const V ea = T::add(El, dual_bits<V>(EXT));
+                    const V e = T::mx(ea, Gu);
+                    V h;
+                    if constexpr (F16) {
+                        // max(dterm, 0, e, Gl) as two three-input maxima,
+                        // the first off both dependency chains
+                        const V x = T::mx(T::mx(dterm, zero), Gu);
+                        h = T::mx(T::mx(x, ea), Gl);
+                    } else {
+                        h = T::mx(T::mx(T::mx(dterm, zero), e), Gl);
+                    }
+                    if (u == S - 1) h = dual_bits<V>(dual_u32(h) & 0xffffu);
+                    if constexpr (F16) {
+                        // two cells per three-input maximum
+                        if (u & 1)
+                            rowmax = dual_seq(T::mx(T::mx(rowmax, hprev), h));
+                        else if (u == S - 1)
+                            rowmax = dual_seq(T::mx(rowmax, h));
+                        else
+                            hprev = h;
+                    } else {
+                        rowmax = dual_seq(T::mx(rowmax, h));
+                    }
+                    const V pn = T::add(h, dual_bits<V>(OE));
                     P[u] = pn;
                     Gl = pn;
                     El = e;
-                    if (u == 0) p0_hi = as_u32(pn) >> 16;
+                    if (u == 0) p0_hi = dual_u32(pn) >> 16;
                 }
             }
         }
-        el_carry = as_u32(El) & 0xffffu;
-        const int rm_lo = (int)(short)(as_u32(rowmax) & 0xffffu);
-        const int rm_hi = (int)(short)(as_u32(rowmax) >> 16);
+        el_carry = dual_u32(El) & 0xffffu;
+        const int rm_lo = T::dec(dual_u32(rowmax));
+        const int rm_hi = T::dec(dual_u32(rowmax) >> 16);
         const bool up_lo = rm_lo > best_lo, up_hi = rm_hi > best_hi;
         if (__ballot(up_lo || up_hi)) {
             const uint32_t m = (up_lo ? 0xffffu : 0u) | (up_hi ? 0xffff0000u : 0u);
 #pragma unroll
-            for (int u = 0; u < S; ++u) SV[u] = as_s16x2(bit_select(m, as_u32(P[u]), as_u32(SV[u])));
+            for (int u = 0; u < S; ++u)
+                SV[u] = dual_bits<V>(bit_select(m, dual_u32(P[u]), dual_u32(SV[u])));
             if (up_lo) {
                 best_lo = rm_lo;
                 row_lo = t;
@@ -458,9 +534,9 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
     int ulo = 0, uhi = 0;
 #pragma unroll
     for (int u = S - 1; u >= 0; --u) {
-        const uint32_t v = as_u32(SV[u]);
-        if ((int)(short)(v & 0xffffu) - oe == best_lo) ulo = u;
-        if ((int)(short)(v >> 16) - oe == best_hi) uhi = u;
+        const uint32_t v = dual_u32(SV[u]);
+        if (T::dec(v) - oe == best_lo) ulo = u;
+        if (T::dec(v >> 16) - oe == best_hi) uhi = u;
     }
     LaneResult lo = {best_lo, row_lo, row_lo - B + ulo};
     LaneResult hi = {best_hi, row_hi, row_hi + 1 + uhi};  // ri = row - B + S + u
@@ -471,16 +547,16 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
     return lo;
 }
 
-template <int LO, int HI>
+template <int LO, int HI, bool F16>
 __device__ __forceinline__ LaneResult sw_dual_dispatch(int band, const uint64_t *q, int qlen,
                                                        const uint64_t *r, int rlen, int rows,
                                                        const SwScoring &sc) {
     if constexpr (LO == HI) {
-        return sw_dual_lane<LO>(q, qlen, r, rlen, rows, sc);
+        return sw_dual_lane<LO, F16>(q, qlen, r, rlen, rows, sc);
     } else {
         constexpr int MID = (LO + HI) / 2;
-        if (band <= MID) return sw_dual_dispatch<LO, MID>(band, q, qlen, r, rlen, rows, sc);
-        return sw_dual_dispatch<MID + 1, HI>(band, q, qlen, r, rlen, rows, sc);
+        if (band <= MID) return sw_dual_dispatch<LO, MID, F16>(band, q, qlen, r, rlen, rows, sc);
+        return sw_dual_dispatch<MID + 1, HI, F16>(band, q, qlen, r, rlen, rows, sc);
     }
 }
 
@@ -501,8 +577,9 @@ __device__ __forceinline__ LaneResult sw_reg_dispatch(int band, const uint64_t *
 // register layout: lanes are served once per distinct band among them
 // (pairs arrive sorted by band, so normally once).  PACKED selects the
 // packed-int16 instantiation for band 128, otherwise bands LO..HI.  DUAL
-// compiles the dual-row int16 lanes in as well; they serve a pass when
-// `dual` is set and every lane of the pass fits them (dual_ok).
+// compiles the dual-row lanes in as well (half-precision and int16
+// encodings); they serve a pass when `dual` is set and every lane of the
+// pass fits one of them (f16_ok, else dual_ok).
 template <int LO, int HI, bool PACKED, bool DUAL = false>
 __device__ __forceinline__ void sw_reg_group(const Batch &b, const int32_t *band_arr,
                                              int32_t band_scalar, const SwScoring &sc,
@@ -529,11 +606,18 @@ __device__ __forceinline__ void sw_reg_group(const Batch &b, const int32_t *band
         if constexpr (PACKED) {
             res = sw_reg_lane_packed<HI>(q, mine ? qlen : 0, r, mine ? rlen : 0, rows, sc);
         } else if constexpr (DUAL) {
-            const bool use_dual =
-                dual && dual_scoring_ok(sc) && __all(!mine || dual_ok(sc, qlen, rlen));
-            if (use_dual)
-                res = sw_dual_dispatch<LO, HI>(bsel, q, mine ? qlen : 0, r, mine ? rlen : 0, rows,
-                                               sc);
+            // half precision where every lane's values stay exact in it,
+            // else int16 where they fit, else int32
+            const bool use_f16 =
+                dual && f16_scoring_ok(sc) && __all(!mine || f16_ok(sc, qlen, rlen));
+            const bool use_i16 = !use_f16 && dual && dual_scoring_ok(sc) &&
+                                 __all(!mine || dual_ok(sc, qlen, rlen));
+            if (use_f16)
+                res = sw_dual_dispatch<LO, HI, true>(bsel, q, mine ? qlen : 0, r,
+                                                     mine ? rlen : 0, rows, sc);
+            else if (use_i16)
+                res = sw_dual_dispatch<LO, HI, false>(bsel, q, mine ? qlen : 0, r,
+                                                      mine ? rlen : 0, rows, sc);
             else
                 res = sw_reg_dispatch<LO, HI>(bsel, q, mine ? qlen : 0, r, mine ? rlen : 0, rows,
                                               sc);

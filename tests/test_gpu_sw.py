@@ -159,8 +159,9 @@ def plan_sw(cgm, dev, up, plan, scoring):
 
 @pytest.mark.parametrize("dual", [1, 0])
 def test_sw_register_lane_forms_equal_oracle(cgm, dev, orc, gen_mod, dual):
-    """The int32 and the dual-row int16 register lanes (tuning sw_dual) on
-    random pairs over several scorings and every band 0-64."""
+    """The int32 and the dual-row register lanes (tuning sw_dual: half
+    precision for these lengths and scorings) on random pairs over several
+    scorings and every band 0-64."""
     rng = np.random.default_rng(31)
     for k, sc in enumerate((CONFIG_SW, DEFAULT_SW, dict(match=1, mismatch=-1, gap_open=-1,
                                                         gap_extend=-1),
@@ -214,6 +215,52 @@ def test_sw_dual_lanes_int16_boundary(cgm, dev, orc):
     assert_same(got, sw_oracle(orc, w, CONFIG_SW), "int16 boundary")
     scores, q_end = got[0], got[1]
     assert scores[0] == 32764 and q_end[0] == 16381 and scores[1] == 32766
+    plan.close()
+
+
+@pytest.mark.parametrize("scoring,t", [
+    (dict(match=2, mismatch=-4, gap_open=-4, gap_extend=-2), 2),
+    (dict(match=3, mismatch=-2, gap_open=-5, gap_extend=-1), 1),
+    (dict(match=8, mismatch=-16, gap_open=-24, gap_extend=-8), 8),
+])
+def test_sw_dual_lanes_f16_boundary(cgm, dev, orc, scoring, t):
+    """The half-precision dual-row lanes hold exact integers while
+    match * min(qlen, rlen) + match <= 2048 t (t: the largest power of two
+    <= 8 dividing every score); one base more runs the int16 lanes.  Pairs at
+    both sides of that length, identical and mutated, alone and sharing a
+    wavefront (which then runs int16), and random pairs of 100-700 bp: every
+    result equals the oracle, identical pairs score match * length."""
+    rng = np.random.default_rng(9 + t)
+    m = scoring["match"]
+    L = (2048 * t - m) // m  # longest length inside the half-precision range
+    a = "".join(rng.choice(list("ACGT"), L + 1))
+
+    def mutate(x, k):
+        y = list(x)
+        for p in rng.choice(len(y), k, replace=False):
+            y[p] = "ACGT"[("ACGT".index(y[p]) + 1 + int(rng.integers(3))) % 4]
+        if len(y) > 10:
+            del y[len(y) // 3]
+            y.insert(2 * len(y) // 3, "A")
+        return "".join(y)
+
+    pairs = [(a[:L], a[:L]), (a, a), (a[:L], mutate(a[:L], 12)), (mutate(a, 12), a),
+             (a[:L], a[:L]), (a, a)]
+    bands = [12, 12, 40, 40, 64, 33]
+    for _ in range(120):
+        n = int(rng.integers(100, 700))
+        x = "".join(rng.choice(list("ACGT"), n))
+        pairs.append((mutate(x, int(rng.integers(0, n // 20 + 1))), x))
+        bands.append(int(rng.integers(0, 65)))
+    w = workload_from_strings(pairs, band=bands)
+    up = Uploaded(dev, w)
+    plan = cgm.MixedPlan(w.n_pairs, 0)
+    plan.set_tuning(sw_dual=1)
+    print(f"[case] dual f16 boundary t={t} L={L}", flush=True)
+    got = plan_sw(cgm, dev, up, plan, scoring)
+    assert_same(got, sw_oracle(orc, w, scoring), f"f16 boundary t={t}")
+    assert got[0][0] == m * L and got[1][0] == L - 1 and got[0][1] == m * (L + 1)
+    assert m * L + m <= 2048 * t < m * (L + 1) + m
     plan.close()
 
 
