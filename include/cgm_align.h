@@ -339,8 +339,11 @@ int cgm_plan_wfa_kernel_ms(void *plan, float *ms);
  *   LANE_MAX_LEN    switch-over: SW pairs longer than this run one workgroup
  *                   per pair instead of one lane per pair
  *   SW_DUAL         1: register SW lanes hold the band frame's two halves,
- *                   one row apart, in the two 16-bit halves of each register
- *                   (packed int16 arithmetic) wherever every value fits int16
+ *                   one row apart, in the two 16-bit halves of each register:
+ *                   IEEE half precision wherever every value is an exact
+ *                   multiple of t (match * min(qlen, rlen) + match <=
+ *                   2048 t, t the largest power of two <= 8 dividing every
+ *                   score), else packed int16 wherever every value fits
  *                   (match * min(qlen, rlen) + match <= 32767, penalties
  *                   >= -8191); 0: int32 lanes only
  *   SW_SPLIT        SW-only batches: the longest-first list of 64-pair units
@@ -454,10 +457,13 @@ int cgm_engine_submit_device(void *engine, const uint64_t *q_packed, int64_t q_w
                              const int32_t *r_len, const int32_t *q_off, const int32_t *r_off,
                              int32_t n_pairs, const int32_t *kind, const int32_t *band,
                              const int32_t *max_error, double arrival_s, int64_t *request_id);
-/* done = 1 once the request's results are ready. */
+/* done = 1 once the request's results are ready; CGM_ERR_NOTFOUND for an
+ * id never issued or already collected. */
 int cgm_engine_poll(void *engine, int64_t request_id, int32_t *done);
 /* Blocks (timeout_ms < 0: forever) until the request is done, copies its
- * n_pairs results (any pointer may be NULL) and releases the request. */
+ * n_pairs results (any pointer may be NULL) and releases the request: its
+ * id is unknown from then on (of two waits on one id, one gets the results
+ * and the other CGM_ERR_NOTFOUND). */
 int cgm_engine_wait(void *engine, int64_t request_id, int32_t timeout_ms, int32_t *score,
                     int32_t *q_end, int32_t *r_end, double *latency_ms);
 /* Tuning of one bin's batches (bin 0: SW pairs, 1: WFA pairs) as a

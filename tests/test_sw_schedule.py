@@ -9,6 +9,8 @@ restatement must equal smith_waterman_banded (align_cpu.mojo:229-309).
 """
 import random
 
+import numpy as np
+
 from conftest import load_oracle
 
 O = load_oracle()
@@ -68,30 +70,51 @@ def _w16(x):
     return x - 0x10000 if x & 0x8000 else x
 
 
-def sw_dual_schedule(q, r, band, match, mismatch, gap_open, gap_extend, extra_rows=0):
-    """The dual-row int16 lane (csrc/cgm_sw_dev.h sw_dual_lane): halves
-    [0, S) and [S, 2S) of the band frame, S = band + 1, the right half one
-    row behind, every value in 16-bit lanes, cell 2S - 1 (past the frame)
-    forced to H = 0, each half's first best row kept and searched for its
-    first best cell.  `extra_rows` adds rows past the query, as a lane whose
-    wavefront runs longer queries does."""
+def _f16(x):
+    """IEEE half precision rounding (round to nearest even)."""
+    return float(np.float16(x))
+
+
+def sw_dual_schedule(q, r, band, match, mismatch, gap_open, gap_extend, extra_rows=0,
+                     enc="int16"):
+    """The dual-row lane (csrc/cgm_sw_dev.h sw_dual_lane): halves [0, S) and
+    [S, 2S) of the band frame, S = band + 1, the right half one row behind,
+    every value in 16-bit lanes, cell 2S - 1 (past the frame) forced to
+    H = 0, each half's first best row kept and searched for its first best
+    cell.  enc "int16": two's-complement halves (v_pk_add_u16 wraps);
+    "f16": IEEE half precision, every constant and every sum rounded to
+    half (v_pk_add_f16).  `extra_rows` adds rows past the query, as a lane
+    whose wavefront runs longer queries does."""
     ql, rl = len(q), len(r)
     B = min(band, max(ql, rl))
     S = B + 1
+    if enc == "int16":
+        cst = _w16
+
+        def add(a, b):
+            return _w16(a + b)
+    else:
+        cst = _f16
+
+        def add(a, b):
+            # the sum of two halves is exact in double; one rounding to half
+            return _f16(a + b)
     oe = gap_open + gap_extend
-    NEG16 = -16384
-    P = [[oe, oe] for _ in range(S)]  # [low half, high half]
-    SV = [[oe, oe] for _ in range(S)]
+    OE, EXT = cst(oe), cst(gap_extend)
+    SUBM, SUBX = cst(match - oe), cst(mismatch - oe)
+    NEG16 = cst(-16384)
+    P = [[OE, OE] for _ in range(S)]  # [low half, high half]
+    SV = [[OE, OE] for _ in range(S)]
     best = [0, 0]
     row = [0, 0]
     el_carry = NEG16
-    p0_hi = oe
+    p0_hi = OE
     rows = ql + extra_rows
     for t in range(rows + 1):
         rs = (t - B, t)
         ok = (t < ql, 1 <= t and t - 1 < ql)
         qb = (q[t] if t < ql else None, q[t - 1] if 1 <= t <= ql else None)
-        Gl = [oe, P[S - 1][0]]
+        Gl = [OE, P[S - 1][0]]
         El = [NEG16, el_carry]
         rowmax = [0, 0]
         for u in range(S):
@@ -102,14 +125,14 @@ def sw_dual_schedule(q, r, band, match, mismatch, gap_open, gap_extend, extra_ro
                 if u + 1 < S:
                     Gu = P[u + 1][k]
                 else:
-                    Gu = p0_hi if k == 0 else oe
-                dterm = _w16(P[u][k] + _w16((match if eq else mismatch) - oe))
-                e = max(_w16(El[k] + gap_extend), Gu)
+                    Gu = p0_hi if k == 0 else OE
+                dterm = add(P[u][k], SUBM if eq else SUBX)
+                e = max(add(El[k], EXT), Gu)
                 h = max(max(dterm, 0), max(e, Gl[k]))
                 if u == S - 1 and k == 1:
                     h = 0
                 rowmax[k] = max(rowmax[k], h)
-                newP[k] = _w16(h + oe)
+                newP[k] = add(h, OE)
                 Gl[k] = newP[k]
                 El[k] = e
             P[u] = newP
@@ -118,7 +141,7 @@ def sw_dual_schedule(q, r, band, match, mismatch, gap_open, gap_extend, extra_ro
         el_carry = El[0]
         for k in (0, 1):
             if rowmax[k] > best[k]:
-                best[k] = rowmax[k]
+                best[k] = int(rowmax[k])
                 row[k] = t - k
                 for u in range(S):
                     SV[u][k] = P[u][k]
@@ -127,7 +150,7 @@ def sw_dual_schedule(q, r, band, match, mismatch, gap_open, gap_extend, extra_ro
         if best[k] == 0:
             cand.append((0, 0, 0))
             continue
-        u = next(u for u in range(S) if SV[u][k] - oe == best[k])
+        u = next(u for u in range(S) if int(SV[u][k]) - oe == best[k])
         cand.append((best[k], row[k], row[k] - B + k * S + u))
     lo, hi = cand
     if hi[0] > lo[0] or (hi[0] == lo[0] and hi[0] > 0 and hi[1] < lo[1]):
@@ -148,6 +171,59 @@ def test_dual_schedule_equals_oracle():
             r = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 40)))
         band = rng.randint(0, 70)
         sc = rng.choice(scorings)
-        got = sw_dual_schedule(q, r, band, *sc, extra_rows=rng.choice([0, 0, 1, 7]))
+        extra = rng.choice([0, 0, 1, 7])
         want = O.sw_banded(q, r, *sc, band=band)
-        assert got == want, (q, r, band, sc, got, want)
+        for enc in ("int16", "f16"):
+            got = sw_dual_schedule(q, r, band, *sc, extra_rows=extra, enc=enc)
+            assert got == want, (q, r, band, sc, enc, got, want)
+
+
+def _f16_unit(match, mismatch, gap_open, gap_extend):
+    """Largest power of two <= 8 dividing every score (sw_dual_lane's t)."""
+    m = abs(match) | abs(mismatch) | abs(gap_open) | abs(gap_extend)
+    return min(m & -m, 8) if m else 8
+
+
+def _mutate(rng, x, subs, indels):
+    y = list(x)
+    for _ in range(subs):
+        p = rng.randrange(len(y))
+        y[p] = rng.choice("ACGT".replace(y[p], ""))
+    for _ in range(indels):
+        p = rng.randrange(len(y))
+        if rng.random() < 0.5:
+            del y[p]
+        else:
+            y.insert(p, rng.choice("ACGT"))
+    return "".join(y)
+
+
+def test_dual_f16_exact_up_to_the_range_limit():
+    """Half-precision lanes at the longest pairs they admit: match *
+    min(qlen, rlen) + match = 2048 t (t = 1 and t = 2), identical and
+    mutated, equal the oracle, so every value of the recurrence is a
+    representable multiple of t there."""
+    rng = random.Random(11)
+    for sc in ((3, -2, -5, -1), (2, -4, -4, -2), (1, -1, -1, -1)):
+        t = _f16_unit(*sc)
+        L = (2048 * t - sc[0]) // sc[0]
+        assert sc[0] * L + sc[0] <= 2048 * t < sc[0] * (L + 1) + sc[0]
+        a = "".join(rng.choice("ACGT") for _ in range(L))
+        for q, r, band in ((a, a, 2), (_mutate(rng, a, 20, 4), a, 5), (a, _mutate(rng, a, 9, 2), 3)):
+            want = O.sw_banded(q, r, *sc, band=band)
+            assert sw_dual_schedule(q, r, band, *sc, enc="f16") == want, (sc, band)
+        assert O.sw_banded(a, a, *sc, band=2)[0] == sc[0] * L
+
+
+def test_dual_f16_is_wrong_past_the_range_limit():
+    """The bound is needed: with t = 1 an identical pair one base past the
+    limit reaches an odd score above 2048, which half precision cannot hold,
+    and the half-precision schedule then differs from the oracle (the GPU
+    runs such a pass on the int16 lanes)."""
+    sc = (3, -2, -5, -1)
+    L = 683  # 3 * 683 = 2049
+    a = "".join(random.Random(3).choice("ACGT") for _ in range(L))
+    want = O.sw_banded(a, a, *sc, band=1)
+    assert want[0] == 2049
+    assert sw_dual_schedule(a, a, 1, *sc, enc="f16") != want
+    assert sw_dual_schedule(a, a, 1, *sc, enc="int16") == want
