@@ -278,3 +278,34 @@ def test_engine_request_ids_after_collection(cgm, serving_pairs):
     assert np.array_equal(s1, s2)
     assert all(g.shape == (4,) for g in got)
     eng.close()
+
+
+@pytest.mark.parametrize("reg_max_band", [64, 32])
+def test_engine_sw_bands_beyond_the_register_lanes(cgm, dev, orc, gen_mod, reg_max_band):
+    """SW requests whose bands reach every SW class (register lanes, LDS
+    lanes above the tuning's reg_max_band, packed lanes at 128, one
+    workgroup per pair beyond 150, one wavefront per pair for reads over
+    200 bp): batches holding them launch those kernels, batches of register
+    bands only leave them out; every result equals the oracle."""
+    w = gen_mod.generate(gen_mod.SW_100_300BP, 3000, seed=41)
+    rng = np.random.default_rng(reg_max_band)
+    choices = np.array([0, 3, 16, 33, 64, 65, 100, 128, 150, 300], np.int32)
+    w.band[:] = choices[rng.integers(0, choices.size, w.n_pairs)]
+    w.band[:1500] = np.minimum(w.band[:1500], 64)  # the first requests: register bands only
+    eng = cgm.ServingEngine(hbm_budget_bytes=8 << 30, n_slots=2, max_batch_pairs=1024,
+                            max_batch_words=1 << 16, max_error_cap=0)
+    eng.set_tuning(0, reg_max_band=reg_max_band)
+    reqs = requests_of(w.n_pairs, rng)
+    got = [np.zeros(w.n_pairs, np.int32) for _ in range(3)]
+    for part in (reqs[: len(reqs) // 2], reqs[len(reqs) // 2:]):
+        ids = [eng.submit_pairs(w, idx) for idx in part]
+        for idx, rid in zip(part, ids):
+            s, q, r, _ = eng.wait(rid, idx.size)
+            got[0][idx], got[1][idx], got[2][idx] = s, q, r
+    print(f"[case] engine SW bands 0-300, reg_max_band {reg_max_band}: {len(reqs)} requests",
+          flush=True)
+    exp = orc.batch(0, w.q_packed, w.r_packed, w.q_off, w.r_off, w.q_len, w.r_len, band=w.band,
+                    threads=8, **CONFIG_SW)
+    assert_same(tuple(got), exp, f"engine SW bands reg_max_band {reg_max_band}")
+    assert eng.stats()["worker_error"] == 0
+    eng.close()
