@@ -160,6 +160,11 @@ void mix_plan_set_hint(MixPlan *P, int32_t max_band, int32_t max_len);
 // The SW persistent kernel's instance with the wavefront-per-pair phase
 // (cgm_mixed_wave.hip), `grid` workgroups on `stream`; tasks: the plan's
 // task records.
+// launch_sw_wave on a plan's workspace: zeroes its counters, runs the
+// wavefront-per-pair kernel and the one-workgroup-per-pair kernel over the
+// pairs that kernel left.
+int mix_enqueue_wave_direct(MixPlan *P, const Batch &b, const int32_t *band, int32_t band_scalar,
+                            SwScoring sc, SwOut out, hipStream_t stream);
 int launch_mix_persistent_wave(unsigned grid, hipStream_t stream, const Batch &b,
                                const int32_t *band, int32_t band_scalar, SwScoring sc, SwOut out,
                                const int32_t *order, const void *tasks, const int32_t *task_order,
@@ -283,10 +288,15 @@ constexpr int kWaveMaxBand = 64;
 constexpr int kWaveMaxLen = 1 << 20;
 int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                    SwOut out, hipStream_t stream);
-// The automatic layout with every SW pair of effective band <= 64 on the
-// wavefront-per-pair kernel (the rest as the automatic layout places them).
+// Every SW pair of effective band <= 64 on one wavefront of its own (pair i
+// on wavefront i, no sort), the rest on the one-workgroup-per-pair kernel.
 int launch_sw_wave(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                    SwOut out, hipStream_t stream);
+// The wavefront-per-pair kernel of launch_sw_wave (cgm_mixed_wave.hip): pairs
+// it does not take are appended to rest[], their count added to
+// rest_range[1] (both zeroed by the caller).
+int launch_sw_wave_direct(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
+                          SwOut out, int32_t *rest, int32_t *rest_range, hipStream_t stream);
 int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                       SwOut out, hipStream_t stream);
 // The launches of the band-bucket layout after classification (order and

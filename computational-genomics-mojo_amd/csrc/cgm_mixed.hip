@@ -618,6 +618,18 @@ int mix_enqueue(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_sc
     return forked ? join_streams(stream, 2, aux) : CGM_OK;
 }
 
+int mix_enqueue_wave_direct(MixPlan *P, const Batch &b, const int32_t *band, int32_t band_scalar,
+                            SwScoring sc, SwOut out, hipStream_t stream) {
+    if (b.n_pairs > P->cap) return set_error(CGM_ERR_INVALID, "batch larger than the plan");
+    if (b.n_pairs == 0) return CGM_OK;
+    int32_t *range = P->small + kSmallRanges + 2 * kMixSwPlain;
+    hipLaunchKernelGGL(mix_zero_kernel, dim3(1), dim3(256), 0, stream, P->small, kSmallWords);
+    if (int rc = launch_sw_wave_direct(b, band, band_scalar, sc, out, P->order, range, stream))
+        return rc;
+    return launch_sw_plain_range(b, band, band_scalar, sc, out, P->order, range, P->cus * 2,
+                                 stream);
+}
+
 int mix_capture(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_scalar,
                 const int32_t *band, int32_t band_scalar, const int32_t *me, int32_t me_scalar,
                 SwScoring sc, WfaPen pen, SwOut out, hipStream_t stream) {

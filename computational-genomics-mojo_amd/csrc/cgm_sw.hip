@@ -363,9 +363,10 @@ int launch_sw_buckets(const Batch &b, const int32_t *band, int32_t band_scalar, 
 // Automatic layout: the persistent task-queue kernel of the mixed path with
 // every pair an SW pair, on a plan cached per (device, stream) and grown
 // when a larger batch arrives.  A batch of at most kWaveBatchPerCu pairs per
-// CU (two rounds of the kernel's 8 wavefronts per CU) runs one wavefront per
-// pair (bands <= 64): one pair per lane would leave most SIMDs idle and
-// every pair's whole band area on one lane's serial chain.
+// CU runs one wavefront per pair (bands <= 64), pair i on wavefront i: one
+// pair per lane would leave most SIMDs idle and every pair's whole band area
+// on one lane's serial chain, and with about as many pairs as wavefronts a
+// longest-first order has nothing to balance.
 constexpr int64_t kWaveBatchPerCu = 16;
 int launch_sw_auto(const Batch &b, const int32_t *band, int32_t band_scalar, SwScoring sc,
                    SwOut out, hipStream_t stream) {
@@ -388,16 +389,7 @@ int launch_sw_wave(const Batch &b, const int32_t *band, int32_t band_scalar, SwS
     if (b.n_pairs == 0) return CGM_OK;
     const WfaPen pen = {4, 6, 2};  // unused: the batch holds no WFA pair
     return with_cached_plan(stream, b.n_pairs, 0, pen, [&](MixPlan *P) {
-        // every non-empty pair counts as long for this call (the plan's
-        // tuning is restored before the lock is released)
-        const MixTuning saved = mix_plan_tuning(P);
-        MixTuning t = saved;
-        t.lane_max_len = 0;
-        mix_plan_set_tuning(P, t);
-        const int rc =
-            mix_enqueue(P, b, nullptr, 0, band, band_scalar, nullptr, 0, sc, pen, out, stream);
-        mix_plan_set_tuning(P, saved);
-        return rc;
+        return mix_enqueue_wave_direct(P, b, band, band_scalar, sc, out, stream);
     });
 }
 
