@@ -145,9 +145,10 @@ def wfa_extra(m, dev, gen, n_pairs=20_000, cpu=True):
     cfg = m.AlignConfig(SW["match"], SW["mismatch"], SW["gap_open"], SW["gap_extend"], 0)
     plan = m.MixedPlan(n, int(w.max_error.max()))
     pout = m.SwResult(n)
-    tuning, best_ms, start_ms = plan.autotune(dev, bufs, n, up["kind"], up["band"],
-                                              up["max_error"], cfg, pout, reps=2)
-    wall, kms = time_steps(m, dev, lambda: plan.align(dev, bufs, n, up["kind"], up["band"],
+    # every pair is a WFA pair: the plan takes the kind as one value
+    tuning, best_ms, start_ms = plan.autotune(dev, bufs, n, 1, up["band"], up["max_error"], cfg,
+                                              pout, reps=2)
+    wall, kms = time_steps(m, dev, lambda: plan.align(dev, bufs, n, 1, up["band"],
                                                       up["max_error"], cfg, pout), 3, 1)
     costs["persistent"] = dev.download(pout.scores, n)
     res["persistent"] = {"pairs_per_s": n * 3 / wall, "ms_per_step": wall / 3 * 1e3,
@@ -156,7 +157,7 @@ def wfa_extra(m, dev, gen, n_pairs=20_000, cpu=True):
     # the WFA kernels' own span, from the plan's events
     wfa_ms = []
     for _ in range(3):
-        plan.align(dev, bufs, n, up["kind"], up["band"], up["max_error"], cfg, pout)
+        plan.align(dev, bufs, n, 1, up["band"], up["max_error"], cfg, pout)
         wfa_ms.append(plan.wfa_kernel_ms())
     kms_wfa = float(np.median(wfa_ms))
     log(f"[bench] wfa persistent (autotuned): {res['persistent']}, WFA kernels {kms_wfa:.3f} ms")

@@ -349,13 +349,16 @@ def batch_arrays(queries: SequenceBatch, refs: SequenceBatch):
 
 
 def _mixed_args(bufs, n, kind, band, max_error, cfg: AlignConfig, out: SwResult, pen):
+    """kind: a device array of per-pair kinds, or an int (0: every pair SW,
+    1: every pair WFA; the WFA-only form launches no SW kernel)."""
     q, r, ql, rl, qo, ro = bufs
     x, o, e = pen
     ptr = lambda a: a.dptr if a is not None else None  # noqa: E731
+    kind_arr, kind_scalar = (None, int(kind)) if isinstance(kind, int) else (kind, 0)
     return (q.dptr, r.dptr, ql.dptr, rl.dptr, qo.dptr, ro.dptr, n, q.count, r.count,
-            ptr(kind), 0, ptr(band), cfg.band_width, ptr(max_error), 0, cfg.match_score,
-            cfg.mismatch, cfg.gap_open, cfg.gap_extend, x, o, e, out.scores.dptr,
-            out.query_end.dptr, out.ref_end.dptr)
+            ptr(kind_arr), kind_scalar, ptr(band), cfg.band_width, ptr(max_error), 0,
+            cfg.match_score, cfg.mismatch, cfg.gap_open, cfg.gap_extend, x, o, e,
+            out.scores.dptr, out.query_end.dptr, out.ref_end.dptr)
 
 
 class MixedPlan:
@@ -454,10 +457,12 @@ class MixedPlan:
         import ctypes
         q, r, ql, rl, qo, ro = bufs
         ptr = lambda v: v.dptr if v is not None else None  # noqa: E731
+        kind_arr, kind_scalar = (None, int(kind)) if isinstance(kind, int) else (kind, 0)
         buf = ctypes.create_string_buffer(4096)
         x, o, e = self.pen
         _lib.call("cgm_plan_profile_key", self.handle, q.dptr, r.dptr, ql.dptr, rl.dptr, qo.dptr,
-                  ro.dptr, n, q.count, r.count, ptr(kind), 0, ptr(band), cfg.band_width,
+                  ro.dptr, n, q.count, r.count, ptr(kind_arr), kind_scalar, ptr(band),
+                  cfg.band_width,
                   cfg.match_score, cfg.mismatch, cfg.gap_open, cfg.gap_extend, x, o, e, buf,
                   len(buf), device.stream)
         return buf.value.decode()

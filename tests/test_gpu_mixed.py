@@ -228,3 +228,22 @@ def test_mixed_autotune_cached_by_profile(cgm, dev, orc, gen_mod, tmp_path):
                                                            out3.ref_end)),
                 sw_oracle(orc, sw, CONFIG_SW), "SW-only profile")
     plan3.close()
+
+
+def test_mixed_kind_as_one_value(cgm, dev, orc, gen_mod):
+    """kind given as one value for the whole batch (0: every pair SW, 1:
+    every pair WFA, the form that launches no SW kernel) equals the per-pair
+    kind array on the SW and on the WFA pairs of a mixed sample."""
+    w = mixed_sample(gen_mod, seed=29)
+    exp = oracle_mixed(orc, w)
+    for k in (0, 1):
+        sub = w.subset(np.nonzero(w.kind == k)[0])
+        bufs, arrs = upload_mixed(dev, sub)
+        plan = cgm.MixedPlan(sub.n_pairs, int(sub.max_error.max()))
+        print(f"[case] kind {k} as one value, {sub.n_pairs} pairs", flush=True)
+        by_array = run_plan(cgm, dev, plan, bufs, arrs, sub.n_pairs)
+        by_value = run_plan(cgm, dev, plan, bufs, (k, arrs[1], arrs[2]), sub.n_pairs)
+        assert_same(by_value, by_array, f"kind {k} value vs array")
+        idx = np.nonzero(w.kind == k)[0]
+        assert_same(by_value, tuple(a[idx] for a in exp), f"kind {k} vs oracle")
+        plan.close()
