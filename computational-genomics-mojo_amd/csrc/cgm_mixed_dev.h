@@ -7,6 +7,7 @@
 
 #include "cgm_sw_dev.h"
 #include "cgm_sw_wave_dev.h"
+#include "cgm_wfa_reg.h"
 
 namespace cgm {
 
@@ -38,9 +39,9 @@ constexpr int kSmallBandTask = 32 + 72;      // first task of each band, widest 
 constexpr int kSmallBandPair = 32 + 2 * 72;  // first pair (in the class) of each band [kRegBands]
 constexpr int kSmallUnits = 28;              // SW kernel: long/short unit counters (64 bit)
 constexpr int kSmallWaveQueue = 30;          // SW wavefront-per-pair kernel: pair counter
-constexpr int kSmallWfaCounts = 256;         // WFA pairs per layout class [8]
-constexpr int kSmallWfaRanges = 264;         // [start, end) per WFA layout class [16]
-constexpr int kSmallWfaQueues = 280;         // WFA class kernels' counters [8]
+constexpr int kSmallWfaCounts = 256;         // WFA pairs per layout class [kWfaClasses]
+constexpr int kSmallWfaRanges = kSmallWfaCounts + wfareg::kWfaClasses;  // [start, end) per class
+constexpr int kSmallWfaQueues = kSmallWfaRanges + 2 * wfareg::kWfaClasses;  // class kernels' counters
 constexpr int kSmallSimdSlots = 512;         // wavefronts started per SIMD [kSimdKeys]
 constexpr int kSimdKeys = 8 << 10;           // XCC (3 bits), CU bits of HW_ID (8), SIMD (2)
 constexpr int kSmallWords = kSmallSimdSlots + kSimdKeys;

@@ -385,9 +385,10 @@ int launch_wfa_classes(const Batch &b, const int32_t *max_error, int32_t max_err
                            gm.g, out, order, ranges + 2 * (C), queues + (C));                    \
     } while (0)
     if (reg && wfa_reg_penalties(p)) {
+        CGM_REG_CLASS(6, s_long);
         CGM_REG_CLASS(5, s_long);
         CGM_REG_CLASS(4, s_long);
-        CGM_REG_CLASS(3, s_long);
+        CGM_REG_CLASS(3, s_short);
         CGM_REG_CLASS(2, s_short);
         CGM_REG_CLASS(1, s_short);
         CGM_REG_CLASS(0, s_short);
@@ -425,25 +426,26 @@ int launch_wfa_wave(const Batch &b, const int32_t *max_error, int32_t max_error_
     int64_t grid = (int64_t)cus * kWfaBlocksPerCu;
     // keep the HBM slabs within 4 GiB
     while (grid > 1 && grid * slab_words * 4 > (int64_t(4) << 30)) grid /= 2;
-    // small: queue counters [0, 8), class counts [8, 16), class ranges [16, 32)
+    // small: queue counters [0, C), class counts [C, 2C), class ranges [2C, 4C)
+    constexpr int C = wfareg::kWfaClasses;
     int32_t *slab = nullptr, *small = nullptr, *order = nullptr, *idx = nullptr;
     uint32_t *keys = nullptr;
-    CGM_HIP(hipMallocAsync((void **)&small, 32 * 4, stream));
+    CGM_HIP(hipMallocAsync((void **)&small, 4 * C * 4, stream));
     CGM_HIP(hipMallocAsync((void **)&slab, (size_t)(grid * slab_words * 4), stream));
     CGM_HIP(hipMallocAsync((void **)&order, (size_t)b.n_pairs * 4, stream));
     CGM_HIP(hipMallocAsync((void **)&keys, (size_t)b.n_pairs * 4, stream));
     CGM_HIP(hipMallocAsync((void **)&idx, (size_t)b.n_pairs * 4, stream));
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, small, 32);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, small, 4 * C);
     // every class longest first (a late long pair would leave the GPU idle)
     int rc = launch_wfa_classify(b, max_error, max_error_scalar, me_max, p, reg, out, keys, idx,
-                                 small + 8, stream);
+                                 small + C, stream);
     if (!rc) rc = sort_desc_by_key(keys, idx, order, b.n_pairs, stream);
-    if (!rc) rc = launch_wfa_class_ranges(small + 8, nullptr, small + 16, stream);
+    if (!rc) rc = launch_wfa_class_ranges(small + C, nullptr, small + 2 * C, stream);
     hipStream_t aux[2];
     if (!rc) rc = fork_streams(stream, 2, aux);
     if (!rc) {
         rc = launch_wfa_classes(b, max_error, max_error_scalar, me_max, p, reg, out, order,
-                                small + 16, small, slab, slab_words, (int)grid, aux[0], aux[1]);
+                                small + 2 * C, small, slab, slab_words, (int)grid, aux[0], aux[1]);
         int rc2 = join_streams(stream, 2, aux);
         if (!rc) rc = rc2;
     }

@@ -107,20 +107,24 @@ __host__ __device__ inline int reg_blocks(int qlen, int rlen, int me, int o, int
     return (hi - lo + 64) / 64;
 }
 
-// Layout classes of a WFA pair, used by every WFA launcher: 0..5 the
+// Layout classes of a WFA pair, used by every WFA launcher: 0..6 the
 // register layout with (NB blocks per wavefront, NW wavefronts) =
-// (2, 1), (4, 1), (8, 1), (8, 4), (8, 8), (8, 16), each within an LDS
-// budget; 6 the 512-thread team layout (52 KB, tiled over an HBM slab
-// beyond it), 7 the 256-thread team layout (int16 rows within 20 KB).
-constexpr int kWfaClasses = 8;
-constexpr int kWfaRegClasses = 6;
-constexpr int kWfaTeamBig = 6, kWfaTeamSmall = 7;
+// (2, 1), (4, 1), (8, 1), (8, 2), (8, 4), (8, 8), (8, 16), each within an
+// LDS budget (a pair of 9-16 blocks on two wavefronts rather than four: half
+// the wavefronts meet at each score's barrier, and the workgroup's block
+// slots are mostly in use); 7 the 512-thread team layout (52 KB, tiled over
+// an HBM slab beyond it), 8 the 256-thread team layout (int16 rows within
+// 20 KB).
+constexpr int kWfaClasses = 9;
+constexpr int kWfaRegClasses = 7;
+constexpr int kWfaTeamBig = 7, kWfaTeamSmall = 8;
 __host__ __device__ constexpr int reg_class_nb(int c) { return c == 0 ? 2 : c == 1 ? 4 : 8; }
 __host__ __device__ constexpr int reg_class_nw(int c) {
-    return c <= 2 ? 1 : c == 3 ? 4 : c == 4 ? 8 : 16;
+    return c <= 2 ? 1 : c == 3 ? 2 : c == 4 ? 4 : c == 5 ? 8 : 16;
 }
 __host__ __device__ constexpr int reg_class_lds(int c) {
-    return c == 0 ? 4096 : c == 1 ? 6144 : c == 2 ? 8192 : c == 3 ? 16384 : c == 4 ? 24576 : 32768;
+    return c == 0 ? 4096 : c == 1 ? 6144 : c == 2 ? 8192 : c == 3 ? 12288 : c == 4 ? 16384
+         : c == 5 ? 24576 : 32768;
 }
 // waves per SIMD each class's registers allow (kept by amdgpu_waves_per_eu)
 __host__ __device__ constexpr int reg_class_wpe(int c) { return c == 0 ? 8 : c == 1 ? 6 : 4; }
@@ -139,9 +143,10 @@ __device__ __forceinline__ int wfa_pair_class(int qlen, int rlen, int me, const 
 }
 
 // Sort key of a WFA pair: its class (first classes first in a descending
-// sort), then its work (longest first); never 0, the key of excluded pairs.
+// sort; bits 24-27, below the mixed path's own class bits), then its work
+// (longest first, 24 bits); never 0, the key of excluded pairs.
 __device__ __forceinline__ uint32_t wfa_class_key(int cls, uint32_t work) {
-    return ((uint32_t)(kWfaClasses - 1 - cls) << 25) | (min(work, (uint32_t)((1 << 25) - 2)) + 1);
+    return ((uint32_t)(kWfaClasses - 1 - cls) << 24) | (min(work, (uint32_t)((1 << 24) - 2)) + 1);
 }
 
 // Bases pos .. pos + 15 of an LSB-first sequence as one word (base pos + i
