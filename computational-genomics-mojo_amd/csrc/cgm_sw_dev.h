@@ -95,6 +95,67 @@ struct AlignedWords {
     }
 };
 
+// Reference words as two 1-bit planes per 32-base word, LSB-first: bit u
+// of lo / hi is the low / high bit of base u of the word.  A 64-bit word
+// holds base u at bits 63-2u (high) and 62-2u (low); reversed, at bits 2u
+// (high) and 2u + 1 (low), so the planes are its even and odd bits.
+__device__ __forceinline__ uint32_t compress_even64(uint64_t x) {
+    x &= 0x5555555555555555ull;
+    x = (x | (x >> 1)) & 0x3333333333333333ull;
+    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+    return (uint32_t)x;
+}
+template <int N>
+struct AlignedPlanes {
+    uint32_t lo[N], hi[N];
+    __device__ __forceinline__ void set(int c, uint64_t w) {
+        const uint64_t v = __builtin_bitreverse64(w);
+        hi[c] = compress_even64(v);
+        lo[c] = compress_even64(v >> 1);
+    }
+    __device__ __forceinline__ void load(const uint64_t *r, int w0, int rwords) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) set(c, (w0 + c >= 0 && w0 + c < rwords) ? r[w0 + c] : 0ull);
+    }
+    __device__ __forceinline__ void advance(const uint64_t *r, int w0_new, int rwords) {
+#pragma unroll
+        for (int c = 0; c + 1 < N; ++c) {
+            lo[c] = lo[c + 1];
+            hi[c] = hi[c + 1];
+        }
+        const int w = w0_new + N - 1;
+        set(N - 1, (w >= 0 && w < rwords) ? r[w] : 0ull);
+    }
+    // bases o .. o + 31 from word c on (bit i: base o + i), o in [0, 32)
+    __device__ __forceinline__ void window(int c, int o, uint32_t &wl, uint32_t &wh) const {
+        wl = __builtin_amdgcn_alignbit(lo[c + 1], lo[c], (uint32_t)o);
+        wh = __builtin_amdgcn_alignbit(hi[c + 1], hi[c], (uint32_t)o);
+    }
+};
+
+// Bits [lo, hi) of a 32-bit mask (empty when lo >= hi).
+__device__ __forceinline__ uint32_t range_mask32(int lo, int hi) {
+    lo = min(max(lo, 0), 32);
+    hi = min(max(hi, 0), 32);
+    const uint32_t m1 = lo == 32 ? 0u : (~0u << lo);
+    const uint32_t m2 = hi == 32 ? ~0u : ((1u << hi) - 1u);
+    return m1 & m2;
+}
+
+// Match mask of a 32-base plane window: bit i set when base i equals the
+// row's query base (qlo, qhi: its bits replicated) and lies inside
+// [0, rlen) on a valid row (rmin as in chunk_mask).
+__device__ __forceinline__ uint32_t plane_mask(uint32_t wl, uint32_t wh, uint32_t qlo,
+                                               uint32_t qhi, int rs, int rlen, bool row_ok,
+                                               int rmin) {
+    uint32_t eq = ~((wl ^ qlo) | (wh ^ qhi));
+    if (rs < 0 || rs + 32 > rmin) eq &= range_mask32(-rs, rlen - rs);
+    return row_ok ? eq : 0u;
+}
+
 // Match mask of one 32-base chunk: bit 62-2u set when base u of the chunk
 // equals the row's query base and lies inside [0, rlen) on a valid row.
 // rmin (wave-uniform) is the shortest reference among the wavefront's lanes
@@ -297,19 +358,22 @@ __device__ __forceinline__ LaneResult sw_reg_lane_packed(const uint64_t *q, int 
     return {best, bq, br};
 }
 
-// ------------------------------------------------ dual-row int16 lanes
+// ------------------------------------------------ dual-row 16-bit lanes
 // The band frame of W = 2B + 1 cells is cut into a left half (j in [0, S))
 // and a right half (j in [S, 2S)), S = B + 1, and both halves are swept in
 // one pass over the row registers: register u holds cell u of the left half
 // in its low 16 bits and cell S + u of the right half in its high 16 bits,
 // the right half one row behind the left one.  Step t computes row t of the
-// left half and row t - 1 of the right half with packed int16 instructions
-// (v_pk_add_u16 / v_pk_max_i16: two cells per instruction).  The right
+// left half and row t - 1 of the right half with packed 16-bit instructions
+// (two cells per instruction; the encodings are DualArith below).  The right
 // half's first cell reads its left neighbour (row t - 1, cell B) from the
 // left half's previous step; the left half's last cell reads its upper
 // neighbour (row t - 1, cell B + 1) from the right half's first cell of this
 // step.  Cell 2S - 1 = W lies past the frame: it is forced to H = 0 so that
-// cell W - 1 reads the zero upper neighbour the frame edge gives.
+// cell W - 1 reads the zero upper neighbour the frame edge gives.  Each
+// row's match masks come from the reference held as 1-bit planes
+// (AlignedPlanes): per half and 32-cell chunk two funnel shifts per plane
+// and one compare against the query base's bits, one mask bit per cell.
 //
 // Exactness: the per-cell recurrence, the forced mismatches outside the
 // matrix and the first-cell boundary values are those of sw_reg_lane, in
@@ -330,12 +394,11 @@ __device__ __forceinline__ bool dual_ok(const SwScoring &sc, int qlen, int rlen)
 }
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
-// Sign-extended bit (30 - 2 * (u % 16)) of a match-mask half (0 / -1).  The
-// bit offset is taken from `base30` (30, unknown to the compiler), so the
-// extraction stays one v_bfe_i32 instead of being turned into a compare and
-// select.
-__device__ __forceinline__ uint32_t bit_mask_at(uint32_t half, int u, int base30) {
-    return (uint32_t)__builtin_amdgcn_sbfe((int)half, base30 - 2 * (u & 15), 1);
+// Sign-extended bit v of a plane match mask (0 / -1).  The bit offset is
+// taken from `base0` (0, unknown to the compiler), so the extraction stays
+// one v_bfe_i32 instead of being turned into a compare and select.
+__device__ __forceinline__ uint32_t bit_mask_at(uint32_t m, int v, int base0) {
+    return (uint32_t)__builtin_amdgcn_sbfe((int)m, base0 + v, 1);
 }
 __device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t a, uint32_t b) {
     return (a & m) | (b & ~m);
@@ -423,8 +486,8 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
     constexpr uint32_t kNegE16 = T::kNegE;  // -16384
     const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
     const int rmin = wave_min_ref(qlen, rlen);
-    int base30;
-    asm volatile("s_mov_b32 %0, 30" : "=s"(base30));
+    int base0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(base0));
     V P[S], SV[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) {
@@ -435,7 +498,7 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
     uint64_t qwin = 0;
     int qb_prev = 0;
     uint32_t el_carry = kNegE16;  // E of the left half's last cell, previous step
-    AlignedWords<NCH + 1> rwl, rwh;
+    AlignedPlanes<NCH + 1> rwl, rwh;
     rwl.load(r, (-B) >> 5, rwords);
     rwh.load(r, 0, rwords);
     uint32_t p0_hi = OE >> 16;  // right half's first cell, G of its latest row
@@ -446,13 +509,11 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
         if ((t & 31) == 0) qwin = ((t >> 5) < qwords) ? q[t >> 5] : 0ull;
         const int qb = (int)(qwin >> 62);
         qwin <<= 2;
-        const uint64_t qrep_lo = (uint64_t)qb * 0x5555555555555555ull;
-        const uint64_t qrep_hi = (uint64_t)qb_prev * 0x5555555555555555ull;
+        const uint32_t ql_lo = (qb & 1) ? ~0u : 0u, qh_lo = (qb & 2) ? ~0u : 0u;
+        const uint32_t ql_hi = (qb_prev & 1) ? ~0u : 0u, qh_hi = (qb_prev & 2) ? ~0u : 0u;
         qb_prev = qb;
         const bool ok_lo = t < qlen, ok_hi = t >= 1 && t - 1 < qlen;
-        const int sh_lo = 2 * (rs_lo & 31), sh_hi = 2 * (rs_hi & 31);
-        // left neighbours of the two first cells: H = 0 and the E boundary
-        // value for the left half; cell B of row t - 1 for the right half
+        const int o_lo = rs_lo & 31, o_hi = rs_hi & 31;
         // left neighbours of the two first cells: H = 0 and the E boundary
         // value for the left half; cell B of row t - 1 for the right half
         V Gl = dual_bits<V>((dual_u32(P[S - 1]) << 16) | (OE & 0xffffu));
@@ -460,19 +521,19 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
         V rowmax = dual_bits<V>(0u), hprev = dual_bits<V>(0u);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-            const uint64_t ml =
-                chunk_mask(rwl.window(c, sh_lo), qrep_lo, rs_lo + 32 * c, rlen, ok_lo, rmin);
-            const uint64_t mh =
-                chunk_mask(rwh.window(c, sh_hi), qrep_hi, rs_hi + 32 * c, rlen, ok_hi, rmin);
-            const uint32_t ml1 = (uint32_t)(ml >> 32), ml0 = (uint32_t)ml;
-            const uint32_t mh1 = (uint32_t)(mh >> 32), mh0 = (uint32_t)mh;
+            uint32_t al, ah, bl, bh;
+            rwl.window(c, o_lo, al, ah);
+            rwh.window(c, o_hi, bl, bh);
+            const uint32_t ml = plane_mask(al, ah, ql_lo, qh_lo, rs_lo + 32 * c, rlen, ok_lo, rmin);
+            const uint32_t mh = plane_mask(bl, bh, ql_hi, qh_hi, rs_hi + 32 * c, rlen, ok_hi, rmin);
 #pragma unroll
             for (int v = 0; v < 32; ++v) {
                 const int u = 32 * c + v;
                 if (u < S) {
-                    const uint32_t bl = bit_mask_at(v < 16 ? ml1 : ml0, v, base30);
-                    const uint32_t bh = bit_mask_at(v < 16 ? mh1 : mh0, v, base30);
-                    const uint32_t sel = bit_select(bit_select(0xffff0000u, bh, bl), SUBM, SUBX);
+                    const uint32_t ml_v = bit_mask_at(ml, v, base0);
+                    const uint32_t mh_v = bit_mask_at(mh, v, base0);
+                    const uint32_t sel =
+                        bit_select(bit_select(0xffff0000u, mh_v, ml_v), SUBM, SUBX);
                     // upper neighbour: register u + 1 of the previous step;
                     // for the last cell, the right half's first cell of this
                     // step (low) and H = 0 past the frame (high)
