@@ -51,11 +51,21 @@ __device__ __forceinline__ void nw_write_bad(const SwOut &o, int64_t i) {
     o.r_end[i] = -1;
 }
 
+// Sign-extended bit u of a 32-bit mask (0 / -1) masked to `val`, one
+// v_bfe_i32 with the offset in an SGPR (a constant offset would let the
+// compiler turn the extraction into a compare and select).
+__device__ __forceinline__ int mask_bit_sel(uint32_t m, int u, int val) {
+    int v;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(v) : "v"(m), "s"(u));
+    return v & val;
+}
+
 // ---------------------------------------------------------------------------
 // Register lanes: one pair per lane, row-major like the reference, the row
 // kept as G = H + (go + ge) in VGPRs.  Per cell: the substitution from the
-// 32-base match mask, max3 over the three predecessors, and the in-matrix
-// mask that forces -1000000 on cells the reference skips.  A lane whose
+// row's 32-base match mask (the reference held as 1-bit planes, one bit per
+// base), max3 over the three predecessors, and the in-matrix mask that
+// forces -1000000 on cells the reference skips.  A lane whose
 // query is exhausted skips the remaining rows of its wavefront (divergent
 // branch), so its row still holds row qlen - 1 at the end.
 // ---------------------------------------------------------------------------
@@ -76,31 +86,31 @@ __device__ __forceinline__ int nw_reg_lane(const uint64_t *q, int qlen, const ui
         G[j] = (ri < 0 ? kNwNeg : ri == 0 ? 0 : sc.gap_open + ri * sc.gap_extend) + gap;
     }
     uint64_t qwin = 0;
-    swdev::AlignedWords<NCH + 1> rw;
+    swdev::AlignedPlanes<NCH + 1> rw;
     rw.load(r, (-B) >> 5, rwords);
     for (int qi = 0; qi < rows; ++qi) {
         const int rs0 = qi - B;
         if (qi > 0 && (rs0 & 31) == 0) rw.advance(r, rs0 >> 5, rwords);
         if ((qi & 31) == 0) qwin = ((qi >> 5) < qwords) ? q[qi >> 5] : 0ull;
-        const uint64_t qrep = (qwin >> 62) * 0x5555555555555555ull;
+        const int qb = (int)(qwin >> 62);
         qwin <<= 2;
-        const int sh = 2 * (rs0 & 31);
+        const uint32_t qlo = (qb & 1) ? ~0u : 0u, qhi = (qb & 2) ? ~0u : 0u;
+        const int o = rs0 & 31;
         if (qi < qlen) {
             int Gl = negG;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const int rs = rs0 + 32 * c;
-                const uint64_t range = swdev::base_range_mask(-rs, rlen - rs);
-                const uint64_t x = rw.window(c, sh) ^ qrep;
-                const uint64_t m = ~(x | (x >> 1)) & range;
-                const uint32_t mh = (uint32_t)(m >> 32), ml = (uint32_t)m;
-                const uint32_t gh = (uint32_t)(range >> 32), gl = (uint32_t)range;
+                const uint32_t range = swdev::range_mask32(-rs, rlen - rs);
+                uint32_t wl, wh;
+                rw.window(c, o, wl, wh);
+                const uint32_t m = ~((wl ^ qlo) | (wh ^ qhi)) & range;
 #pragma unroll
                 for (int u = 0; u < 32; ++u) {
                     const int j = 32 * c + u;
                     if (j < W) {
-                        const int sel = swdev::match_sel(u < 16 ? mh : ml, u, diff);
-                        const int inside = swdev::match_sel(u < 16 ? gh : gl, u, -1);
+                        const int sel = mask_bit_sel(m, u, diff);
+                        const int inside = mask_bit_sel(range, u, -1);
                         const int Gu = (j + 1 < W) ? G[j + 1] : negG;
                         const int dterm = G[j] + cmis + sel;
                         // ordered (volatile) max3: otherwise the substitution and
