@@ -394,12 +394,7 @@ __device__ __forceinline__ bool dual_ok(const SwScoring &sc, int qlen, int rlen)
 }
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
-// Sign-extended bit v of a plane match mask (0 / -1).  The bit offset is
-// taken from `base0` (0, unknown to the compiler), so the extraction stays
-// one v_bfe_i32 instead of being turned into a compare and select.
-__device__ __forceinline__ uint32_t bit_mask_at(uint32_t m, int v, int base0) {
-    return (uint32_t)__builtin_amdgcn_sbfe((int)m, base0 + v, 1);
-}
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t a, uint32_t b) {
     return (a & m) | (b & ~m);
 }
@@ -481,13 +476,13 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
     const int oe = sc.gap_open + sc.gap_extend;
     const uint32_t OE = T::enc(oe) * 0x10001u;
     const uint32_t EXT = T::enc(sc.gap_extend) * 0x10001u;
-    const uint32_t SUBM = T::enc(sc.match - oe) * 0x10001u;
+    // the diagonal term's addend: SUBX (mismatch - oe) or SUBM (match - oe),
+    // selected per half as SUBX + bit * DSUB on the 16-bit patterns
     const uint32_t SUBX = T::enc(sc.mismatch - oe) * 0x10001u;
+    const uint32_t DSUB = ((T::enc(sc.match - oe) - T::enc(sc.mismatch - oe)) & 0xffffu) * 0x10001u;
     constexpr uint32_t kNegE16 = T::kNegE;  // -16384
     const int qwords = (qlen + 31) >> 5, rwords = (rlen + 31) >> 5;
     const int rmin = wave_min_ref(qlen, rlen);
-    int base0;
-    asm volatile("s_mov_b32 %0, 0" : "=s"(base0));
     V P[S], SV[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) {
@@ -526,14 +521,22 @@ __device__ __forceinline__ LaneResult sw_dual_lane(const uint64_t *q, int qlen, 
             rwh.window(c, o_hi, bl, bh);
             const uint32_t ml = plane_mask(al, ah, ql_lo, qh_lo, rs_lo + 32 * c, rlen, ok_lo, rmin);
             const uint32_t mh = plane_mask(bl, bh, ql_hi, qh_hi, rs_hi + 32 * c, rlen, ok_hi, rmin);
+            // cells 0-15 / 16-31 of the chunk: the left half's bits in the low
+            // 16 bits, the right half's in the high 16 bits
+            const uint32_t x0 = __builtin_amdgcn_perm(mh, ml, 0x05040100u);
+            const uint32_t x1 = __builtin_amdgcn_perm(mh, ml, 0x07060302u);
 #pragma unroll
             for (int v = 0; v < 32; ++v) {
                 const int u = 32 * c + v;
                 if (u < S) {
-                    const uint32_t ml_v = bit_mask_at(ml, v, base0);
-                    const uint32_t mh_v = bit_mask_at(mh, v, base0);
-                    const uint32_t sel =
-                        bit_select(bit_select(0xffff0000u, mh_v, ml_v), SUBM, SUBX);
+                    // each half's bit v % 16 as 0 / 1, then SUBX + bit (SUBM - SUBX)
+                    // on the halves' 16-bit patterns (exact modulo 2^16, for both
+                    // encodings): one shift, one and, one v_pk_mad_u16
+                    const uint32_t b01 = ((v < 16 ? x0 : x1) >> (v & 15)) & 0x00010001u;
+                    const u16x2 bm = __builtin_bit_cast(u16x2, b01);
+                    const u16x2 sv = bm * __builtin_bit_cast(u16x2, DSUB) +
+                                     __builtin_bit_cast(u16x2, SUBX);
+                    const uint32_t sel = __builtin_bit_cast(uint32_t, sv);
                     // upper neighbour: register u + 1 of the previous step;
                     // for the last cell, the right half's first cell of this
                     // step (low) and H = 0 past the frame (high)
