@@ -737,7 +737,7 @@ int mix_autotune(MixPlan *P, const Batch &b, const int32_t *kind, int32_t kind_s
             if (x != cur.lane_max_len) (c[nc] = cur, c[nc++].lane_max_len = x);
         try_all(c, nc);
     }
-    {   // register SW lanes: dual-row int16 or int32
+    {   // register SW lanes: dual-row 16-bit (half precision or int16) or int32
         MixTuning c = cur;
         c.sw_dual = 1 - cur.sw_dual;
         try_all(&c, 1);
